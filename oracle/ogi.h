@@ -1,0 +1,101 @@
+// ORACLE — test infrastructure only (see ovec.h).
+// Restates src/surfel.rs, src/formfactor.rs, src/solve.rs and the frame
+// pipeline of src/main.rs (Cam, render_frame) and src/film.rs.
+#pragma once
+#include <atomic>
+#include <functional>
+#include <thread>
+#include <vector>
+
+#include "olight.h"
+
+namespace orc {
+
+// src/surfel.rs:22-38
+struct Surfel {
+    V3 p, n, albedo;
+    float area;
+    V3 direct, irr, rad;
+};
+
+// Order-preserving parallel loop: each index writes only its own slot, so
+// the result is identical for any thread count (the reference's
+// par_iter().collect()).
+void par_for(size_t n, int threads, const std::function<void(size_t)>& f);
+
+// src/surfel.rs:61-141
+struct Grid {
+    V3 lo;
+    float inv_cell;
+    int32_t dim[3];
+    std::vector<uint32_t> starts, items;
+    void build(const std::vector<V3>& points, V3 lo, V3 hi, float cell);
+    void cell_of(V3 p, int32_t c[3]) const;
+};
+
+// src/surfel.rs:147-255
+std::vector<Surfel> generate(const Scene& sc, size_t target);
+// src/surfel.rs:262-293
+V3 gather(const std::vector<Surfel>& s, const Grid& g, V3 p, V3 n, float radius);
+
+// src/formfactor.rs:31-39
+struct Cluster {
+    V3 p, n;
+    float area, radius;
+    std::vector<uint32_t> members;
+};
+
+// src/formfactor.rs:48-66
+struct Transfer {
+    std::vector<Cluster> clusters;
+    float mean_row_sum = 0;
+    std::vector<uint32_t> starts, idx;
+    std::vector<float> w;
+    size_t nc = 0, links = 0;
+    // Per-row closure sky fraction and final row sum (diagnostics only).
+    std::vector<float> sky, row_sum;
+};
+
+int normal_bucket(V3 n);
+std::vector<Cluster> build_clusters(const Scene& sc, const std::vector<Surfel>& s, int32_t res);
+// src/formfactor.rs:153-277.  Rows [row_begin, row_end) only when a sample
+// is asked for (CPU-baseline timing); the full matrix otherwise.
+uint64_t build_transfer(const Scene& sc, const std::vector<Surfel>& s, std::vector<Cluster> clusters,
+                        Transfer& out, int threads, size_t row_begin = 0, size_t row_end = SIZE_MAX);
+
+// src/solve.rs:27-108
+uint64_t light_surfels(const Scene& sc, std::vector<Surfel>& s, int threads);
+struct SolveStats {
+    uint32_t iterations;
+    float residual;
+};
+SolveStats solve(std::vector<Surfel>& s, const Transfer& t, uint32_t max_bounces, int threads);
+
+// src/main.rs:25-51
+enum Mode { BEAUTY = 0, DIRECT = 1, INDIRECT = 2, NORMALS = 3, DEPTH = 4, STEPS = 5 };
+
+// src/main.rs:188-218
+struct Cam {
+    V3 origin, fwd, right, up;
+    float tan_half, aspect;
+    static Cam make(V3 pos, V3 look, V3 up_hint, float fov, uint32_t w, uint32_t h);
+    V3 ray(uint32_t x, uint32_t y, uint32_t w, uint32_t h) const;
+};
+
+struct Gi {
+    const std::vector<Surfel>* surfels;
+    Grid grid;
+    float radius;
+};
+
+// src/main.rs:229-367 (modes beauty..steps; the path tracer is out of scope).
+// Rows [y0, y1) of the film only when a sample is asked for.
+uint64_t render_frame(const Scene& sc, const Cam& cam, Mode mode, const Gi* gi, float tmax, uint32_t w,
+                      uint32_t h, std::vector<V3>& film, int threads, uint32_t y0 = 0, uint32_t y1 = UINT32_MAX);
+
+// src/film.rs:8-61
+uint8_t srgb8(float c);
+V3 aces(V3 x);
+void encode(const std::vector<V3>& film, float exposure, bool tonemap, std::vector<uint8_t>& out);
+
+}  // namespace orc
