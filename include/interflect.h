@@ -1,0 +1,141 @@
+/*
+ * interflect-mi355x: C ABI of the gfx950 transfer -> solve -> shade path.
+ *
+ * The reference (Cherie05/interflect v0.1.3) is one Rust binary with no FFI;
+ * its stage functions, called in sequence by `main`, are the seams this
+ * library replaces.  Each entry point below names the reference function it
+ * stands in for.  Conventions:
+ *   - every call returns an ifx_status: 0 on success, negative on error, with
+ *     a message available from ifx_last_error();
+ *   - the caller owns every host buffer; the library owns device memory;
+ *   - calls are synchronous on return; a context is used from one thread;
+ *   - vectors cross the boundary as packed xyz float triples (N x 3 f32).
+ */
+#ifndef INTERFLECT_H
+#define INTERFLECT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ifx_ctx ifx_ctx;
+typedef int32_t ifx_status;
+
+#define IFX_OK 0
+#define IFX_E_INVALID (-1) /* bad argument */
+#define IFX_E_PARSE (-2)   /* scene text rejected (src/scene.rs:365 error paths) */
+#define IFX_E_STATE (-3)   /* stage called out of order */
+#define IFX_E_DEVICE (-4)  /* HIP runtime error */
+#define IFX_E_NOMEM (-5)   /* device allocation failed */
+#define IFX_E_LIMIT (-6)   /* scene exceeds a kernel limit */
+
+/* Shading modes of src/main.rs:25-51 (the path tracer is not part of this path). */
+#define IFX_MODE_BEAUTY 0
+#define IFX_MODE_DIRECT 1
+#define IFX_MODE_INDIRECT 2
+#define IFX_MODE_NORMALS 3
+#define IFX_MODE_DEPTH 4
+#define IFX_MODE_STEPS 5
+
+typedef struct {
+    uint32_t n_objects, n_planes, n_quad_lights, n_materials; /* materials exclude the implicit default */
+    uint32_t n_bvh_nodes, n_sphere_lights, n_point_lights, max_stack;
+    uint32_t width, height, bounces, reserved;
+    uint64_t surfels;
+    float exposure, blend_k;
+    float lo[3], hi[3];
+    float cam_pos[3], cam_look[3], cam_up[3], cam_fov;
+} ifx_scene_info;
+
+typedef struct {
+    float pos[3], look[3], up[3], fov;
+} ifx_camera;
+
+typedef struct {
+    uint32_t clusters, reserved;
+    uint64_t candidates; /* surfel-cluster pairs that passed the cosine and form-factor culls */
+    uint64_t links;
+    uint64_t evals;      /* SDF evaluations under the reference's counting rule */
+    float mean_row_sum;
+    float ms;            /* wall time of the stage, H2D of the surfels included */
+} ifx_transfer_stats;
+
+typedef struct {
+    uint32_t iterations;
+    float residual;
+    uint64_t evals;
+    float ms;
+    float reserved;
+} ifx_solve_stats;
+
+typedef struct {
+    uint64_t evals;
+    float ms; /* grid build + per-pixel pass + encode + D2H */
+    float reserved;
+} ifx_shade_stats;
+
+/* Context lifetime.  `device` is the HIP device ordinal. */
+ifx_status ifx_create(ifx_ctx** out, int32_t device);
+ifx_status ifx_destroy(ifx_ctx* ctx);
+const char* ifx_last_error(const ifx_ctx* ctx);
+
+/* scene::parse + Bvh::build + world_bounds (src/scene.rs:365-571, src/bvh.rs:53). */
+ifx_status ifx_load_scene(ifx_ctx* ctx, const char* rad_text);
+ifx_status ifx_get_scene_info(ifx_ctx* ctx, ifx_scene_info* out);
+/* CLI overrides of render {} (src/main.rs:394-413); re-validated like the reference. */
+ifx_status ifx_set_render(ifx_ctx* ctx, uint32_t width, uint32_t height, uint64_t surfels, uint32_t bounces);
+/* Scene::use_bvh, the --no-bvh switch (src/scene.rs:172-174). */
+ifx_status ifx_set_use_bvh(ifx_ctx* ctx, int32_t on);
+
+/* surfel::generate (src/surfel.rs:147-255), on the host. */
+ifx_status ifx_generate_surfels(ifx_ctx* ctx, uint64_t target, uint64_t* placed);
+/* Supply surfels directly (the pipeline input "given identical surfels"). */
+ifx_status ifx_set_surfels(ifx_ctx* ctx, const float* p, const float* n, const float* albedo, const float* area,
+                           uint64_t count);
+ifx_status ifx_num_surfels(ifx_ctx* ctx, uint64_t* count);
+ifx_status ifx_get_surfels(ifx_ctx* ctx, float* p, float* n, float* albedo, float* area);
+
+/* formfactor::build_clusters + formfactor::build (src/formfactor.rs:91, :153). */
+ifx_status ifx_transfer(ifx_ctx* ctx, int32_t cluster_res, ifx_transfer_stats* out);
+/* Transfer CSR: starts[N+1], idx[links], w[links]; sky[N] closure, row_sum[N] (-1 = no links). */
+ifx_status ifx_get_transfer(ifx_ctx* ctx, uint32_t* starts, uint32_t* idx, float* w, float* sky, float* row_sum);
+/* Clusters: p[C*3], n[C*3], area[C], radius[C], member_starts[C+1], members[N]. */
+ifx_status ifx_get_clusters(ifx_ctx* ctx, float* p, float* n, float* area, float* radius, uint32_t* member_starts,
+                            uint32_t* members);
+
+/* solve::light_surfels + solve::solve (src/solve.rs:27, :47). */
+ifx_status ifx_solve(ifx_ctx* ctx, uint32_t max_bounces, ifx_solve_stats* out);
+ifx_status ifx_get_radiosity(ifx_ctx* ctx, float* direct, float* irr, float* rad);
+/* Replace the solved radiosity (e.g. to shade from an externally solved cache). */
+ifx_status ifx_set_radiosity(ifx_ctx* ctx, const float* direct, const float* irr, const float* rad);
+
+/* Grid::build + render_frame + Film::encode (src/surfel.rs:71, src/main.rs:229, src/film.rs:51).
+ * cam = NULL uses the scene camera.  film_rgb (w*h*3 f32, linear) and rgb8
+ * (w*h*3 bytes, exposure + tone map + sRGB) may each be NULL. */
+ifx_status ifx_shade(ifx_ctx* ctx, const ifx_camera* cam, uint32_t width, uint32_t height, int32_t mode,
+                     float* film_rgb, uint8_t* rgb8, ifx_shade_stats* out);
+
+/* Kernel-family probes, one launch each.
+ * Scene::dist_mat / Scene::normal (src/scene.rs:191, :218); nrm may be NULL. */
+ifx_status ifx_dist_mat(ifx_ctx* ctx, const float* pts, uint64_t n, float* d, uint32_t* mat, float* nrm);
+/* trace::trace (src/trace.rs:27). */
+ifx_status ifx_trace_rays(ifx_ctx* ctx, const float* ro, const float* rd, uint64_t n, float tmin, float tmax,
+                          float* t, float* nrm, uint32_t* mat, int32_t* hit, uint32_t* steps);
+/* trace::soft_shadow (src/trace.rs:84), per-ray tmax and k. */
+ifx_status ifx_soft_shadows(ifx_ctx* ctx, const float* ro, const float* rd, uint64_t n, float tmin,
+                            const float* tmax, const float* k, float* vis, uint32_t* steps);
+
+/* Per-kernel timing with HIP events on the context's stream (no counterpart
+ * in the reference; used for the roofline report). */
+ifx_status ifx_set_profiling(ifx_ctx* ctx, int32_t on);
+/* Kernel `name`'s launch count and summed duration since the last reset. */
+ifx_status ifx_kernel_time(ifx_ctx* ctx, const char* name, uint64_t* launches, double* total_ms);
+ifx_status ifx_reset_kernel_times(ifx_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INTERFLECT_H */

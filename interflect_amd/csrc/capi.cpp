@@ -1,0 +1,334 @@
+// extern "C" boundary of libinterflect_hip.so (declared in include/interflect.h).
+// No C++ exception crosses it: every entry point catches and maps to a status.
+#include <chrono>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <thread>
+
+#include "../../include/interflect.h"
+#include "engine.h"
+
+struct ifx_ctx {
+    ifx::Engine eng;
+    std::string err;
+    bool have_scene = false;
+};
+
+namespace {
+
+int host_threads() {
+    unsigned n = std::thread::hardware_concurrency();
+    return (int)(n == 0 ? 1 : (n > 16 ? 16 : n));
+}
+
+template <class F>
+ifx_status guarded(ifx_ctx* ctx, F&& f) {
+    if (!ctx) return IFX_E_INVALID;
+    try {
+        ctx->err.clear();
+        return f();
+    } catch (const std::bad_alloc&) {
+        ctx->err = "host allocation failed";
+        return IFX_E_NOMEM;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return IFX_E_INVALID;
+    } catch (...) {
+        ctx->err = "unknown error";
+        return IFX_E_INVALID;
+    }
+}
+
+ifx_status need_scene(ifx_ctx* ctx) {
+    if (!ctx->have_scene) {
+        ctx->err = "no scene loaded";
+        return IFX_E_STATE;
+    }
+    return IFX_OK;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+ifx_status ifx_create(ifx_ctx** out, int32_t device) {
+    if (!out) return IFX_E_INVALID;
+    *out = nullptr;
+    ifx_ctx* c = new (std::nothrow) ifx_ctx();
+    if (!c) return IFX_E_NOMEM;
+    int rc = c->eng.init(device, c->err);
+    if (rc) {
+        // Keep the context so the caller can read the message, then destroy it.
+        *out = c;
+        return rc;
+    }
+    *out = c;
+    return IFX_OK;
+}
+
+ifx_status ifx_destroy(ifx_ctx* ctx) {
+    delete ctx;
+    return IFX_OK;
+}
+
+const char* ifx_last_error(const ifx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+ifx_status ifx_load_scene(ifx_ctx* ctx, const char* rad_text) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (!rad_text) {
+            ctx->err = "null scene text";
+            return IFX_E_INVALID;
+        }
+        int rc = ctx->eng.load_scene(rad_text, ctx->err);
+        ctx->have_scene = rc == 0;
+        return rc;
+    });
+}
+
+ifx_status ifx_get_scene_info(ifx_ctx* ctx, ifx_scene_info* o) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        if (!o) return IFX_E_INVALID;
+        const ifx::HostScene& s = ctx->eng.scene;
+        std::memset(o, 0, sizeof(*o));
+        o->n_objects = (uint32_t)s.objects.size();
+        o->n_planes = (uint32_t)s.planes.size();
+        o->n_quad_lights = (uint32_t)s.quads.size();
+        o->n_materials = (uint32_t)s.mats.size() - 1;
+        o->n_bvh_nodes = (uint32_t)(s.node_rec.size() / 2);
+        o->n_sphere_lights = (uint32_t)s.spheres.size();
+        o->n_point_lights = (uint32_t)s.points.size();
+        o->max_stack = s.max_stack;
+        o->width = s.cfg.width;
+        o->height = s.cfg.height;
+        o->bounces = s.cfg.bounces;
+        o->surfels = s.cfg.surfels;
+        o->exposure = s.cfg.exposure;
+        o->blend_k = s.blend_k;
+        const float* v[5] = {&s.lo.x, &s.hi.x, &s.cam.pos.x, &s.cam.look.x, &s.cam.up.x};
+        float* d[5] = {o->lo, o->hi, o->cam_pos, o->cam_look, o->cam_up};
+        for (int k = 0; k < 5; ++k) std::memcpy(d[k], v[k], 12);
+        o->cam_fov = s.cam.fov;
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_set_render(ifx_ctx* ctx, uint32_t width, uint32_t height, uint64_t surfels, uint32_t bounces) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        ifx::HCfg c = ctx->eng.scene.cfg;
+        c.width = width;
+        c.height = height;
+        c.surfels = surfels;
+        c.bounces = bounces;
+        if (!ifx::validate_cfg(c, ctx->err)) return IFX_E_INVALID;
+        ctx->eng.scene.cfg = c;
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_set_use_bvh(ifx_ctx* ctx, int32_t on) {
+    return guarded(ctx, [&]() -> ifx_status {
+        ctx->eng.use_bvh = on != 0;
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_generate_surfels(ifx_ctx* ctx, uint64_t target, uint64_t* placed) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        if (target == 0 || target > 2000000ull) {
+            ctx->err = "surfel target must be in 1..2000000";
+            return IFX_E_INVALID;
+        }
+        std::vector<ifx::HSurfel> s = ifx::generate_surfels(ctx->eng.scene, target, host_threads());
+        if (placed) *placed = s.size();
+        return ctx->eng.set_surfels(s, ctx->err);
+    });
+}
+
+ifx_status ifx_set_surfels(ifx_ctx* ctx, const float* p, const float* n, const float* albedo, const float* area,
+                           uint64_t count) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        if (count && (!p || !n || !albedo || !area)) return IFX_E_INVALID;
+        std::vector<ifx::HSurfel> s(count);
+        for (uint64_t i = 0; i < count; ++i) {
+            s[i].p = ifx::mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+            s[i].n = ifx::mk(n[3 * i], n[3 * i + 1], n[3 * i + 2]);
+            s[i].albedo = ifx::mk(albedo[3 * i], albedo[3 * i + 1], albedo[3 * i + 2]);
+            s[i].area = area[i];
+        }
+        return ctx->eng.set_surfels(s, ctx->err);
+    });
+}
+
+ifx_status ifx_num_surfels(ifx_ctx* ctx, uint64_t* count) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (!count) return IFX_E_INVALID;
+        *count = ctx->eng.surfels.size();
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_get_surfels(ifx_ctx* ctx, float* p, float* n, float* albedo, float* area) {
+    return guarded(ctx, [&]() -> ifx_status {
+        const auto& s = ctx->eng.surfels;
+        for (size_t i = 0; i < s.size(); ++i) {
+            if (p) std::memcpy(p + 3 * i, &s[i].p, 12);
+            if (n) std::memcpy(n + 3 * i, &s[i].n, 12);
+            if (albedo) std::memcpy(albedo + 3 * i, &s[i].albedo, 12);
+            if (area) area[i] = s[i].area;
+        }
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_transfer(ifx_ctx* ctx, int32_t cluster_res, ifx_transfer_stats* o) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        auto t0 = std::chrono::steady_clock::now();
+        ifx::TransferOut t;
+        int rc = ctx->eng.transfer(cluster_res, t, ctx->err);
+        if (rc) return rc;
+        if (o) {
+            std::memset(o, 0, sizeof(*o));
+            o->clusters = t.clusters;
+            o->candidates = t.candidates;
+            o->links = t.links;
+            o->evals = t.evals;
+            o->mean_row_sum = t.mean_row_sum;
+            o->ms = (float)ms_since(t0);
+        }
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_get_transfer(ifx_ctx* ctx, uint32_t* starts, uint32_t* idx, float* w, float* sky, float* row_sum) {
+    return guarded(ctx, [&]() -> ifx_status { return ctx->eng.download_transfer(starts, idx, w, sky, row_sum, ctx->err); });
+}
+
+ifx_status ifx_get_clusters(ifx_ctx* ctx, float* p, float* n, float* area, float* radius, uint32_t* member_starts,
+                            uint32_t* members) {
+    return guarded(ctx, [&]() -> ifx_status {
+        return ctx->eng.download_clusters(p, n, area, radius, member_starts, members, ctx->err);
+    });
+}
+
+ifx_status ifx_solve(ifx_ctx* ctx, uint32_t max_bounces, ifx_solve_stats* o) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        if (max_bounces == 0 || max_bounces > 4096) {
+            ctx->err = "bounces must be in 1..4096";
+            return IFX_E_INVALID;
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        ifx::SolveOut r;
+        int rc = ctx->eng.solve(max_bounces, r, ctx->err);
+        if (rc) return rc;
+        if (o) {
+            std::memset(o, 0, sizeof(*o));
+            o->iterations = r.iterations;
+            o->residual = r.residual;
+            o->evals = r.evals;
+            o->ms = (float)ms_since(t0);
+        }
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_get_radiosity(ifx_ctx* ctx, float* direct, float* irr, float* rad) {
+    return guarded(ctx, [&]() -> ifx_status { return ctx->eng.download_radiosity(direct, irr, rad, ctx->err); });
+}
+
+ifx_status ifx_set_radiosity(ifx_ctx* ctx, const float* direct, const float* irr, const float* rad) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (!ctx->eng.have_transfer) {
+            ctx->err = "set_radiosity needs surfels uploaded by a transfer";
+            return IFX_E_STATE;
+        }
+        return ctx->eng.upload_radiosity(direct, irr, rad, ctx->err);
+    });
+}
+
+ifx_status ifx_shade(ifx_ctx* ctx, const ifx_camera* cam, uint32_t width, uint32_t height, int32_t mode,
+                     float* film_rgb, uint8_t* rgb8, ifx_shade_stats* o) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        ifx::HCamera c = ctx->eng.scene.cam;
+        if (cam) {
+            c.pos = ifx::mk(cam->pos[0], cam->pos[1], cam->pos[2]);
+            c.look = ifx::mk(cam->look[0], cam->look[1], cam->look[2]);
+            c.up = ifx::mk(cam->up[0], cam->up[1], cam->up[2]);
+            c.fov = cam->fov;
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        uint64_t ev = 0;
+        int rc = ctx->eng.shade(c, width, height, mode, film_rgb, rgb8, ev, ctx->err);
+        if (rc) return rc;
+        if (o) {
+            std::memset(o, 0, sizeof(*o));
+            o->evals = ev;
+            o->ms = (float)ms_since(t0);
+        }
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_dist_mat(ifx_ctx* ctx, const float* pts, uint64_t n, float* d, uint32_t* mat, float* nrm) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        if (n && (!pts || !d || !mat)) return IFX_E_INVALID;
+        return ctx->eng.probe_dist(pts, n, d, mat, nrm, ctx->err);
+    });
+}
+
+ifx_status ifx_trace_rays(ifx_ctx* ctx, const float* ro, const float* rd, uint64_t n, float tmin, float tmax,
+                          float* t, float* nrm, uint32_t* mat, int32_t* hit, uint32_t* steps) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        if (n && (!ro || !rd || !t || !nrm || !mat || !hit || !steps)) return IFX_E_INVALID;
+        return ctx->eng.probe_trace(ro, rd, n, tmin, tmax, t, nrm, mat, hit, steps, ctx->err);
+    });
+}
+
+ifx_status ifx_soft_shadows(ifx_ctx* ctx, const float* ro, const float* rd, uint64_t n, float tmin,
+                            const float* tmax, const float* k, float* vis, uint32_t* steps) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        if (n && (!ro || !rd || !tmax || !k || !vis || !steps)) return IFX_E_INVALID;
+        return ctx->eng.probe_shadow(ro, rd, n, tmin, tmax, k, vis, steps, ctx->err);
+    });
+}
+
+ifx_status ifx_set_profiling(ifx_ctx* ctx, int32_t on) {
+    return guarded(ctx, [&]() -> ifx_status {
+        ctx->eng.set_profiling(on != 0);
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_kernel_time(ifx_ctx* ctx, const char* name, uint64_t* launches, double* total_ms) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (!name) return IFX_E_INVALID;
+        const auto& m = ctx->eng.kernel_stats();
+        auto it = m.find(name);
+        if (launches) *launches = it == m.end() ? 0 : it->second.launches;
+        if (total_ms) *total_ms = it == m.end() ? 0.0 : it->second.ms;
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_reset_kernel_times(ifx_ctx* ctx) {
+    return guarded(ctx, [&]() -> ifx_status {
+        ctx->eng.reset_kernel_stats();
+        return IFX_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// Device engine behind the C ABI: owns the HIP stream and device buffers of
+// one scene and runs the transfer -> solve -> shade stages on gfx950.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "host_scene.h"
+#include "host_surfels.h"
+
+namespace ifx {
+
+constexpr int kBlock = 256;       // threads per block of every traversal kernel
+constexpr int kStackCap = 32;     // LDS traversal-stack entries per lane
+
+// Grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+    ~DevBuf() { release(); }
+};
+
+struct TransferOut {
+    uint32_t clusters = 0;
+    uint64_t candidates = 0, links = 0, evals = 0;
+    float mean_row_sum = 0;
+};
+
+struct SolveOut {
+    uint32_t iterations = 0;
+    float residual = 0;
+    uint64_t evals = 0;
+};
+
+struct KernelStat {
+    uint64_t launches = 0;
+    double ms = 0;
+};
+
+class Engine {
+   public:
+    Engine() = default;
+    ~Engine();
+    int init(int device, std::string& err);
+
+    int load_scene(const std::string& src, std::string& err);
+    int set_surfels(const std::vector<HSurfel>& s, std::string& err);
+    int transfer(int32_t res, TransferOut& out, std::string& err);
+    int solve(uint32_t bounces, SolveOut& out, std::string& err);
+    int shade(const HCamera& cam, uint32_t w, uint32_t h, int32_t mode, float* film, uint8_t* rgb8, uint64_t& evals,
+              std::string& err);
+
+    // Kernel-family probes (one launch each).
+    int probe_dist(const float* pts, uint64_t n, float* d, uint32_t* mat, float* nrm, std::string& err);
+    int probe_trace(const float* ro, const float* rd, uint64_t n, float tmin, float tmax, float* t, float* nrm,
+                    uint32_t* mat, int32_t* hit, uint32_t* steps, std::string& err);
+    int probe_shadow(const float* ro, const float* rd, uint64_t n, float tmin, const float* tmax, const float* k,
+                     float* vis, uint32_t* steps, std::string& err);
+
+    int download_transfer(uint32_t* starts, uint32_t* idx, float* w, float* sky, float* row_sum, std::string& err);
+    int download_clusters(float* p, float* n, float* area, float* radius, uint32_t* mstart, uint32_t* members,
+                          std::string& err);
+    int download_radiosity(float* direct, float* irr, float* rad, std::string& err);
+    int upload_radiosity(const float* direct, const float* irr, const float* rad, std::string& err);
+
+    void set_profiling(bool on) { prof_ = on; }
+    const std::map<std::string, KernelStat>& kernel_stats() const { return kstats_; }
+    void reset_kernel_stats() { kstats_.clear(); }
+
+    HostScene scene;
+    std::vector<HSurfel> surfels;
+    bool use_bvh = true;
+    bool have_transfer = false, have_solve = false;
+    uint32_t n_clusters = 0;
+    uint64_t n_links = 0;
+    hipStream_t stream = nullptr;
+
+   private:
+    int check(hipError_t e, const char* what, std::string& err);
+    int need_device(std::string& err) const;
+    SceneView dev_view() const;
+    LightsView dev_lights() const;
+    template <class F>
+    void launch(const char* name, F&& f);
+    void drain_events();
+
+    int device_ = 0;
+    bool has_device_ = false;
+    bool prof_ = false;
+    std::map<std::string, KernelStat> kstats_;
+    struct Pending {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> event_pool_;
+
+    // scene
+    DevBuf d_obj, d_planes, d_subs, d_nodes, d_lin, d_quad, d_sph, d_pnt, d_mat, d_dirs, d_srgb;
+    // surfels (float4: pos.xyz+area, nrm, albedo)
+    DevBuf d_spos, d_snrm, d_salb;
+    bool surfels_dirty = true;
+    // clusters
+    DevBuf d_ckey, d_ckey2, d_cval, d_cval2, d_cflag, d_cseg, d_cpos, d_cnrm, d_crange, d_cvalid, d_cidx;
+    DevBuf d_craw_pos, d_craw_nrm;
+    // candidates and links
+    DevBuf d_ccount, d_cstart, d_cand_ci, d_cand_row, d_cand_v, d_sky, d_lcount, d_lstart, d_scale, d_rowsum, d_idx, d_w;
+    // solve
+    DevBuf d_direct, d_irr, d_rad, d_bc, d_resid;
+    // shade
+    DevBuf d_gkey, d_gkey2, d_gval, d_gval2, d_gstart, d_film, d_rgb8;
+    // misc
+    DevBuf d_counter, d_tmp, d_probe_in, d_probe_out;
+    uint32_t gdim_[3] = {1, 1, 1};
+};
+
+}  // namespace ifx

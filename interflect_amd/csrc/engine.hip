@@ -1,0 +1,1315 @@
+// gfx950 kernels of the transfer -> solve -> shade path and the engine that
+// drives them.  Every floating-point reduction is either serial per row /
+// cluster / pixel in the reference's order or a max (order-free); integer
+// atomics are used only for counters, so results are bit-identical across
+// runs and launch configurations.
+//
+// Stage map (reference):
+//   transfer: build_clusters  src/formfactor.rs:91-147   -> k_cluster_*
+//             build           src/formfactor.rs:153-277  -> k_ff<>, k_visibility, k_closure, k_row_sum, k_compact
+//   solve:    light_surfels   src/solve.rs:27-39         -> k_direct_e
+//             solve           src/solve.rs:47-108        -> k_aggregate, k_spmv
+//   shade:    Grid::build     src/surfel.rs:71-114       -> k_grid_keys + radix sort
+//             render_frame    src/main.rs:229-367        -> k_shade
+//             Film::encode    src/film.rs:51-61          -> k_encode
+#include <hipcub/hipcub.hpp>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "engine.h"
+
+namespace ifx {
+
+// ====================================================================== memory
+hipError_t DevBuf::ensure(size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (bytes <= cap) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        cap = 0;
+        (void)hipGetLastError();
+        return e;
+    }
+    cap = bytes;
+    return hipSuccess;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+// ====================================================================== device helpers
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+// Adds a per-lane step count to a 64-bit counter, one atomic per wave.
+__device__ __forceinline__ void count_steps(unsigned long long* ctr, uint32_t steps) {
+    uint32_t t = wave_sum(steps);
+    if (lane_id() == 0 && t) atomicAdd(ctr, (unsigned long long)t);
+}
+
+// Number of set bits of `m` at lanes below this one.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+#define IFX_LDS_STACK()                                  \
+    __shared__ uint16_t stk_mem[kStackCap * kBlock];     \
+    LdsStack stk{stk_mem + threadIdx.x, (uint32_t)kBlock}
+
+// src/formfactor.rs:70-89
+__device__ __forceinline__ uint32_t normal_bucket(f3 n) {
+    f3 a = abs3(n);
+    if (a.x >= a.y && a.x >= a.z) return n.x >= 0.0f ? 0u : 1u;
+    if (a.y >= a.z) return n.y >= 0.0f ? 2u : 3u;
+    return n.z >= 0.0f ? 4u : 5u;
+}
+
+// Surfel -> cluster candidate geometry and culls (src/formfactor.rs:169-192).
+__device__ __forceinline__ bool ff_geom(f3 sp, f3 sn, float4 cp, float4 cn, float& r, f3& dir, float& ff) {
+    f3 d = ld3(cp) - sp;
+    float r2 = dot3(d, d);
+    if (r2 < 1e-9f) return false;
+    r = sqrtf(r2);
+    dir = d / r;
+    float cos_i = dot3(sn, dir);
+    float cos_c = dot3(ld3(cn), -dir);
+    if (cos_i <= 1e-3f || cos_c <= 1e-3f) return false;
+    float area = cp.w;
+    ff = cos_i * cos_c * area / (kPi * r2 + area);
+    return !(ff < 2e-5f);
+}
+
+// ====================================================================== probes
+__global__ __launch_bounds__(kBlock) void k_probe_dist(SceneView S, const float* pts, uint32_t n, float* d,
+                                                      uint32_t* mat, float* nrm) {
+    IFX_LDS_STACK();
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    f3 p = mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+    DistHit h = dist_mat(S, p, stk);
+    d[i] = h.d;
+    mat[i] = h.mat;
+    if (nrm) {
+        f3 q = scene_normal(S, p, stk);
+        nrm[3 * i] = q.x, nrm[3 * i + 1] = q.y, nrm[3 * i + 2] = q.z;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_probe_trace(SceneView S, const float* ro, const float* rd, uint32_t n,
+                                                       float tmin, float tmax, float* t, float* nrm, uint32_t* mat,
+                                                       int32_t* hit, uint32_t* steps) {
+    IFX_LDS_STACK();
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s = 0;
+    Hit h = trace<true>(S, mk(ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]), mk(rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]),
+                        tmin, tmax, s, stk);
+    t[i] = h.t;
+    nrm[3 * i] = h.n.x, nrm[3 * i + 1] = h.n.y, nrm[3 * i + 2] = h.n.z;
+    mat[i] = h.mat;
+    hit[i] = h.hit ? 1 : 0;
+    steps[i] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_probe_shadow(SceneView S, const float* ro, const float* rd, uint32_t n,
+                                                        float tmin, const float* tmax, const float* k, float* vis,
+                                                        uint32_t* steps) {
+    IFX_LDS_STACK();
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s = 0;
+    vis[i] = soft_shadow(S, mk(ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]), mk(rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]),
+                         tmin, tmax[i], k[i], s, stk);
+    steps[i] = s;
+}
+
+// ====================================================================== transfer: clusters
+// Key of (voxel x, y, z, normal bucket), packed so integer order equals the
+// reference's sorted tuple order (src/formfactor.rs:101-116).
+__global__ void k_cluster_keys(const float4* __restrict__ spos, const float4* __restrict__ snrm, uint32_t n, f3 lo,
+                               f3 inv, int32_t res, uint64_t* keys, uint32_t* vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 c = (ld3(spos[i]) - lo) * inv;
+    uint64_t kx = (uint64_t)clampi(sat_i32(c.x), 0, res - 1);
+    uint64_t ky = (uint64_t)clampi(sat_i32(c.y), 0, res - 1);
+    uint64_t kz = (uint64_t)clampi(sat_i32(c.z), 0, res - 1);
+    uint64_t r = (uint64_t)res;
+    keys[i] = ((kx * r + ky) * r + kz) * 6ull + normal_bucket(ld3(snrm[i]));
+    vals[i] = i;
+}
+
+__global__ void k_seg_flags(const uint64_t* keys, uint32_t n, uint32_t* flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+__global__ void k_seg_begin(const uint32_t* flags, const uint32_t* seg, uint32_t n, uint32_t* begin) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (flags[i]) begin[seg[i]] = i;
+    if (i == n - 1) begin[seg[i] + flags[i]] = n;
+}
+
+// Serial area-weighted sums in ascending member order (src/formfactor.rs:120-145).
+__global__ void k_cluster_build(const uint32_t* begin, uint32_t nseg, const uint32_t* members,
+                                const float4* __restrict__ spos, const float4* __restrict__ snrm, float4* cpos,
+                                float4* cnrm, uint32_t* valid) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nseg) return;
+    f3 p = spl(0.0f), n = spl(0.0f);
+    float area = 0.0f;
+    for (uint32_t k = begin[c]; k < begin[c + 1]; ++k) {
+        uint32_t m = members[k];
+        float4 P = spos[m];
+        p = p + ld3(P) * P.w;
+        n = n + ld3(snrm[m]) * P.w;
+        area += P.w;
+    }
+    valid[c] = area <= 1e-12f ? 0u : 1u;
+    f3 nn = dot3(n, n) > 1e-12f ? nrm3(n) : mk(0.0f, 1.0f, 0.0f);
+    cpos[c] = st4(p / area, area);
+    cnrm[c] = st4(nn, sqrtf(area / kPi));
+}
+
+__global__ void k_cluster_compact(uint32_t nseg, const uint32_t* valid, const uint32_t* vidx, const uint32_t* begin,
+                                  const float4* cpos_raw, const float4* cnrm_raw, float4* cpos, float4* cnrm,
+                                  uint2* range) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nseg || !valid[c]) return;
+    uint32_t j = vidx[c];
+    cpos[j] = cpos_raw[c];
+    cnrm[j] = cnrm_raw[c];
+    range[j] = make_uint2(begin[c], begin[c + 1]);
+}
+
+// ====================================================================== transfer: links
+// One wavefront per surfel row; lanes sweep the clusters 64 at a time and a
+// ballot keeps survivors in ascending cluster order.  EMIT=false counts,
+// EMIT=true writes candidates at the row's scanned offset.
+template <bool EMIT>
+__global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, const float4* __restrict__ snrm,
+                                               uint32_t N, const float4* __restrict__ cpos,
+                                               const float4* __restrict__ cnrm, uint32_t C, uint32_t* count,
+                                               const uint32_t* __restrict__ start, uint32_t* cand_ci,
+                                               uint32_t* cand_row) {
+    uint32_t lane = lane_id();
+    uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    uint32_t nwaves = (gridDim.x * kBlock) >> 6;
+    for (uint32_t i = wave; i < N; i += nwaves) {
+        f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
+        uint32_t base = EMIT ? start[i] : 0u;
+        uint32_t run = 0;
+        for (uint32_t c0 = 0; c0 < C; c0 += 64) {
+            uint32_t c = c0 + lane;
+            bool pass = false;
+            if (c < C) {
+                float r, ff;
+                f3 dir;
+                pass = ff_geom(sp, sn, cpos[c], cnrm[c], r, dir, ff);
+            }
+            uint64_t m = __ballot(pass);
+            if (EMIT && pass) {
+                uint32_t o = base + run + lanes_below(m);
+                cand_ci[o] = c;
+                cand_row[o] = i;
+            }
+            run += (uint32_t)__popcll(m);
+        }
+        if (!EMIT && lane == 0) count[i] = run;
+    }
+}
+
+// Cone-traced link visibility, one candidate per lane (src/formfactor.rs:194-214).
+// cand_v = ff * vis for a kept link, 0 for a culled one.
+__global__ __launch_bounds__(kBlock) void k_visibility(SceneView S, const float4* __restrict__ spos,
+                                                       const float4* __restrict__ snrm,
+                                                       const float4* __restrict__ cpos,
+                                                       const float4* __restrict__ cnrm,
+                                                       const uint32_t* __restrict__ cand_ci,
+                                                       const uint32_t* __restrict__ cand_row, uint32_t M,
+                                                       float off, float tmin, float* cand_v,
+                                                       unsigned long long* evals) {
+    IFX_LDS_STACK();
+    uint32_t acc = 0;
+    uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < M; k += stride) {
+        uint32_t i = cand_row[k], c = cand_ci[k];
+        f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
+        float4 CN = cnrm[c];
+        float r = 0.0f, ff = 0.0f;
+        f3 dir = spl(0.0f);
+        ff_geom(sp, sn, cpos[c], CN, r, dir, ff);
+        float kk = clmp(r / fmx(CN.w, 1e-4f), 1.5f, 32.0f);
+        uint32_t s = 0;
+        float vis = soft_shadow(S, sp + sn * off, dir, tmin, r - CN.w * 0.75f, kk, s, stk);
+        acc += s;
+        cand_v[k] = vis <= 1e-3f ? 0.0f : ff * vis;
+    }
+    count_steps(evals, acc);
+}
+
+// Hemisphere closure: 16 lanes per surfel, one Halton direction each
+// (src/trace.rs:165-186).  dirs[k] = (r cos phi, r sin phi, sqrt(1 - u1)).
+__global__ __launch_bounds__(kBlock) void k_closure(SceneView S, const float4* __restrict__ spos,
+                                                    const float4* __restrict__ snrm, uint32_t N,
+                                                    const float4* __restrict__ dirs, float tmax, float* sky,
+                                                    unsigned long long* evals) {
+    IFX_LDS_STACK();
+    uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t row = tid >> 4, k = tid & 15u;
+    bool active = row < N;
+    uint32_t steps = 0;
+    bool open = false;
+    if (active) {
+        f3 p = ld3(spos[row]), n = ld3(snrm[row]);
+        f3 t, b;
+        frame(n, t, b);
+        float4 D = dirs[k];
+        f3 d = (t * D.x + b * D.y) + n * D.z;
+        Hit h = trace<false>(S, p + n * 1e-3f, nrm3(d), 1e-3f, tmax, steps, stk);
+        open = !h.hit;
+    }
+    uint64_t m = __ballot(open);
+    uint32_t cnt = (uint32_t)__popcll((m >> (lane_id() & 48u)) & 0xffffull);
+    if (active && k == 0) sky[row] = (float)cnt / 16.0f;
+    count_steps(evals, steps);
+}
+
+// Serial per-row link sum in cluster order plus the closure calibration
+// (src/formfactor.rs:207-240).  row_sum = -1 marks a row without links.
+__global__ void k_row_sum(uint32_t N, const uint32_t* __restrict__ cstart, const float* __restrict__ cand_v,
+                          const float* __restrict__ sky, uint32_t* lcount, float* scale, float* row_sum) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float sum = 0.0f;
+    uint32_t cnt = 0;
+    for (uint32_t k = cstart[i]; k < cstart[i + 1]; ++k) {
+        float v = cand_v[k];
+        if (v > 0.0f) {
+            sum += v;
+            ++cnt;
+        }
+    }
+    float target = clmp(1.0f - sky[i], 0.0f, 1.0f);
+    float sc = 1.0f;
+    if (sum > 1e-5f) {
+        sc = fmn(target / sum, 8.0f);
+        sum *= sc;
+    }
+    lcount[i] = cnt;
+    scale[i] = sc;
+    row_sum[i] = cnt ? fmn(sum, 1.0f) : -1.0f;
+}
+
+__global__ void k_compact(uint32_t N, const uint32_t* __restrict__ cstart, const float* __restrict__ cand_v,
+                          const uint32_t* __restrict__ cand_ci, const uint32_t* __restrict__ lstart,
+                          const float* __restrict__ scale, uint32_t* idx, float* w) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    uint32_t pos = lstart[i];
+    float sc = scale[i];
+    for (uint32_t k = cstart[i]; k < cstart[i + 1]; ++k) {
+        float v = cand_v[k];
+        if (v > 0.0f) {
+            idx[pos] = cand_ci[k];
+            w[pos] = v * sc;
+            ++pos;
+        }
+    }
+}
+
+// ====================================================================== solve
+__global__ __launch_bounds__(kBlock) void k_direct_e(SceneView S, LightsView L, const float4* __restrict__ spos,
+                                                     const float4* __restrict__ snrm, const float4* __restrict__ salb,
+                                                     uint32_t N, float4* direct, float4* rad, float4* irr,
+                                                     unsigned long long* evals) {
+    IFX_LDS_STACK();
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t steps = 0;
+    if (i < N) {
+        f3 e = direct_diffuse(S, L, ld3(spos[i]), ld3(snrm[i]), ld3(salb[i]), steps, stk);
+        direct[i] = st4(e);
+        rad[i] = st4(e);
+        irr[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    count_steps(evals, steps);
+}
+
+// Jacobi bounce `it` is a no-op once bounce it-1 converged (residual < 1e-4),
+// which reproduces the reference's early `break` with a fixed launch count.
+__device__ __forceinline__ bool converged_before(const uint32_t* resid, uint32_t it) {
+    return it > 0 && bitsf(resid[it - 1]) < 1e-4f;
+}
+
+// Cluster radiosity: serial area-weighted sum over members (src/solve.rs:66-73).
+__global__ void k_aggregate(uint32_t it, const uint32_t* resid, uint32_t C, const uint2* __restrict__ range,
+                            const uint32_t* __restrict__ members, const float4* __restrict__ spos,
+                            const float4* __restrict__ cpos, const float4* __restrict__ rad, float4* bc) {
+    if (converged_before(resid, it)) return;
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    uint2 r = range[c];
+    f3 acc = spl(0.0f);
+    for (uint32_t k = r.x; k < r.y; ++k) {
+        uint32_t m = members[k];
+        acc = acc + ld3(rad[m]) * spos[m].w;
+    }
+    bc[c] = st4(acc / fmx(cpos[c].w, 1e-12f));
+}
+
+// Row gather B = E + rho * sum_k w_k Bc[idx_k] in link order, and the
+// max-norm update as an integer max of non-negative float bits (src/solve.rs:76-93).
+__global__ __launch_bounds__(kBlock) void k_spmv(uint32_t it, uint32_t* resid, uint32_t N,
+                                                 const uint32_t* __restrict__ lstart,
+                                                 const uint32_t* __restrict__ idx, const float* __restrict__ w,
+                                                 const float4* __restrict__ bc, const float4* __restrict__ direct,
+                                                 const float4* __restrict__ salb, float4* rad, float4* irr) {
+    if (converged_before(resid, it)) return;
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t delta = 0;
+    if (i < N) {
+        f3 g = spl(0.0f);
+        uint32_t e = lstart[i + 1];
+        for (uint32_t k = lstart[i]; k < e; ++k) g = g + ld3(bc[idx[k]]) * w[k];
+        f3 nr = ld3(direct[i]) + ld3(salb[i]) * g;
+        delta = fbits(maxe(abs3(nr - ld3(rad[i]))));
+        irr[i] = st4(g);
+        rad[i] = st4(nr);
+    }
+    uint32_t m = wave_max(delta);
+    if (lane_id() == 0 && m) atomicMax(&resid[it], m);
+}
+
+// ====================================================================== shade
+__global__ void k_grid_keys(const float4* __restrict__ spos, uint32_t N, f3 lo, float inv_cell, int32_t dx, int32_t dy,
+                            int32_t dz, uint32_t* keys, uint32_t* vals, uint32_t* counts) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    f3 p = ld3(spos[i]);
+    int32_t cx = clampi(sat_i32((p.x - lo.x) * inv_cell), 0, dx - 1);
+    int32_t cy = clampi(sat_i32((p.y - lo.y) * inv_cell), 0, dy - 1);
+    int32_t cz = clampi(sat_i32((p.z - lo.z) * inv_cell), 0, dz - 1);
+    uint32_t c = (uint32_t)(cx + dx * (cy + dy * cz));
+    keys[i] = c;
+    vals[i] = i;
+    atomicAdd(&counts[c], 1u);
+}
+
+struct CamV {
+    f3 origin, fwd, right, up;
+    float tan_half, aspect;
+};
+
+struct ShadeArgs {
+    CamV cam;
+    uint32_t W, H;
+    int32_t mode;
+    int32_t use_gi;
+    float tmax, radius;
+    GridView grid;
+    const float4* spos;
+    const float4* snrm;
+    const float4* irr;
+};
+
+enum { M_BEAUTY = 0, M_DIRECT = 1, M_INDIRECT = 2, M_NORMALS = 3, M_DEPTH = 4, M_STEPS = 5 };
+
+// One pixel of render_frame (src/main.rs:247-361).
+template <class Stk>
+__device__ f3 shade_pixel(const SceneView& S, const LightsView& L, const ShadeArgs& A, uint32_t x, uint32_t y,
+                          uint32_t& local, Stk& stk) {
+    const f3 bg = mk(0.02f, 0.03f, 0.05f);
+    const CamV& cam = A.cam;
+    float px = (2.0f * ((float)x + 0.5f) / (float)A.W - 1.0f) * cam.aspect * cam.tan_half;
+    float py = (1.0f - 2.0f * ((float)y + 0.5f) / (float)A.H) * cam.tan_half;
+    f3 rd = nrm3((cam.fwd + cam.right * px) + cam.up * py);
+    uint32_t steps = 0;
+    Hit hh = trace<true>(S, cam.origin, rd, 1e-3f, A.tmax, steps, stk);
+    local += steps;
+    float surf_t = hh.hit ? hh.t : kFltMax;
+    bool tonemapped = A.mode <= M_INDIRECT;
+    f3 e;
+    if (emitter_hit(L, cam.origin, rd, fmn(surf_t, A.tmax), e)) return tonemapped ? e : spl(0.9f);
+    if (!hh.hit) return bg;
+    if (A.mode == M_NORMALS) return hh.n * 0.5f + spl(0.5f);
+    if (A.mode == M_DEPTH) {
+        float d = 1.0f - clmp(hh.t / A.tmax, 0.0f, 1.0f);
+        return spl(d * d);
+    }
+    if (A.mode == M_STEPS) {
+        float f = (float)steps / (float)kMaxSteps;
+        return mk(f, 1.0f - f, 0.2f);
+    }
+    MatV m = load_mat(L, hh.mat);
+    f3 c = spl(0.0f);
+    if (A.mode != M_INDIRECT) {
+        uint32_t s = 0;
+        f3 d = direct(S, L, hh.p, hh.n, -rd, m, s, stk);
+        local += s;
+        c = c + (d + m.emissive);
+    }
+    if (A.use_gi) {
+        f3 irr = gather(A.grid, A.spos, A.snrm, A.irr, hh.p, hh.n, A.radius);
+        c = c + m.kd * irr;
+        if (m.roughness < 0.45f || m.metallic > 0.05f) {
+            f3 refl = rd - hh.n * (2.0f * dot3(hh.n, rd));
+            uint32_t s = 0;
+            Hit rh = trace<true>(S, hh.p + hh.n * 1e-3f, refl, 1e-3f, A.tmax, s, stk);
+            local += s;
+            float rt = rh.hit ? rh.t : kFltMax;
+            f3 refl_col, e2;
+            if (emitter_hit(L, hh.p, refl, fmn(rt, A.tmax), e2)) {
+                refl_col = e2;
+            } else if (rh.hit) {
+                MatV m2 = load_mat(L, rh.mat);
+                uint32_t s2 = 0;
+                f3 d2 = direct(S, L, rh.p, rh.n, -refl, m2, s2, stk);
+                local += s2;
+                f3 i2 = gather(A.grid, A.spos, A.snrm, A.irr, rh.p, rh.n, A.radius);
+                refl_col = (d2 + m2.kd * i2) + m2.emissive;
+            } else {
+                refl_col = bg;
+            }
+            f3 f = fresnel_schlick(fmx(dot3(hh.n, -rd), 0.0f), m.f0);
+            float gloss = clmp(1.0f - m.roughness / 0.45f, 0.0f, 1.0f);
+            gloss = gloss * gloss;
+            c = c + (f * refl_col) * gloss;
+        }
+    }
+    return c;
+}
+
+// 16x16 pixel blocks, one 8x8 tile per wavefront.
+__global__ __launch_bounds__(kBlock) void k_shade(SceneView S, LightsView L, ShadeArgs A, float4* film,
+                                                  unsigned long long* evals) {
+    IFX_LDS_STACK();
+    uint32_t wv = threadIdx.x >> 6, l = threadIdx.x & 63u;
+    uint32_t x = blockIdx.x * 16u + (wv & 1u) * 8u + (l & 7u);
+    uint32_t y = blockIdx.y * 16u + (wv >> 1) * 8u + (l >> 3);
+    uint32_t local = 0;
+    if (x < A.W && y < A.H) {
+        f3 c = shade_pixel(S, L, A, x, y, local, stk);
+        film[(size_t)y * A.W + x] = st4(c);
+    }
+    count_steps(evals, local);
+}
+
+// Film::encode: exposure, ACES fit, sRGB through host-built thresholds
+// (thr[k-1] = least f32 that the sRGB transfer maps to >= k).
+__device__ __forceinline__ uint8_t srgb_lut(float c, const float* __restrict__ thr) {
+    c = clmp(c, 0.0f, 1.0f);
+    uint32_t lo = 0, hi = 255;  // count of thresholds <= c
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (c >= thr[mid])
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return (uint8_t)lo;
+}
+
+__global__ void k_encode(const float4* __restrict__ film, uint32_t npx, float exposure, int32_t tonemap,
+                         const float* __restrict__ thr, uint8_t* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    f3 x = ld3(film[i]) * exposure;
+    if (tonemap) {
+        f3 num = x * (x * 2.51f + spl(0.03f));
+        f3 den = x * (x * 2.43f + spl(0.59f)) + spl(0.14f);
+        x = min3(max3(num / den, spl(0.0f)), spl(1.0f));
+    }
+    out[3 * i] = srgb_lut(x.x, thr);
+    out[3 * i + 1] = srgb_lut(x.y, thr);
+    out[3 * i + 2] = srgb_lut(x.z, thr);
+}
+
+// ====================================================================== host side
+namespace {
+uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+
+uint32_t bits_for(uint64_t maxv) {
+    uint32_t b = 1;
+    while (b < 64 && (maxv >> b) != 0) ++b;
+    return b;
+}
+
+float host_srgb(float c) {
+    c = clmp(c, 0.0f, 1.0f);
+    float v = c <= 0.0031308f ? c * 12.92f : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
+    float q = v * 255.0f + 0.5f;
+    return (float)(!(q > 0.0f) ? 0 : (q >= 255.0f ? 255 : (int)q));
+}
+}  // namespace
+
+int Engine::check(hipError_t e, const char* what, std::string& err) {
+    if (e == hipSuccess) return 0;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    (void)hipGetLastError();
+    return e == hipErrorOutOfMemory ? -5 : -4;
+}
+
+#define CK(call)                                       \
+    do {                                               \
+        int rc_ = check((call), #call, err);           \
+        if (rc_) return rc_;                           \
+    } while (0)
+
+Engine::~Engine() {
+    if (stream) {
+        (void)hipSetDevice(device_);
+        (void)hipStreamSynchronize(stream);
+        drain_events();
+        for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(stream);
+    }
+}
+
+int Engine::init(int device, std::string& err) {
+    if (device < 0) {  // host-only context: loader and surfel placement, no kernels
+        has_device_ = false;
+        return 0;
+    }
+    int n = 0;
+    CK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        err = "no HIP device " + std::to_string(device);
+        return -1;
+    }
+    device_ = device;
+    CK(hipSetDevice(device));
+    CK(hipStreamCreate(&stream));
+    // Closure directions and sRGB thresholds are scene independent.
+    float4 dirs[16];
+    for (uint32_t k = 0; k < 16; ++k) {
+        auto ri = [](uint32_t i, uint32_t base) {
+            float f = 1.0f, r = 0.0f, inv = 1.0f / (float)base;
+            while (i > 0) f *= inv, r += f * (float)(i % base), i /= base;
+            return r;
+        };
+        float u1 = ri(k + 1, 2), u2 = ri(k + 1, 3);
+        float r = sqrtf(u1);
+        float phi = kTau * u2;
+        dirs[k] = make_float4(r * cosf(phi), r * sinf(phi), sqrtf(fmx(1.0f - u1, 0.0f)), 0.0f);
+    }
+    CK(d_dirs.ensure(sizeof(dirs)));
+    CK(hipMemcpy(d_dirs.p, dirs, sizeof(dirs), hipMemcpyHostToDevice));
+    float thr[255];
+    for (int k = 1; k <= 255; ++k) {
+        uint32_t lo = 0, hi = fbits(1.0f);  // srgb(1.0) = 255
+        while (lo < hi) {
+            uint32_t mid = lo + (hi - lo) / 2;
+            if (host_srgb(bitsf(mid)) >= (float)k)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        thr[k - 1] = bitsf(lo);
+    }
+    CK(d_srgb.ensure(sizeof(thr)));
+    CK(hipMemcpy(d_srgb.p, thr, sizeof(thr), hipMemcpyHostToDevice));
+    CK(d_counter.ensure(64));
+    has_device_ = true;
+    return 0;
+}
+
+int Engine::need_device(std::string& err) const {
+    if (has_device_) return 0;
+    err = "context was created without a device";
+    return -3;
+}
+
+template <class F>
+void Engine::launch(const char* name, F&& f) {
+    if (!prof_) {
+        f();
+        return;
+    }
+    hipEvent_t a, b;
+    if (event_pool_.size() >= 2) {
+        a = event_pool_.back();
+        event_pool_.pop_back();
+        b = event_pool_.back();
+        event_pool_.pop_back();
+    } else {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+    }
+    (void)hipEventRecord(a, stream);
+    f();
+    (void)hipEventRecord(b, stream);
+    pending_.push_back(Pending{name, a, b});
+}
+
+void Engine::drain_events() {
+    for (Pending& p : pending_) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            KernelStat& k = kstats_[p.name];
+            k.launches += 1;
+            k.ms += ms;
+        }
+        event_pool_.push_back(p.a);
+        event_pool_.push_back(p.b);
+    }
+    pending_.clear();
+}
+
+SceneView Engine::dev_view() const {
+    SceneView v = scene.host_view();
+    v.obj = d_obj.as<float4>();
+    v.planes = d_planes.as<float4>();
+    v.subs = d_subs.as<float4>();
+    v.nodes = d_nodes.as<float4>();
+    v.lin = d_lin.as<uint32_t>();
+    v.use_bvh = use_bvh ? 1u : 0u;
+    return v;
+}
+
+LightsView Engine::dev_lights() const {
+    LightsView l = scene.host_lights();
+    l.quad = d_quad.as<float4>();
+    l.sph = d_sph.as<float4>();
+    l.pnt = d_pnt.as<float4>();
+    l.mat = d_mat.as<float4>();
+    return l;
+}
+
+int Engine::load_scene(const std::string& src, std::string& err) {
+    HostScene s;
+    if (!load_rad(src, s, err)) return -2;
+    if (s.max_stack > (uint32_t)kStackCap) {
+        err = "BVH needs a traversal stack of " + std::to_string(s.max_stack) + " entries; the kernels hold " +
+              std::to_string(kStackCap);
+        return -6;
+    }
+    if (s.node_rec.size() / 2 > 65535) {
+        err = "scene has more than 65535 BVH nodes";
+        return -6;
+    }
+    if (!has_device_) {
+        scene = std::move(s);
+        surfels.clear();
+        surfels_dirty = true;
+        have_transfer = have_solve = false;
+        return 0;
+    }
+    CK(hipSetDevice(device_));
+    auto up = [&](DevBuf& b, const void* h, size_t bytes) -> int {
+        CK(b.ensure(bytes));
+        if (bytes) CK(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+        return 0;
+    };
+    int rc;
+    if ((rc = up(d_obj, s.obj_rec.data(), s.obj_rec.size() * 16))) return rc;
+    if ((rc = up(d_planes, s.plane_rec.data(), s.plane_rec.size() * 16))) return rc;
+    if ((rc = up(d_subs, s.sub_rec.data(), s.sub_rec.size() * 16))) return rc;
+    if ((rc = up(d_nodes, s.node_rec.data(), s.node_rec.size() * 16))) return rc;
+    if ((rc = up(d_lin, s.lin.data(), s.lin.size() * 4))) return rc;
+    if ((rc = up(d_quad, s.quad_rec.data(), s.quad_rec.size() * 16))) return rc;
+    if ((rc = up(d_sph, s.sph_rec.data(), s.sph_rec.size() * 16))) return rc;
+    if ((rc = up(d_pnt, s.pnt_rec.data(), s.pnt_rec.size() * 16))) return rc;
+    if ((rc = up(d_mat, s.mat_rec.data(), s.mat_rec.size() * 16))) return rc;
+    scene = std::move(s);
+    surfels.clear();
+    surfels_dirty = true;
+    have_transfer = have_solve = false;
+    return 0;
+}
+
+int Engine::set_surfels(const std::vector<HSurfel>& s, std::string& err) {
+    if (s.size() >= (1ull << 31)) {
+        err = "too many surfels";
+        return -1;
+    }
+    surfels = s;
+    surfels_dirty = true;
+    have_transfer = have_solve = false;
+    (void)err;
+    return 0;
+}
+
+int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
+    if (int rc0 = need_device(err)) return rc0;
+    if (res < 1 || res > 65535) {
+        err = "cluster resolution must be in 1..65535";
+        return -1;
+    }
+    CK(hipSetDevice(device_));
+    const uint32_t N = (uint32_t)surfels.size();
+    have_transfer = have_solve = false;
+    out = TransferOut();
+    // H2D of the surfel records (part of the transfer stage).
+    if (surfels_dirty) {
+        std::vector<float4> hp(N), hn(N), ha(N);
+        for (uint32_t i = 0; i < N; ++i) {
+            hp[i] = st4(surfels[i].p, surfels[i].area);
+            hn[i] = st4(surfels[i].n);
+            ha[i] = st4(surfels[i].albedo);
+        }
+        CK(d_spos.ensure(16ull * N));
+        CK(d_snrm.ensure(16ull * N));
+        CK(d_salb.ensure(16ull * N));
+        if (N) {
+            CK(hipMemcpyAsync(d_spos.p, hp.data(), 16ull * N, hipMemcpyHostToDevice, stream));
+            CK(hipMemcpyAsync(d_snrm.p, hn.data(), 16ull * N, hipMemcpyHostToDevice, stream));
+            CK(hipMemcpyAsync(d_salb.p, ha.data(), 16ull * N, hipMemcpyHostToDevice, stream));
+            CK(hipStreamSynchronize(stream));
+        }
+        surfels_dirty = false;
+    }
+    unsigned long long* ctr = d_counter.as<unsigned long long>();
+    CK(hipMemsetAsync(ctr, 0, 64, stream));
+    n_clusters = 0;
+    n_links = 0;
+    if (N == 0) {
+        have_transfer = true;
+        return 0;
+    }
+    const float4* spos = d_spos.as<float4>();
+    const float4* snrm = d_snrm.as<float4>();
+
+    // ---- clusters: key, stable sort, segments, serial sums, compaction
+    const f3 lo = scene.lo, ext = scene.hi - scene.lo;
+    const f3 inv = mk((float)res / fmx(ext.x, 1e-6f), (float)res / fmx(ext.y, 1e-6f), (float)res / fmx(ext.z, 1e-6f));
+    const uint64_t rr = (uint64_t)res;
+    const uint32_t key_bits = bits_for(rr * rr * rr * 6ull - 1ull);
+    CK(d_ckey.ensure(8ull * N));
+    CK(d_ckey2.ensure(8ull * N));
+    CK(d_cval.ensure(4ull * N));
+    CK(d_cval2.ensure(4ull * N));
+    CK(d_cflag.ensure(4ull * (N + 1)));
+    CK(d_cseg.ensure(4ull * (N + 1)));
+    launch("cluster_keys", [&] {
+        k_cluster_keys<<<blocks_for(N, 256), 256, 0, stream>>>(spos, snrm, N, lo, inv, res, d_ckey.as<uint64_t>(),
+                                                               d_cval.as<uint32_t>());
+    });
+    size_t tmp = 0;
+    CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_ckey.as<uint64_t>(), d_ckey2.as<uint64_t>(),
+                                          d_cval.as<uint32_t>(), d_cval2.as<uint32_t>(), (int)N, 0, (int)key_bits,
+                                          stream));
+    CK(d_tmp.ensure(tmp));
+    CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_ckey.as<uint64_t>(), d_ckey2.as<uint64_t>(),
+                                          d_cval.as<uint32_t>(), d_cval2.as<uint32_t>(), (int)N, 0, (int)key_bits,
+                                          stream));
+    const uint32_t* members = d_cval2.as<uint32_t>();
+    launch("seg_flags", [&] {
+        k_seg_flags<<<blocks_for(N, 256), 256, 0, stream>>>(d_ckey2.as<uint64_t>(), N, d_cflag.as<uint32_t>());
+    });
+    tmp = 0;
+    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_cflag.as<uint32_t>(), d_cseg.as<uint32_t>(), (int)N, stream));
+    CK(d_tmp.ensure(tmp));
+    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_cflag.as<uint32_t>(), d_cseg.as<uint32_t>(), (int)N, stream));
+    uint32_t tail[2];
+    CK(hipMemcpyAsync(&tail[0], d_cseg.as<uint32_t>() + N - 1, 4, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(&tail[1], d_cflag.as<uint32_t>() + N - 1, 4, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    const uint32_t nseg = tail[0] + tail[1];
+    DevBuf& seg_begin = d_ckey;  // reuse: the unsorted keys are dead
+    CK(d_cpos.ensure(16ull * nseg));
+    CK(d_cnrm.ensure(16ull * nseg));
+    CK(d_crange.ensure(8ull * nseg));
+    CK(d_cvalid.ensure(4ull * (nseg + 1)));
+    CK(d_cidx.ensure(4ull * (nseg + 1)));
+    DevBuf& raw_pos = d_craw_pos;
+    DevBuf& raw_nrm = d_craw_nrm;
+    CK(raw_pos.ensure(16ull * nseg));
+    CK(raw_nrm.ensure(16ull * nseg));
+    launch("seg_begin", [&] {
+        k_seg_begin<<<blocks_for(N, 256), 256, 0, stream>>>(d_cflag.as<uint32_t>(), d_cseg.as<uint32_t>(), N,
+                                                            seg_begin.as<uint32_t>());
+    });
+    launch("cluster_build", [&] {
+        k_cluster_build<<<blocks_for(nseg, 64), 64, 0, stream>>>(seg_begin.as<uint32_t>(), nseg, members, spos, snrm,
+                                                                 raw_pos.as<float4>(), raw_nrm.as<float4>(),
+                                                                 d_cvalid.as<uint32_t>());
+    });
+    tmp = 0;
+    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_cvalid.as<uint32_t>(), d_cidx.as<uint32_t>(), (int)nseg,
+                                        stream));
+    CK(d_tmp.ensure(tmp));
+    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_cvalid.as<uint32_t>(), d_cidx.as<uint32_t>(), (int)nseg,
+                                        stream));
+    launch("cluster_compact", [&] {
+        k_cluster_compact<<<blocks_for(nseg, 256), 256, 0, stream>>>(
+            nseg, d_cvalid.as<uint32_t>(), d_cidx.as<uint32_t>(), seg_begin.as<uint32_t>(), raw_pos.as<float4>(),
+            raw_nrm.as<float4>(), d_cpos.as<float4>(), d_cnrm.as<float4>(), d_crange.as<uint2>());
+    });
+    CK(hipMemcpyAsync(&tail[0], d_cidx.as<uint32_t>() + nseg - 1, 4, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(&tail[1], d_cvalid.as<uint32_t>() + nseg - 1, 4, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    const uint32_t C = tail[0] + tail[1];
+    n_clusters = C;
+    out.clusters = C;
+
+    // ---- candidates: count, scan, emit
+    if ((uint64_t)N * C >= (1ull << 32)) {
+        err = "surfels x clusters exceeds 2^32 candidate pairs";
+        return -6;
+    }
+    CK(d_ccount.ensure(4ull * (N + 1)));
+    CK(d_cstart.ensure(4ull * (N + 1)));
+    CK(hipMemsetAsync(d_ccount.as<uint32_t>() + N, 0, 4, stream));
+    const uint32_t ff_blocks = std::min<uint32_t>(blocks_for(N, 4), 8192);
+    launch("ff_count", [&] {
+        k_ff<false><<<ff_blocks, kBlock, 0, stream>>>(spos, snrm, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
+                                                       d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr);
+    });
+    tmp = 0;
+    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_ccount.as<uint32_t>(), d_cstart.as<uint32_t>(), (int)(N + 1),
+                                        stream));
+    CK(d_tmp.ensure(tmp));
+    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_ccount.as<uint32_t>(), d_cstart.as<uint32_t>(), (int)(N + 1),
+                                        stream));
+    uint32_t M = 0;
+    CK(hipMemcpyAsync(&M, d_cstart.as<uint32_t>() + N, 4, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    out.candidates = M;
+    if (M >= (1u << 31)) {
+        err = "candidate count exceeds 2^31";
+        return -6;
+    }
+    CK(d_cand_ci.ensure(4ull * M));
+    CK(d_cand_row.ensure(4ull * M));
+    CK(d_cand_v.ensure(4ull * M));
+    launch("ff_emit", [&] {
+        k_ff<true><<<ff_blocks, kBlock, 0, stream>>>(spos, snrm, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
+                                                      nullptr, d_cstart.as<uint32_t>(), d_cand_ci.as<uint32_t>(),
+                                                      d_cand_row.as<uint32_t>());
+    });
+
+    // ---- visibility and closure
+    const float diag = scene.diag();
+    const SceneView S = dev_view();
+    if (M) {
+        const uint32_t vis_blocks = std::min<uint32_t>(blocks_for(M, kBlock), 65535u * 4u);
+        launch("visibility", [&] {
+            k_visibility<<<vis_blocks, kBlock, 0, stream>>>(S, spos, snrm, d_cpos.as<float4>(), d_cnrm.as<float4>(),
+                                                            d_cand_ci.as<uint32_t>(), d_cand_row.as<uint32_t>(), M,
+                                                            diag * 5e-4f, diag * 1e-3f, d_cand_v.as<float>(), ctr);
+        });
+    }
+    CK(d_sky.ensure(4ull * N));
+    launch("closure", [&] {
+        k_closure<<<blocks_for(16ull * N, kBlock), kBlock, 0, stream>>>(S, spos, snrm, N, d_dirs.as<float4>(),
+                                                                         diag * 2.0f, d_sky.as<float>(), ctr);
+    });
+
+    // ---- calibration, link scan, compaction
+    CK(d_lcount.ensure(4ull * (N + 1)));
+    CK(d_lstart.ensure(4ull * (N + 1)));
+    CK(d_scale.ensure(4ull * N));
+    CK(d_rowsum.ensure(4ull * N));
+    CK(hipMemsetAsync(d_lcount.as<uint32_t>() + N, 0, 4, stream));
+    launch("row_sum", [&] {
+        k_row_sum<<<blocks_for(N, 256), 256, 0, stream>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
+                                                          d_sky.as<float>(), d_lcount.as<uint32_t>(),
+                                                          d_scale.as<float>(), d_rowsum.as<float>());
+    });
+    tmp = 0;
+    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_lcount.as<uint32_t>(), d_lstart.as<uint32_t>(), (int)(N + 1),
+                                        stream));
+    CK(d_tmp.ensure(tmp));
+    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_lcount.as<uint32_t>(), d_lstart.as<uint32_t>(), (int)(N + 1),
+                                        stream));
+    uint32_t L = 0;
+    CK(hipMemcpyAsync(&L, d_lstart.as<uint32_t>() + N, 4, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    CK(d_idx.ensure(4ull * L));
+    CK(d_w.ensure(4ull * L));
+    launch("compact", [&] {
+        k_compact<<<blocks_for(N, 256), 256, 0, stream>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
+                                                          d_cand_ci.as<uint32_t>(), d_lstart.as<uint32_t>(),
+                                                          d_scale.as<float>(), d_idx.as<uint32_t>(), d_w.as<float>());
+    });
+    // mean_row_sum: serial f32 sum over rows with links, in row order.
+    std::vector<float> rs(N);
+    unsigned long long ev = 0;
+    CK(hipMemcpyAsync(rs.data(), d_rowsum.p, 4ull * N, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(&ev, ctr, 8, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    drain_events();
+    float acc = -0.0f;
+    uint64_t interior = 0;
+    for (uint32_t i = 0; i < N; ++i)
+        if (rs[i] >= 0.0f) acc += rs[i], ++interior;
+    out.mean_row_sum = interior ? acc / (float)interior : 0.0f;
+    out.links = L;
+    out.evals = ev;
+    n_links = L;
+    have_transfer = true;
+    return 0;
+}
+
+int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
+    if (!have_transfer) {
+        err = "solve called before transfer";
+        return -3;
+    }
+    CK(hipSetDevice(device_));
+    out = SolveOut();
+    const uint32_t N = (uint32_t)surfels.size();
+    CK(d_direct.ensure(16ull * N));
+    CK(d_irr.ensure(16ull * N));
+    CK(d_rad.ensure(16ull * N));
+    CK(d_bc.ensure(16ull * (n_clusters + 1)));
+    CK(d_resid.ensure(4ull * (bounces + 1)));
+    unsigned long long* ctr = d_counter.as<unsigned long long>();
+    CK(hipMemsetAsync(ctr, 0, 64, stream));
+    CK(hipMemsetAsync(d_resid.p, 0, 4ull * (bounces + 1), stream));
+    const SceneView S = dev_view();
+    const LightsView Lv = dev_lights();
+    if (N) {
+        launch("direct_e", [&] {
+            k_direct_e<<<blocks_for(N, kBlock), kBlock, 0, stream>>>(
+                S, Lv, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, d_direct.as<float4>(),
+                d_rad.as<float4>(), d_irr.as<float4>(), ctr);
+        });
+    }
+    const uint32_t C = n_clusters;
+    if (C > 0 && N > 0) {
+        uint32_t* resid = d_resid.as<uint32_t>();
+        for (uint32_t it = 0; it < bounces; ++it) {
+            launch("aggregate", [&] {
+                k_aggregate<<<blocks_for(C, 64), 64, 0, stream>>>(it, resid, C, d_crange.as<uint2>(),
+                                                                  d_cval2.as<uint32_t>(), d_spos.as<float4>(),
+                                                                  d_cpos.as<float4>(), d_rad.as<float4>(),
+                                                                  d_bc.as<float4>());
+            });
+            launch("spmv", [&] {
+                k_spmv<<<blocks_for(N, kBlock), kBlock, 0, stream>>>(
+                    it, resid, N, d_lstart.as<uint32_t>(), d_idx.as<uint32_t>(), d_w.as<float>(), d_bc.as<float4>(),
+                    d_direct.as<float4>(), d_salb.as<float4>(), d_rad.as<float4>(), d_irr.as<float4>());
+            });
+        }
+    }
+    std::vector<uint32_t> res(bounces + 1, 0);
+    unsigned long long ev = 0;
+    CK(hipMemcpyAsync(res.data(), d_resid.p, 4ull * bounces, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(&ev, ctr, 8, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    drain_events();
+    out.evals = ev;
+    if (C > 0 && N > 0) {
+        out.iterations = bounces;
+        for (uint32_t it = 0; it < bounces; ++it) {
+            if (bitsf(res[it]) < 1e-4f) {
+                out.iterations = it + 1;
+                break;
+            }
+        }
+        out.residual = bitsf(res[out.iterations - 1]);
+    }
+    have_solve = true;
+    return 0;
+}
+
+int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, float* film, uint8_t* rgb8,
+                  uint64_t& evals, std::string& err) {
+    if (int rc0 = need_device(err)) return rc0;
+    if (mode < 0 || mode > 5) {
+        err = "unknown shading mode";
+        return -1;
+    }
+    HCfg c = scene.cfg;
+    c.width = W;
+    c.height = H;
+    if (!validate_cfg(c, err)) return -1;
+    CK(hipSetDevice(device_));
+    const bool needs_gi = mode == M_BEAUTY || mode == M_INDIRECT;
+    if (needs_gi && !have_solve) {
+        err = "beauty / indirect shading needs a solve first";
+        return -3;
+    }
+    const uint32_t N = (uint32_t)surfels.size();
+    const float diag = scene.diag();
+    unsigned long long* ctr = d_counter.as<unsigned long long>();
+    CK(hipMemsetAsync(ctr, 0, 64, stream));
+
+    ShadeArgs A;
+    // Camera basis (src/main.rs:199-211), evaluated on the host like the reference.
+    A.cam.origin = cam.pos;
+    A.cam.fwd = nrm3(cam.look - cam.pos);
+    A.cam.right = nrm3(cross3(A.cam.fwd, cam.up));
+    A.cam.up = cross3(A.cam.right, A.cam.fwd);
+    const float rads_per_deg = kPi / 180.0f;
+    A.cam.tan_half = tanf(cam.fov * rads_per_deg * 0.5f);
+    A.cam.aspect = (float)W / (float)H;
+    A.W = W;
+    A.H = H;
+    A.mode = mode;
+    A.use_gi = needs_gi ? 1 : 0;
+    A.tmax = diag * 4.0f;
+    A.radius = (diag / sqrtf((float)scene.cfg.surfels)) * 2.6f;
+    A.spos = d_spos.as<float4>();
+    A.snrm = d_snrm.as<float4>();
+    A.irr = d_irr.as<float4>();
+    A.grid.lo = scene.lo;
+    A.grid.inv_cell = 1.0f;
+    A.grid.dim[0] = A.grid.dim[1] = A.grid.dim[2] = 1;
+    A.grid.starts = nullptr;
+    A.grid.items = nullptr;
+
+    if (needs_gi) {
+        // Grid::build (src/surfel.rs:71-114): cell keys + counts, scan, stable sort.
+        const float inv_cell = 1.0f / A.radius;
+        const f3 ext = scene.hi - scene.lo;
+        int32_t dx = clampi(sat_i32(ceilf(ext.x * inv_cell)), 1, 512);
+        int32_t dy = clampi(sat_i32(ceilf(ext.y * inv_cell)), 1, 512);
+        int32_t dz = clampi(sat_i32(ceilf(ext.z * inv_cell)), 1, 512);
+        uint64_t ncells = (uint64_t)dx * dy * dz;
+        CK(d_gkey.ensure(4ull * N));
+        CK(d_gkey2.ensure(4ull * N));
+        CK(d_gval.ensure(4ull * N));
+        CK(d_gval2.ensure(4ull * N));
+        CK(d_gstart.ensure(4ull * (ncells + 1)));
+        DevBuf& counts = d_cand_row;  // reuse a transfer scratch array
+        CK(counts.ensure(4ull * (ncells + 1)));
+        CK(hipMemsetAsync(counts.p, 0, 4ull * (ncells + 1), stream));
+        if (N) {
+            launch("grid_keys", [&] {
+                k_grid_keys<<<blocks_for(N, 256), 256, 0, stream>>>(A.spos, N, scene.lo, inv_cell, dx, dy, dz,
+                                                                    d_gkey.as<uint32_t>(), d_gval.as<uint32_t>(),
+                                                                    counts.as<uint32_t>());
+            });
+        }
+        size_t tmp = 0;
+        CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, counts.as<uint32_t>(), d_gstart.as<uint32_t>(),
+                                            (int)(ncells + 1), stream));
+        CK(d_tmp.ensure(tmp));
+        CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, counts.as<uint32_t>(), d_gstart.as<uint32_t>(),
+                                            (int)(ncells + 1), stream));
+        if (N) {
+            const int kb = (int)bits_for(ncells - 1 ? ncells - 1 : 1);
+            tmp = 0;
+            CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_gkey.as<uint32_t>(), d_gkey2.as<uint32_t>(),
+                                                  d_gval.as<uint32_t>(), d_gval2.as<uint32_t>(), (int)N, 0, kb,
+                                                  stream));
+            CK(d_tmp.ensure(tmp));
+            CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_gkey.as<uint32_t>(), d_gkey2.as<uint32_t>(),
+                                                  d_gval.as<uint32_t>(), d_gval2.as<uint32_t>(), (int)N, 0, kb,
+                                                  stream));
+        }
+        A.grid.inv_cell = inv_cell;
+        A.grid.dim[0] = dx;
+        A.grid.dim[1] = dy;
+        A.grid.dim[2] = dz;
+        A.grid.starts = d_gstart.as<uint32_t>();
+        A.grid.items = d_gval2.as<uint32_t>();
+    }
+    const uint64_t npx = (uint64_t)W * H;
+    CK(d_film.ensure(16ull * npx));
+    CK(d_rgb8.ensure(3ull * npx));
+    const SceneView S = dev_view();
+    const LightsView Lv = dev_lights();
+    dim3 grid((W + 15) / 16, (H + 15) / 16);
+    launch("shade", [&] { k_shade<<<grid, kBlock, 0, stream>>>(S, Lv, A, d_film.as<float4>(), ctr); });
+    const bool tonemapped = mode <= M_INDIRECT;
+    if (rgb8) {
+        launch("encode", [&] {
+            k_encode<<<blocks_for(npx, 256), 256, 0, stream>>>(d_film.as<float4>(), (uint32_t)npx,
+                                                               scene.cfg.exposure, tonemapped ? 1 : 0,
+                                                               d_srgb.as<float>(), d_rgb8.as<uint8_t>());
+        });
+        CK(hipMemcpyAsync(rgb8, d_rgb8.p, 3ull * npx, hipMemcpyDeviceToHost, stream));
+    }
+    if (film) {
+        CK(hipMemcpy2DAsync(film, 12, d_film.p, 16, 12, npx, hipMemcpyDeviceToHost, stream));
+    }
+    unsigned long long ev = 0;
+    CK(hipMemcpyAsync(&ev, ctr, 8, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    drain_events();
+    evals = ev;
+    return 0;
+}
+
+// ---------------------------------------------------------------------- probes / downloads
+int Engine::probe_dist(const float* pts, uint64_t n, float* d, uint32_t* mat, float* nrm, std::string& err) {
+    if (int rc0 = need_device(err)) return rc0;
+    CK(hipSetDevice(device_));
+    if (n == 0) return 0;
+    CK(d_probe_in.ensure(12ull * n));
+    CK(d_probe_out.ensure(20ull * n));
+    CK(hipMemcpyAsync(d_probe_in.p, pts, 12ull * n, hipMemcpyHostToDevice, stream));
+    float* od = d_probe_out.as<float>();
+    uint32_t* om = (uint32_t*)(od + n);
+    float* on = nrm ? od + 2 * n : nullptr;
+    const SceneView S = dev_view();
+    launch("probe_dist", [&] {
+        k_probe_dist<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(S, d_probe_in.as<float>(), (uint32_t)n, od, om, on);
+    });
+    CK(hipMemcpyAsync(d, od, 4ull * n, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(mat, om, 4ull * n, hipMemcpyDeviceToHost, stream));
+    if (nrm) CK(hipMemcpyAsync(nrm, on, 12ull * n, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    drain_events();
+    return 0;
+}
+
+int Engine::probe_trace(const float* ro, const float* rd, uint64_t n, float tmin, float tmax, float* t, float* nrm,
+                        uint32_t* mat, int32_t* hit, uint32_t* steps, std::string& err) {
+    if (int rc0 = need_device(err)) return rc0;
+    CK(hipSetDevice(device_));
+    if (n == 0) return 0;
+    CK(d_probe_in.ensure(24ull * n));
+    CK(d_probe_out.ensure(28ull * n));
+    float* in = d_probe_in.as<float>();
+    CK(hipMemcpyAsync(in, ro, 12ull * n, hipMemcpyHostToDevice, stream));
+    CK(hipMemcpyAsync(in + 3 * n, rd, 12ull * n, hipMemcpyHostToDevice, stream));
+    float* o = d_probe_out.as<float>();
+    float* ot = o;
+    float* on = o + n;
+    uint32_t* om = (uint32_t*)(o + 4 * n);
+    int32_t* oh = (int32_t*)(o + 5 * n);
+    uint32_t* os = (uint32_t*)(o + 6 * n);
+    const SceneView S = dev_view();
+    launch("probe_trace", [&] {
+        k_probe_trace<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(S, in, in + 3 * n, (uint32_t)n, tmin, tmax, ot, on,
+                                                                    om, oh, os);
+    });
+    CK(hipMemcpyAsync(t, ot, 4ull * n, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(nrm, on, 12ull * n, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(mat, om, 4ull * n, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(hit, oh, 4ull * n, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(steps, os, 4ull * n, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    drain_events();
+    return 0;
+}
+
+int Engine::probe_shadow(const float* ro, const float* rd, uint64_t n, float tmin, const float* tmax, const float* k,
+                         float* vis, uint32_t* steps, std::string& err) {
+    if (int rc0 = need_device(err)) return rc0;
+    CK(hipSetDevice(device_));
+    if (n == 0) return 0;
+    CK(d_probe_in.ensure(32ull * n));
+    CK(d_probe_out.ensure(8ull * n));
+    float* in = d_probe_in.as<float>();
+    CK(hipMemcpyAsync(in, ro, 12ull * n, hipMemcpyHostToDevice, stream));
+    CK(hipMemcpyAsync(in + 3 * n, rd, 12ull * n, hipMemcpyHostToDevice, stream));
+    CK(hipMemcpyAsync(in + 6 * n, tmax, 4ull * n, hipMemcpyHostToDevice, stream));
+    CK(hipMemcpyAsync(in + 7 * n, k, 4ull * n, hipMemcpyHostToDevice, stream));
+    float* ov = d_probe_out.as<float>();
+    uint32_t* os = (uint32_t*)(ov + n);
+    const SceneView S = dev_view();
+    launch("probe_shadow", [&] {
+        k_probe_shadow<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(S, in, in + 3 * n, (uint32_t)n, tmin, in + 6 * n,
+                                                                     in + 7 * n, ov, os);
+    });
+    CK(hipMemcpyAsync(vis, ov, 4ull * n, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(steps, os, 4ull * n, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    drain_events();
+    return 0;
+}
+
+int Engine::download_transfer(uint32_t* starts, uint32_t* idx, float* w, float* sky, float* row_sum,
+                              std::string& err) {
+    if (!have_transfer) {
+        err = "no transfer matrix";
+        return -3;
+    }
+    const uint64_t N = surfels.size();
+    if (N == 0) {
+        if (starts) starts[0] = 0;
+        return 0;
+    }
+    if (starts) CK(hipMemcpyAsync(starts, d_lstart.p, 4 * (N + 1), hipMemcpyDeviceToHost, stream));
+    if (idx && n_links) CK(hipMemcpyAsync(idx, d_idx.p, 4 * n_links, hipMemcpyDeviceToHost, stream));
+    if (w && n_links) CK(hipMemcpyAsync(w, d_w.p, 4 * n_links, hipMemcpyDeviceToHost, stream));
+    if (sky) CK(hipMemcpyAsync(sky, d_sky.p, 4 * N, hipMemcpyDeviceToHost, stream));
+    if (row_sum) CK(hipMemcpyAsync(row_sum, d_rowsum.p, 4 * N, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+int Engine::download_clusters(float* p, float* n, float* area, float* radius, uint32_t* mstart, uint32_t* members,
+                              std::string& err) {
+    if (!have_transfer) {
+        err = "no clusters";
+        return -3;
+    }
+    const uint32_t C = n_clusters;
+    const uint64_t N = surfels.size();
+    std::vector<float4> hp(C), hn(C);
+    std::vector<uint2> hr(C);
+    std::vector<uint32_t> mem(N);
+    if (C) {
+        CK(hipMemcpyAsync(hp.data(), d_cpos.p, 16ull * C, hipMemcpyDeviceToHost, stream));
+        CK(hipMemcpyAsync(hn.data(), d_cnrm.p, 16ull * C, hipMemcpyDeviceToHost, stream));
+        CK(hipMemcpyAsync(hr.data(), d_crange.p, 8ull * C, hipMemcpyDeviceToHost, stream));
+        CK(hipMemcpyAsync(mem.data(), d_cval2.p, 4ull * N, hipMemcpyDeviceToHost, stream));
+    }
+    CK(hipStreamSynchronize(stream));
+    uint32_t off = 0;
+    for (uint32_t c = 0; c < C; ++c) {
+        if (p) p[3 * c] = hp[c].x, p[3 * c + 1] = hp[c].y, p[3 * c + 2] = hp[c].z;
+        if (n) n[3 * c] = hn[c].x, n[3 * c + 1] = hn[c].y, n[3 * c + 2] = hn[c].z;
+        if (area) area[c] = hp[c].w;
+        if (radius) radius[c] = hn[c].w;
+        if (mstart) mstart[c] = off;
+        for (uint32_t k = hr[c].x; k < hr[c].y; ++k) {
+            if (members) members[off] = mem[k];
+            ++off;
+        }
+    }
+    if (mstart) mstart[C] = off;
+    return 0;
+}
+
+int Engine::download_radiosity(float* direct, float* irr, float* rad, std::string& err) {
+    if (!have_solve) {
+        err = "no solve";
+        return -3;
+    }
+    const uint64_t N = surfels.size();
+    if (!N) return 0;
+    if (direct) CK(hipMemcpy2DAsync(direct, 12, d_direct.p, 16, 12, N, hipMemcpyDeviceToHost, stream));
+    if (irr) CK(hipMemcpy2DAsync(irr, 12, d_irr.p, 16, 12, N, hipMemcpyDeviceToHost, stream));
+    if (rad) CK(hipMemcpy2DAsync(rad, 12, d_rad.p, 16, 12, N, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+int Engine::upload_radiosity(const float* direct, const float* irr, const float* rad, std::string& err) {
+    if (int rc0 = need_device(err)) return rc0;
+    const uint64_t N = surfels.size();
+    CK(d_direct.ensure(16ull * N));
+    CK(d_irr.ensure(16ull * N));
+    CK(d_rad.ensure(16ull * N));
+    if (N) {
+        CK(hipMemsetAsync(d_direct.p, 0, 16ull * N, stream));
+        CK(hipMemsetAsync(d_irr.p, 0, 16ull * N, stream));
+        CK(hipMemsetAsync(d_rad.p, 0, 16ull * N, stream));
+        if (direct) CK(hipMemcpy2DAsync(d_direct.p, 16, direct, 12, 12, N, hipMemcpyHostToDevice, stream));
+        if (irr) CK(hipMemcpy2DAsync(d_irr.p, 16, irr, 12, 12, N, hipMemcpyHostToDevice, stream));
+        if (rad) CK(hipMemcpy2DAsync(d_rad.p, 16, rad, 12, 12, N, hipMemcpyHostToDevice, stream));
+    }
+    CK(hipStreamSynchronize(stream));
+    have_solve = true;
+    return 0;
+}
+
+}  // namespace ifx

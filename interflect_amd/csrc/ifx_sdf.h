@@ -1,0 +1,294 @@
+// Device scene layout and the scene distance field: primitive and CSG object
+// distances, BVH nearest-distance traversal, scene distance with material
+// attribution and the tetrahedral normal.  Shared by the gfx950 kernels and
+// the host-side surfel generator (one implementation, two targets).
+//
+// Follows the reference's src/sdf.rs:50-173, src/bvh.rs:45-163 and
+// src/scene.rs:187-236: the traversal order, the `>=` prune and the strict
+// `<` material update are kept exactly, so the same point yields the same
+// (distance, material) bits as the reference.
+#pragma once
+#include "ifx_math.h"
+
+namespace ifx {
+
+constexpr float kFltMax = 3.40282346638528859812e+38f;
+
+enum PrimKind : uint32_t { K_SPHERE = 0, K_ROUNDBOX = 1, K_PLANE = 2, K_CAPSULE = 3, K_TORUS = 4, K_CYLINDER = 5, K_CONE = 6 };
+
+IFX_HD uint32_t fbits(float f) {
+    union {
+        float f;
+        uint32_t u;
+    } x;
+    x.f = f;
+    return x.u;
+}
+IFX_HD float bitsf(uint32_t u) {
+    union {
+        float f;
+        uint32_t u;
+    } x;
+    x.u = u;
+    return x.f;
+}
+
+// One primitive or CSG object: 4 x float4 (64 B).
+//   g0 = (anchor.xyz, s0)   anchor = centre / endpoint a / plane normal
+//   g1 = (vec.xyz,    s1)   vec    = box half-size minus round / b - a
+//   g2 = (s2, s3, kind, 0)
+//   g3 = (mat, sub_begin, sub_count, blended) as u32 bits (objects only)
+// Per kind: sphere (c, r); roundbox (c, r | b - r); plane (n, h);
+// capsule / cylinder (a, r | b - a, |b - a|^2); torus (c, major | -, minor);
+// cone (a, ra | b - a, |b - a|^2 | rb, rb - ra).  The precomputed terms are
+// the very f32 expressions the reference evaluates per call.
+struct ObjRec {
+    float4 g0, g1, g2, g3;
+};
+
+// BVH node: 2 x float4.  lo.w = leaf start or interior right-child index,
+// hi.w = leaf object count (0 = interior); the left child is node + 1.
+struct SceneView {
+    const float4* obj;     // bounded objects, stored in BVH leaf order
+    const float4* planes;  // unbounded objects, always evaluated first
+    const float4* subs;    // CSG subtraction primitives
+    const float4* nodes;
+    const uint32_t* lin;   // linear-scan order: lin[i] = slot of object i
+    uint32_t n_obj, n_planes, n_nodes, use_bvh;
+    float blend_k;         // EXT blend-group radius, 0 = reference behaviour
+};
+
+struct DistHit {
+    float d;
+    uint32_t mat;
+};
+
+IFX_HD float prim_dist(float4 g0, float4 g1, float4 g2, f3 p) {
+    uint32_t kind = fbits(g2.z);
+    f3 a = ld3(g0);
+    switch (kind) {
+        case K_SPHERE:
+            return len3(p - a) - g0.w;
+        case K_ROUNDBOX: {
+            f3 q = abs3(p - a) - ld3(g1);
+            return (len3(max3(q, spl(0.0f))) + fmn(fmx(q.x, fmx(q.y, q.z)), 0.0f)) - g0.w;
+        }
+        case K_PLANE:
+            return dot3(p, a) - g0.w;
+        case K_CAPSULE: {
+            f3 pa = p - a;
+            f3 ba = ld3(g1);
+            float h = clmp(dot3(pa, ba) / g1.w, 0.0f, 1.0f);
+            return len3(pa - ba * h) - g0.w;
+        }
+        case K_TORUS: {
+            f3 q = p - a;
+            float xz = sqrtf(q.x * q.x + q.z * q.z) - g0.w;
+            return sqrtf(xz * xz + q.y * q.y) - g2.x;
+        }
+        case K_CYLINDER: {
+            f3 ba = ld3(g1);
+            float baba = g1.w;
+            float r = g0.w;
+            if (baba < 1e-12f) return len3(p - a) - r;
+            f3 pa = p - a;
+            float paba = dot3(pa, ba);
+            float x = len3(pa * baba - ba * paba) - r * baba;
+            float y = fabsf(paba - baba * 0.5f) - baba * 0.5f;
+            float x2 = x * x;
+            float y2 = y * y * baba;
+            float d = fmx(x, y) < 0.0f ? -fmn(x2, y2) : (x > 0.0f ? x2 : 0.0f) + (y > 0.0f ? y2 : 0.0f);
+            float sg = (fbits(d) >> 31) ? -1.0f : 1.0f;  // f32::signum, d is never NaN
+            return sg * sqrtf(fabsf(d)) / baba;
+        }
+        default: {  // K_CONE
+            f3 ba = ld3(g1);
+            float baba = g1.w;
+            float ra = g0.w, rb = g2.x, rba = g2.y;
+            if (baba < 1e-12f) return len3(p - a) - fmx(ra, rb);
+            f3 pa = p - a;
+            float papa = dot3(pa, pa);
+            float paba = dot3(pa, ba) / baba;
+            float x = sqrtf(fmx(papa - paba * paba * baba, 0.0f));
+            float cax = fmx(0.0f, x - (paba < 0.5f ? ra : rb));
+            float cay = fabsf(paba - 0.5f) - 0.5f;
+            float k = rba * rba + baba;
+            float f = clmp((rba * (x - ra) + paba * baba) / k, 0.0f, 1.0f);
+            float cbx = x - ra - f * rba;
+            float cby = paba - f;
+            float s = (cbx < 0.0f && cay < 0.0f) ? -1.0f : 1.0f;
+            return s * sqrtf(fmn(cax * cax + cay * cay * baba, cbx * cbx + cby * cby * baba));
+        }
+    }
+}
+
+// src/sdf.rs:164-173: base primitive, then each subtraction in order.
+IFX_HD float obj_dist(const float4* __restrict__ rec, const float4* __restrict__ subs, f3 p, uint32_t& mat,
+                      bool& blended) {
+    float4 g0 = rec[0], g1 = rec[1], g2 = rec[2], g3 = rec[3];
+    mat = fbits(g3.x);
+    uint32_t sb = fbits(g3.y), sc = fbits(g3.z);
+    blended = fbits(g3.w) != 0u;
+    float d = prim_dist(g0, g1, g2, p);
+    for (uint32_t s = sb; s < sb + sc; ++s) {
+        const float4* q = subs + 4 * s;
+        d = fmx(d, -prim_dist(q[0], q[1], q[2], p));
+    }
+    return d;
+}
+
+// src/bvh.rs:45-50
+IFX_HD float aabb_dist(f3 p, float4 lo, float4 hi) {
+    f3 d = max3(max3(ld3(lo) - p, p - ld3(hi)), spl(0.0f));
+    return len3(d);
+}
+
+// EXT: polynomial smooth minimum (Quilez), k > 0.
+IFX_HD float smin_poly(float a, float b, float k) {
+    float h = fmx(k - fabsf(a - b), 0.0f) / k;
+    return fmn(a, b) - h * h * k * 0.25f;
+}
+
+// Per-lane traversal stacks.  Host: a plain array.  Device: a column of a
+// block-wide LDS array (entry i of lane t at base[i * stride]), so a
+// divergent push / pop never spills to scratch memory.
+struct ArrStack {
+    uint32_t v[64];
+    IFX_HD void put(int i, uint32_t x) { v[i] = x; }
+    IFX_HD uint32_t get(int i) const { return v[i]; }
+};
+
+struct LdsStack {
+    uint16_t* base;
+    uint32_t stride;
+    __device__ __forceinline__ void put(int i, uint32_t x) { base[i * stride] = (uint16_t)x; }
+    __device__ __forceinline__ uint32_t get(int i) const { return base[i * stride]; }
+};
+
+// Blend-group state (EXT): nearest hard hit, nearest two blended objects.
+struct BlendState {
+    DistHit hard, b1;
+    float b2;
+};
+
+IFX_HD void blend_take(BlendState& bs, float d, uint32_t mat, bool blended) {
+    if (blended) {
+        if (d < bs.b1.d) {
+            bs.b2 = bs.b1.d;
+            bs.b1.d = d;
+            bs.b1.mat = mat;
+        } else if (d < bs.b2) {
+            bs.b2 = d;
+        }
+    } else if (d < bs.hard.d) {
+        bs.hard.d = d;
+        bs.hard.mat = mat;
+    }
+}
+
+// src/bvh.rs:97-150 (BLEND=false) and the EXT blend-group variant.
+template <bool BLEND, class Stk>
+IFX_HD void bvh_query(const SceneView& S, f3 p, BlendState& bs, Stk& stk) {
+    if (S.n_nodes == 0) return;
+    int sp = 0;
+    stk.put(sp++, 0u);
+    while (sp > 0) {
+        uint32_t ni = stk.get(--sp);
+        float4 lo = S.nodes[2 * ni], hi = S.nodes[2 * ni + 1];
+        float thr = BLEND ? fmn(bs.hard.d, smin_poly(bs.b1.d, bs.b2, S.blend_k)) + S.blend_k : bs.hard.d;
+        if (aabb_dist(p, lo, hi) >= thr) continue;
+        uint32_t w0 = fbits(lo.w), cnt = fbits(hi.w);
+        if (cnt > 0) {
+            for (uint32_t k = w0; k < w0 + cnt; ++k) {
+                uint32_t mat;
+                bool bl;
+                float d = obj_dist(S.obj + 4 * k, S.subs, p, mat, bl);
+                if (BLEND) {
+                    blend_take(bs, d, mat, bl);
+                } else if (d < bs.hard.d) {
+                    bs.hard.d = d;
+                    bs.hard.mat = mat;
+                }
+            }
+        } else {
+            uint32_t l = ni + 1, r = w0;
+            float dl = aabb_dist(p, S.nodes[2 * l], S.nodes[2 * l + 1]);
+            float dr = aabb_dist(p, S.nodes[2 * r], S.nodes[2 * r + 1]);
+            if (dl < dr) {
+                stk.put(sp++, r);
+                stk.put(sp++, l);
+            } else {
+                stk.put(sp++, l);
+                stk.put(sp++, r);
+            }
+        }
+    }
+}
+
+// src/scene.rs:191-208
+template <class Stk>
+IFX_HD DistHit dist_mat(const SceneView& S, f3 p, Stk& stk) {
+    BlendState bs;
+    bs.hard = DistHit{kFltMax, 0u};
+    bs.b1 = DistHit{kFltMax, 0u};
+    bs.b2 = kFltMax;
+    for (uint32_t i = 0; i < S.n_planes; ++i) {
+        uint32_t mat;
+        bool bl;
+        float d = obj_dist(S.planes + 4 * i, S.subs, p, mat, bl);
+        if (d < bs.hard.d) {
+            bs.hard.d = d;
+            bs.hard.mat = mat;
+        }
+    }
+    if (!(S.blend_k > 0.0f)) {
+        if (S.use_bvh) {
+            bvh_query<false>(S, p, bs, stk);
+        } else {
+            for (uint32_t i = 0; i < S.n_obj; ++i) {
+                uint32_t mat;
+                bool bl;
+                float d = obj_dist(S.obj + 4 * S.lin[i], S.subs, p, mat, bl);
+                if (d < bs.hard.d) {
+                    bs.hard.d = d;
+                    bs.hard.mat = mat;
+                }
+            }
+        }
+        return bs.hard;
+    }
+    if (S.use_bvh) {
+        bvh_query<true>(S, p, bs, stk);
+    } else {
+        for (uint32_t i = 0; i < S.n_obj; ++i) {
+            uint32_t mat;
+            bool bl;
+            float d = obj_dist(S.obj + 4 * S.lin[i], S.subs, p, mat, bl);
+            blend_take(bs, d, mat, bl);
+        }
+    }
+    float s = smin_poly(bs.b1.d, bs.b2, S.blend_k);
+    if (s < bs.hard.d) return DistHit{s, bs.b1.mat};
+    return bs.hard;
+}
+
+template <class Stk>
+IFX_HD float dist(const SceneView& S, f3 p, Stk& stk) {
+    return dist_mat(S, p, stk).d;
+}
+
+// src/scene.rs:218-236: tetrahedral 4-tap gradient.
+template <class Stk>
+IFX_HD f3 scene_normal(const SceneView& S, f3 p, Stk& stk) {
+    const float H = 5e-4f;
+    f3 n = spl(0.0f);
+    n = n + mk(1.0f, -1.0f, -1.0f) * dist(S, p + mk(1.0f, -1.0f, -1.0f) * H, stk);
+    n = n + mk(-1.0f, -1.0f, 1.0f) * dist(S, p + mk(-1.0f, -1.0f, 1.0f) * H, stk);
+    n = n + mk(-1.0f, 1.0f, -1.0f) * dist(S, p + mk(-1.0f, 1.0f, -1.0f) * H, stk);
+    n = n + mk(1.0f, 1.0f, 1.0f) * dist(S, p + mk(1.0f, 1.0f, 1.0f) * H, stk);
+    float l = len3(n);
+    if (l > 1e-12f) return n / l;
+    return mk(0.0f, 1.0f, 0.0f);
+}
+
+}  // namespace ifx

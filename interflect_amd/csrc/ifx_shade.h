@@ -1,0 +1,504 @@
+// Ray primitives and lighting shared by the gfx950 kernels (and the host
+// surfel generator for `trace`-free parts): sphere tracing, cone soft
+// shadows, quad form factors, direct light, emitter visibility and the
+// surfel irradiance gather.
+//
+// Follows src/trace.rs:27-148, src/ltc.rs:25-127, src/shade.rs:10-181 and
+// src/surfel.rs:127-140,262-293 of the reference, operation for operation.
+// EXT terms (sphere / point emitters) are marked and vanish on scenes that
+// have none.
+#pragma once
+#include "ifx_sdf.h"
+
+namespace ifx {
+
+constexpr uint32_t kMaxSteps = 128;    // src/trace.rs:10
+constexpr uint32_t kShadowSteps = 96;  // src/trace.rs:11
+
+// Quad light: 7 x float4 = verts[0..3], (emit, area), (centre, radius), (normal, -).
+// radius = max(sqrt(area / pi), 1e-4) and normal = normalize((v1-v0) x (v3-v0))
+// are the reference's per-call expressions, evaluated once on the host.
+// Sphere light (EXT): (centre, r), (emit, -).  Point light (EXT): (pos, -), (emit, -).
+// Material: (albedo, roughness), (emissive, metallic), (f0, -), (kd, -).
+struct LightsView {
+    const float4* quad;
+    const float4* sph;
+    const float4* pnt;
+    const float4* mat;
+    uint32_t n_quad, n_sph, n_pnt, n_mat;
+};
+
+struct Hit {
+    float t;
+    f3 p, n;
+    uint32_t mat;
+    bool hit;
+};
+
+// src/trace.rs:27-58.  NORMAL=false skips the 4-tap normal (closure rays
+// discard it); the step count still includes its 4 evaluations.
+template <bool NORMAL, class Stk>
+IFX_HD Hit trace(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, uint32_t& steps_out, Stk& stk) {
+    float t = tmin;
+    uint32_t steps = 0;
+    while (steps < kMaxSteps) {
+        f3 p = ro + rd * t;
+        DistHit h = dist_mat(S, p, stk);
+        steps += 1;
+        float eps = 1e-4f * fmx(t, 1.0f);
+        if (h.d < eps) {
+            Hit r;
+            r.t = t;
+            r.p = p;
+            r.n = NORMAL ? scene_normal(S, p, stk) : spl(0.0f);
+            r.mat = h.mat;
+            r.hit = true;
+            steps_out = steps + 4;
+            return r;
+        }
+        t += h.d;
+        if (t > tmax) break;
+    }
+    steps_out = steps;
+    Hit r;
+    r.t = 0.0f;
+    r.p = spl(0.0f);
+    r.n = spl(0.0f);
+    r.mat = 0;
+    r.hit = false;
+    return r;
+}
+
+// src/trace.rs:84-124
+template <class Stk>
+IFX_HD float soft_shadow(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, float k, uint32_t& steps_out,
+                         Stk& stk) {
+    float res = 1.0f;
+    float t = tmin;
+    uint32_t steps = 0;
+    float ph = 1e20f;
+    while (steps < kShadowSteps && t < tmax) {
+        float h = dist(S, ro + rd * t, stk);
+        steps += 1;
+        if (h < 1e-4f) {
+            steps_out = steps;
+            return 0.0f;
+        }
+        float y, d;
+        if (h < ph) {
+            y = h * h / (2.0f * ph);
+            d = sqrtf(fmx(h * h - y * y, 0.0f));
+        } else {
+            y = 0.0f;
+            d = h;
+        }
+        res = fmn(res, k * d / fmx(t - y, 1e-4f));
+        if (res < 1e-3f) {
+            steps_out = steps;
+            return 0.0f;
+        }
+        ph = h;
+        t += clmp(h, 0.006f, 0.12f);
+    }
+    steps_out = steps;
+    return clmp(res, 0.0f, 1.0f);
+}
+
+// src/trace.rs:128-148
+IFX_HD bool intersect_quad(const float4* L, f3 ro, f3 rd, float& t_out) {
+    f3 v0 = ld3(L[0]);
+    f3 e1 = ld3(L[1]) - v0;
+    f3 e2 = ld3(L[3]) - v0;
+    f3 n = cross3(e1, e2);
+    float denom = dot3(n, rd);
+    if (fabsf(denom) < 1e-9f) return false;
+    float t = dot3(v0 - ro, n) / denom;
+    if (t < 1e-4f) return false;
+    f3 p = (ro + rd * t) - v0;
+    float u = dot3(p, e1) / dot3(e1, e1);
+    float v = dot3(p, e2) / dot3(e2, e2);
+    if (0.0f <= u && u <= 1.0f && 0.0f <= v && v <= 1.0f) {
+        t_out = t;
+        return true;
+    }
+    return false;
+}
+
+// src/ltc.rs:88-96 (Duff et al. branchless frame)
+IFX_HD void frame(f3 n, f3& t, f3& b) {
+    float s = n.z >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (s + n.z);
+    float bb = n.x * n.y * a;
+    t = mk(1.0f + s * n.x * n.x * a, s * bb, -s * n.x);
+    b = mk(bb, s + n.y * n.y * a, -n.y);
+}
+
+// src/ltc.rs:25-83 on the quad given in world space (to_local, :108-115,
+// reverses the winding on the way in).
+IFX_HD float quad_ff(const float4* L, f3 p, f3 t, f3 b, f3 n) {
+    f3 l[4];
+    for (int i = 0; i < 4; ++i) {
+        f3 d = ld3(L[3 - i]) - p;
+        l[i] = mk(dot3(d, t), dot3(d, b), dot3(d, n));
+    }
+    f3 c[5];
+    int m = 0;
+    for (int i = 0; i < 4; ++i) {
+        f3 a = l[i];
+        f3 bb = l[(i + 1) & 3];
+        float da = a.z, db = bb.z;
+        if (da >= 0.0f) c[m++] = a;
+        if ((da >= 0.0f) != (db >= 0.0f)) {
+            float tt = da / (da - db);
+            c[m++] = a + (bb - a) * tt;
+        }
+        if (m >= 5) break;
+    }
+    if (m < 3) return 0.0f;
+    for (int i = 0; i < m; ++i) {
+        float len = len3(c[i]);
+        if (len > 1e-9f) c[i] = c[i] / len;
+    }
+    float sum = 0.0f;
+    for (int i = 0; i < m; ++i) {
+        f3 a = c[i];
+        f3 bb = c[i + 1 == m ? 0 : i + 1];
+        float cos_t = clmp(dot3(a, bb), -1.0f, 1.0f);
+        float theta = acosf(cos_t);
+        f3 cr = cross3(a, bb);
+        float len = len3(cr);
+        if (len > 1e-9f) sum += theta * (cr.z / len);
+    }
+    return fmx(sum * 0.15915494f, 0.0f);
+}
+
+// src/ltc.rs:119-127
+IFX_HD f3 closest_on_quad(const float4* L, f3 q) {
+    f3 o = ld3(L[0]);
+    f3 e1 = ld3(L[1]) - o;
+    f3 e2 = ld3(L[3]) - o;
+    f3 d = q - o;
+    float u = clmp(dot3(d, e1) / dot3(e1, e1), 0.0f, 1.0f);
+    float v = clmp(dot3(d, e2) / dot3(e2, e2), 0.0f, 1.0f);
+    return (o + e1 * u) + e2 * v;
+}
+
+// src/shade.rs:10-31
+IFX_HD f3 fresnel_schlick(float cos_theta, f3 f0) {
+    float m = clmp(1.0f - cos_theta, 0.0f, 1.0f);
+    float m2 = m * m;
+    return f0 + (spl(1.0f) - f0) * (m2 * m2 * m);
+}
+IFX_HD float d_ggx(float n_dot_h, float alpha) {
+    float a2 = alpha * alpha;
+    float d = n_dot_h * n_dot_h * (a2 - 1.0f) + 1.0f;
+    return a2 / fmx(kPi * d * d, 1e-9f);
+}
+IFX_HD float v_smith(float n_dot_v, float n_dot_l, float alpha) {
+    float a2 = alpha * alpha;
+    float gv = n_dot_l * sqrtf(n_dot_v * n_dot_v * (1.0f - a2) + a2);
+    float gl = n_dot_v * sqrtf(n_dot_l * n_dot_l * (1.0f - a2) + a2);
+    return 0.5f / fmx(gv + gl, 1e-9f);
+}
+
+// EXT: horizon-clipped sphere form factor (Snyder 1996; Lagarde & de
+// Rousiers 2014, "Moving Frostbite to PBR", sec. 4.7.2), divided by pi.
+IFX_HD float sphere_ff(float cos_t, float s2) {
+    float ill;
+    if (cos_t * cos_t > s2) {
+        ill = kPi * s2 * clmp(cos_t, 0.0f, 1.0f);
+    } else {
+        float sin_t = sqrtf(fmx(1.0f - cos_t * cos_t, 0.0f));
+        float x = sqrtf(1.0f / s2 - 1.0f);
+        float y = -x * (cos_t / sin_t);
+        float sty = sin_t * sqrtf(fmx(1.0f - y * y, 0.0f));
+        ill = (cos_t * acosf(clmp(y, -1.0f, 1.0f)) - x * sty) * s2 + atanf(sty / x);
+    }
+    return fmx(ill, 0.0f) / kPi;
+}
+
+struct ExtGeo {
+    f3 ldir;
+    float dist, ff, radius, inv_d2;
+};
+
+IFX_HD bool sphere_geo(const float4* L, f3 p, f3 n, ExtGeo& g) {
+    f3 c = ld3(L[0]);
+    float r = L[0].w;
+    f3 to = c - p;
+    float dist = len3(to);
+    if (!(dist > r)) return false;
+    g.ldir = to / dist;
+    g.dist = dist;
+    float cos_t = dot3(n, g.ldir);
+    float s2 = (r * r) / (dist * dist);
+    g.ff = sphere_ff(cos_t, s2);
+    g.radius = fmx(r, 1e-4f);
+    g.inv_d2 = 0.0f;
+    return g.ff > 1e-6f;
+}
+
+IFX_HD bool point_geo(const float4* L, f3 p, f3 n, ExtGeo& g) {
+    f3 to = ld3(L[0]) - p;
+    float d2 = dot3(to, to);
+    float dist = sqrtf(d2);
+    if (!(dist > 1e-6f)) return false;
+    g.ldir = to / dist;
+    g.dist = dist;
+    float cos_t = dot3(n, g.ldir);
+    if (cos_t <= 0.0f) return false;
+    g.ff = cos_t / (kPi * d2);
+    g.radius = 1e-4f;
+    g.inv_d2 = 1.0f / d2;
+    return true;
+}
+
+template <class Stk>
+IFX_HD float ext_shadow(const SceneView& S, f3 p, f3 n, const ExtGeo& g, uint32_t& steps, Stk& stk) {
+    float k = clmp(g.dist / g.radius, 1.5f, 96.0f);
+    float end = fmx(g.dist - g.radius * 1.25f, 0.05f);
+    uint32_t s = 0;
+    float vis = soft_shadow(S, p + n * 3e-3f, g.ldir, 0.015f, end, k, s, stk);
+    steps += s;
+    return vis;
+}
+
+// src/shade.rs:133-164: diffuse direct light, the radiosity E term.
+template <class Stk>
+IFX_HD f3 direct_diffuse(const SceneView& S, const LightsView& Lv, f3 p, f3 n, f3 albedo, uint32_t& steps_out,
+                         Stk& stk) {
+    f3 out = spl(0.0f);
+    uint32_t steps = 0;
+    f3 tan, bit;
+    frame(n, tan, bit);
+    for (uint32_t li = 0; li < Lv.n_quad; ++li) {
+        const float4* L = Lv.quad + 7 * li;
+        f3 lc = ld3(L[5]);
+        f3 to_light = lc - p;
+        if (dot3(ld3(L[6]), -nrm3(to_light)) <= 0.0f) continue;
+        float ff = quad_ff(L, p, tan, bit, n);
+        if (ff <= 1e-6f) continue;
+        float dist = len3(to_light);
+        float radius = L[5].w;
+        float k = clmp(dist / radius, 1.5f, 96.0f);
+        float end = fmx(dist - radius * 1.25f, 0.05f);
+        uint32_t s = 0;
+        float vis = soft_shadow(S, p + n * 3e-3f, to_light / dist, 0.015f, end, k, s, stk);
+        steps += s;
+        if (vis <= 1e-4f) continue;
+        out = out + (albedo * ld3(L[4])) * (ff * vis);
+    }
+    for (uint32_t li = 0; li < Lv.n_sph; ++li) {  // EXT
+        const float4* L = Lv.sph + 2 * li;
+        ExtGeo g;
+        if (!sphere_geo(L, p, n, g)) continue;
+        float vis = ext_shadow(S, p, n, g, steps, stk);
+        if (vis <= 1e-4f) continue;
+        out = out + (albedo * ld3(L[1])) * (g.ff * vis);
+    }
+    for (uint32_t li = 0; li < Lv.n_pnt; ++li) {  // EXT
+        const float4* L = Lv.pnt + 2 * li;
+        ExtGeo g;
+        if (!point_geo(L, p, n, g)) continue;
+        float vis = ext_shadow(S, p, n, g, steps, stk);
+        if (vis <= 1e-4f) continue;
+        out = out + (albedo * ld3(L[1])) * (g.ff * vis);
+    }
+    steps_out = steps;
+    return out;
+}
+
+struct MatV {
+    f3 albedo, emissive, f0, kd;
+    float roughness, metallic;
+};
+
+IFX_HD MatV load_mat(const LightsView& Lv, uint32_t id) {
+    const float4* m = Lv.mat + 4 * (id < Lv.n_mat ? id : 0u);
+    MatV r;
+    r.albedo = ld3(m[0]);
+    r.roughness = m[0].w;
+    r.emissive = ld3(m[1]);
+    r.metallic = m[1].w;
+    r.f0 = ld3(m[2]);
+    r.kd = ld3(m[3]);
+    return r;
+}
+
+IFX_HD f3 ggx_lobe(f3 n, f3 v, f3 l, float n_dot_v, float n_dot_l, float wide, float energy, f3 f0) {
+    f3 h = nrm3(l + v);
+    float d = d_ggx(fmx(dot3(n, h), 0.0f), wide);
+    float vis_s = v_smith(n_dot_v, n_dot_l, wide);
+    f3 f = fresnel_schlick(fmx(dot3(l, h), 0.0f), f0);
+    return f * (d * vis_s * energy * n_dot_l);
+}
+
+// src/shade.rs:50-125: diffuse + representative-point GGX specular.
+template <class Stk>
+IFX_HD f3 direct(const SceneView& S, const LightsView& Lv, f3 p, f3 n, f3 v, const MatV& m, uint32_t& steps_out,
+                 Stk& stk) {
+    f3 out = spl(0.0f);
+    uint32_t steps = 0;
+    f3 tan, bit;
+    frame(n, tan, bit);
+    float n_dot_v = fmx(dot3(n, v), 1e-4f);
+    float alpha = fmx(m.roughness * m.roughness, 1e-3f);
+    for (uint32_t li = 0; li < Lv.n_quad; ++li) {
+        const float4* L = Lv.quad + 7 * li;
+        f3 lc = ld3(L[5]);
+        f3 to_light = lc - p;
+        f3 plane_n = ld3(L[6]);
+        if (dot3(plane_n, -nrm3(to_light)) <= 0.0f) continue;
+        float ff = quad_ff(L, p, tan, bit, n);
+        if (ff <= 1e-6f) continue;
+        float dist = len3(to_light);
+        float radius = L[5].w;
+        float k = clmp(dist / radius, 1.5f, 96.0f);
+        f3 ldir = to_light / dist;
+        float end = fmx(dist - radius * 1.25f, 0.05f);
+        uint32_t s = 0;
+        float vis = soft_shadow(S, p + n * 3e-3f, ldir, 0.015f, end, k, s, stk);
+        steps += s;
+        if (vis <= 1e-4f) continue;
+        f3 radiance = ld3(L[4]) * vis;
+        out = out + (m.kd * radiance) * ff;
+        f3 r = (-v) - n * (2.0f * dot3(n, -v));
+        float denom = dot3(r, plane_n);
+        f3 rep = lc;
+        if (fabsf(denom) > 1e-6f) {
+            float t = dot3(ld3(L[0]) - p, plane_n) / denom;
+            if (t > 0.0f) rep = closest_on_quad(L, p + r * t);
+        }
+        f3 l = nrm3(rep - p);
+        float n_dot_l = dot3(n, l);
+        if (n_dot_l <= 0.0f) continue;
+        float rep_dist = fmx(len3(rep - p), 1e-4f);
+        float wide = clmp(alpha + radius / (2.0f * rep_dist), alpha, 1.0f);
+        float energy = (alpha / wide) * (alpha / wide);
+        out = out + ggx_lobe(n, v, l, n_dot_v, n_dot_l, wide, energy, m.f0) * radiance;
+    }
+    for (uint32_t li = 0; li < Lv.n_sph; ++li) {  // EXT
+        const float4* L = Lv.sph + 2 * li;
+        ExtGeo g;
+        if (!sphere_geo(L, p, n, g)) continue;
+        float vis = ext_shadow(S, p, n, g, steps, stk);
+        if (vis <= 1e-4f) continue;
+        f3 radiance = ld3(L[1]) * vis;
+        out = out + (m.kd * radiance) * g.ff;
+        f3 r = (-v) - n * (2.0f * dot3(n, -v));
+        f3 to = ld3(L[0]) - p;
+        f3 ctr = r * dot3(to, r) - to;
+        float lc = len3(ctr);
+        f3 rep_v = lc > 1e-9f ? to + ctr * clmp(L[0].w / lc, 0.0f, 1.0f) : to;
+        f3 l = nrm3(rep_v);
+        float n_dot_l = dot3(n, l);
+        if (n_dot_l <= 0.0f) continue;
+        float rep_dist = fmx(len3(rep_v), 1e-4f);
+        float wide = clmp(alpha + g.radius / (2.0f * rep_dist), alpha, 1.0f);
+        float energy = (alpha / wide) * (alpha / wide);
+        out = out + ggx_lobe(n, v, l, n_dot_v, n_dot_l, wide, energy, m.f0) * radiance;
+    }
+    for (uint32_t li = 0; li < Lv.n_pnt; ++li) {  // EXT
+        const float4* L = Lv.pnt + 2 * li;
+        ExtGeo g;
+        if (!point_geo(L, p, n, g)) continue;
+        float vis = ext_shadow(S, p, n, g, steps, stk);
+        if (vis <= 1e-4f) continue;
+        f3 emit = ld3(L[1]);
+        out = out + (m.kd * emit) * (g.ff * vis);
+        f3 l = g.ldir;
+        float n_dot_l = dot3(n, l);
+        f3 h = nrm3(l + v);
+        float d = d_ggx(fmx(dot3(n, h), 0.0f), alpha);
+        float vis_s = v_smith(n_dot_v, n_dot_l, alpha);
+        f3 f = fresnel_schlick(fmx(dot3(l, h), 0.0f), m.f0);
+        out = out + (f * (d * vis_s * n_dot_l)) * (emit * (vis * g.inv_d2));
+    }
+    steps_out = steps;
+    return out;
+}
+
+// src/shade.rs:168-181 (+ EXT visible sphere emitters).
+IFX_HD bool emitter_hit(const LightsView& Lv, f3 ro, f3 rd, float tmax, f3& emit) {
+    bool found = false;
+    float bt = 0.0f;
+    for (uint32_t li = 0; li < Lv.n_quad; ++li) {
+        const float4* L = Lv.quad + 7 * li;
+        float t;
+        if (intersect_quad(L, ro, rd, t)) {
+            if (t < tmax && dot3(ld3(L[6]), rd) < 0.0f && (!found || t < bt)) {
+                found = true;
+                bt = t;
+                emit = ld3(L[4]);
+            }
+        }
+    }
+    for (uint32_t li = 0; li < Lv.n_sph; ++li) {
+        const float4* L = Lv.sph + 2 * li;
+        f3 oc = ro - ld3(L[0]);
+        float b = dot3(oc, rd);
+        float c = dot3(oc, oc) - L[0].w * L[0].w;
+        float disc = b * b - c;
+        if (c <= 0.0f || disc < 0.0f) continue;
+        float t = -b - sqrtf(disc);
+        if (t < 1e-4f) continue;
+        if (t < tmax && (!found || t < bt)) {
+            found = true;
+            bt = t;
+            emit = ld3(L[1]);
+        }
+    }
+    return found;
+}
+
+// Uniform grid over surfel positions, CSR by cell, items ascending per cell
+// (src/surfel.rs:61-114).
+struct GridView {
+    f3 lo;
+    float inv_cell;
+    int32_t dim[3];
+    const uint32_t* starts;
+    const uint32_t* items;
+};
+
+IFX_HD void grid_cell(const GridView& g, f3 p, int32_t c[3]) {
+    c[0] = clampi(sat_i32((p.x - g.lo.x) * g.inv_cell), 0, g.dim[0] - 1);
+    c[1] = clampi(sat_i32((p.y - g.lo.y) * g.inv_cell), 0, g.dim[1] - 1);
+    c[2] = clampi(sat_i32((p.z - g.lo.z) * g.inv_cell), 0, g.dim[2] - 1);
+}
+
+// src/surfel.rs:262-293: visit cells z, y, x then items in order.
+IFX_HD f3 gather(const GridView& g, const float4* __restrict__ sp, const float4* __restrict__ sn,
+                 const float4* __restrict__ irr, f3 p, f3 n, float radius) {
+    f3 acc = spl(0.0f);
+    float wsum = 0.0f;
+    float r2 = radius * radius;
+    int32_t c[3];
+    grid_cell(g, p, c);
+    int32_t z1 = c[2] + 1 < g.dim[2] - 1 ? c[2] + 1 : g.dim[2] - 1;
+    int32_t y1 = c[1] + 1 < g.dim[1] - 1 ? c[1] + 1 : g.dim[1] - 1;
+    int32_t x1 = c[0] + 1 < g.dim[0] - 1 ? c[0] + 1 : g.dim[0] - 1;
+    for (int32_t z = c[2] - 1 > 0 ? c[2] - 1 : 0; z <= z1; ++z)
+        for (int32_t y = c[1] - 1 > 0 ? c[1] - 1 : 0; y <= y1; ++y)
+            for (int32_t x = c[0] - 1 > 0 ? c[0] - 1 : 0; x <= x1; ++x) {
+                uint32_t ci = (uint32_t)(x + g.dim[0] * (y + g.dim[1] * z));
+                uint32_t e = g.starts[ci + 1];
+                for (uint32_t k = g.starts[ci]; k < e; ++k) {
+                    uint32_t it = g.items[k];
+                    f3 q = ld3(sp[it]);
+                    f3 dq = q - p;
+                    float d2 = dot3(dq, dq);
+                    if (d2 > r2) continue;
+                    float align = dot3(ld3(sn[it]), n);
+                    if (align <= 0.15f) continue;
+                    float t = 1.0f - d2 / r2;
+                    float w = t * t * align;
+                    acc = acc + ld3(irr[it]) * w;
+                    wsum += w;
+                }
+            }
+    if (wsum > 1e-6f) return acc / wsum;
+    return spl(0.0f);
+}
+
+}  // namespace ifx

@@ -1,0 +1,252 @@
+"""GPU parity: the gfx950 kernels against the oracle on identical inputs.
+
+Bar (from the path's contract): SDF distances, materials, trace steps and
+hits, soft-shadow visibilities and the transfer CSR (link indices and
+weights) bit-exact; radiosity within rel-L2 1e-4 (the only transcendental on
+the path, acos in the quad form factor, may differ by ulps between the
+device math library and glibc); the encoded image within 1/255 per channel
+on >= 99.9% of pixels.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import interflect_amd as ia
+from conftest import scene_text
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def random_points(lo, hi, n, seed):
+    rng = np.random.default_rng(seed)
+    return (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+
+
+def pair(name):
+    src = scene_text(name)
+    ctx = ia.Context(0)
+    ctx.load_scene(src)
+    return ctx, po.Scene(src)
+
+
+# --------------------------------------------------------------------------- kernel family 1+2: SDF + BVH
+@pytest.mark.parametrize("name", ["product.rad", "stress.rad", "cornell_bench.rad", "dog.rad"])
+def test_dist_mat_and_normals_bitexact(name):
+    say("dist_mat", name)
+    ctx, sc = pair(name)
+    pts = random_points(sc.lo, sc.hi, 100000, 1)
+    d, m, nrm = ctx.dist_mat(pts, normals=True)
+    d0, m0 = sc.dist_mat(pts)
+    n0 = sc.normal(pts)
+    assert np.array_equal(bits(d), bits(d0))
+    assert np.array_equal(m, m0)
+    assert np.array_equal(bits(nrm), bits(n0))
+
+
+def test_dist_mat_linear_scan_bitexact():
+    """--no-bvh (src/bvh.rs:152-163) on the 513-primitive stress scene."""
+    ctx, sc = pair("stress.rad")
+    ctx.set_use_bvh(False)
+    sc.set_use_bvh(False)
+    pts = random_points(sc.lo, sc.hi, 20000, 2)
+    d, m = ctx.dist_mat(pts)
+    d0, m0 = sc.dist_mat(pts)
+    assert np.array_equal(bits(d), bits(d0)) and np.array_equal(m, m0)
+
+
+# --------------------------------------------------------------------------- kernel family 3: tracing
+def camera_rays(sc, n, seed):
+    rng = np.random.default_rng(seed)
+    pos, look, up, fov = sc.camera[0:3], sc.camera[3:6], sc.camera[6:9], sc.camera[9]
+    fwd = (look - pos) / np.linalg.norm(look - pos)
+    right = np.cross(fwd, up)
+    right /= np.linalg.norm(right)
+    upv = np.cross(right, fwd)
+    t = np.tan(np.radians(fov) / 2)
+    uv = rng.uniform(-1, 1, (n, 2)) * t
+    rd = fwd + right * uv[:, :1] * 1.33 + upv * uv[:, 1:]
+    rd = (rd / np.linalg.norm(rd, axis=1, keepdims=True)).astype(np.float32)
+    ro = np.tile(pos, (n, 1)).astype(np.float32)
+    return ro, rd
+
+
+@pytest.mark.parametrize("name", ["product.rad", "stress.rad"])
+def test_trace_bitexact(name):
+    say("trace", name)
+    ctx, sc = pair(name)
+    ro, rd = camera_rays(sc, 50000, 3)
+    tmax = np.float32(sc.diag * 4)
+    g = ctx.trace_rays(ro, rd, 1e-3, tmax)
+    o = sc.trace(ro, rd, 1e-3, tmax)
+    assert np.array_equal(g[0], o[0])
+    assert np.array_equal(g[4], o[4])
+    h = o[0]
+    assert np.array_equal(bits(g[1][h]), bits(o[1][h]))
+    assert np.array_equal(bits(g[2][h]), bits(o[2][h]))
+    assert np.array_equal(g[3][h], o[3][h])
+
+
+@pytest.mark.parametrize("name", ["product.rad", "stress.rad", "flush_ceiling_light.rad"])
+def test_soft_shadow_bitexact(name):
+    say("soft_shadow", name)
+    ctx, sc = pair(name)
+    rng = np.random.default_rng(4)
+    n = 50000
+    ro = random_points(sc.lo, sc.hi, n, 5)
+    tgt = random_points(sc.lo, sc.hi, n, 6)
+    d = tgt - ro
+    dist = np.linalg.norm(d, axis=1).astype(np.float32)
+    rd = (d / dist[:, None]).astype(np.float32)
+    k = rng.uniform(1.5, 96, n).astype(np.float32)
+    v, s = ctx.soft_shadows(ro, rd, 0.015, dist, k)
+    v0, s0 = sc.soft_shadow(ro, rd, 0.015, dist, k)
+    assert np.array_equal(bits(v), bits(v0)) and np.array_equal(s, s0)
+
+
+# --------------------------------------------------------------------------- kernel family 4: transfer
+def run_both(name, target=None, res=8, rows=None):
+    """Oracle surfels fed to both sides ("given identical surfels")."""
+    src = scene_text(name)
+    sc = po.Scene(src)
+    sc.generate(target)
+    p, n, a, ar = sc.surfels()
+    ctx = ia.Context(0)
+    ctx.load_scene(src)
+    ctx.set_surfels(p, n, a, ar)
+    tg = ctx.transfer(res)
+    to = sc.transfer(res, rows=rows)
+    return ctx, sc, tg, to
+
+
+def compare_transfer(ctx, sc, tg, to, rows=None):
+    cg, co = ctx.clusters(), sc.clusters()
+    assert tg["clusters"] == to["clusters"]
+    for x, y in zip(cg, co):
+        assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))
+    sg, ig, wg, skyg, rsg = ctx.transfer_arrays()
+    so, io, wo, skyo, rso = sc.transfer_arrays()
+    N = len(sg) - 1
+    r0, r1 = (0, N) if rows is None else rows
+    mism = 0
+    for i in range(r0, r1):
+        a, b = slice(sg[i], sg[i + 1]), slice(so[i], so[i + 1])
+        if not (np.array_equal(ig[a], io[b]) and np.array_equal(bits(wg[a]), bits(wo[b]))):
+            mism += 1
+    assert mism == 0, "%d of %d rows differ" % (mism, r1 - r0)
+    assert np.array_equal(bits(skyg[r0:r1]), bits(skyo[r0:r1]))
+    if rows is None:
+        assert np.array_equal(sg, so)
+        assert tg["links"] == to["links"]
+        assert tg["evals"] == to["evals"]
+        assert np.float32(tg["mean_row_sum"]) == np.float32(to["mean_row_sum"])
+
+
+@pytest.mark.parametrize("name,target", [("product.rad", None), ("closed_box.rad", 3000), ("cornell_bench.rad", None)])
+def test_transfer_links_exact(name, target):
+    say("transfer", name)
+    ctx, sc, tg, to = run_both(name, target)
+    compare_transfer(ctx, sc, tg, to)
+
+
+def test_transfer_stress_rows_exact():
+    """513 primitives, BVH depth 10: rows 0..1500 checked against the oracle."""
+    say("transfer stress rows")
+    ctx, sc, tg, to = run_both("stress.rad", rows=(0, 1500))
+    compare_transfer(ctx, sc, tg, to, rows=(0, 1500))
+
+
+# --------------------------------------------------------------------------- kernel family 5: solve
+@pytest.mark.parametrize("name,target", [("product.rad", None), ("closed_box.rad", 3000)])
+def test_solve_matches_oracle(name, target):
+    say("solve", name)
+    ctx, sc, tg, to = run_both(name, target)
+    info = ctx.info()
+    g = ctx.solve(info["bounces"])
+    o = sc.solve(info["bounces"])
+    assert g["iterations"] == o["iterations"]
+    assert g["evals"] == o["evals"]
+    dg, ig, rg = ctx.radiosity()
+    do, io, ro = sc.radiosity()
+    for x, y in ((dg, do), (rg, ro), (ig, io)):
+        assert np.linalg.norm(x - y) <= 1e-4 * max(np.linalg.norm(y), 1e-30)
+
+
+# --------------------------------------------------------------------------- kernel family 6: shade
+def image_close(a, b):
+    diff = np.abs(a.astype(np.int32) - b.astype(np.int32)).max(axis=2)
+    return float(np.mean(diff <= 1)), int(diff.max())
+
+
+@pytest.fixture(scope="module")
+def product_pair():
+    ctx, sc, tg, to = run_both("product.rad")
+    ctx.solve()
+    sc.solve()
+    return ctx, sc
+
+
+@pytest.mark.parametrize("mode", ["beauty", "direct", "indirect", "normals", "depth", "steps"])
+def test_shade_modes_match_oracle(product_pair, mode):
+    say("shade", mode)
+    ctx, sc = product_pair
+    w, h = (640, 480) if mode == "beauty" else (160, 120)
+    g = ctx.shade(mode, w, h)
+    film, ev = sc.render(mode, w, h)
+    tonemap = mode in ("beauty", "direct", "indirect")
+    b = po.encode(film, sc.exposure if tonemap else 1.0, tonemap)
+    frac, worst = image_close(g["rgb8"], b)
+    assert frac >= 0.999, (frac, worst)
+    assert abs(g["evals"] - ev) <= 1e-4 * ev
+    rel = np.linalg.norm(g["film"] - film) / max(np.linalg.norm(film), 1e-30)
+    assert rel < 1e-3
+
+
+def test_encode_bitexact_against_oracle(product_pair):
+    """k_encode's threshold table reproduces the host sRGB transfer exactly."""
+    ctx, sc = product_pair
+    g = ctx.shade("direct", 160, 120)
+    assert np.array_equal(g["rgb8"], po.encode(g["film"], sc.exposure, True))
+
+
+# --------------------------------------------------------------------------- the published run, on the GPU
+def test_product_published_counts_on_gpu():
+    say("product published")
+    ctx = ia.Context(0)
+    ctx.load_scene(scene_text("product.rad"))
+    n = ctx.generate_surfels()
+    t = ctx.transfer(8)
+    s = ctx.solve()
+    r = ctx.shade("beauty")
+    assert n == 12270
+    assert (t["clusters"], t["links"]) == (311, 110026)
+    assert "%.3f" % t["mean_row_sum"] == "0.109" and "%.1f" % (t["evals"] / 1e6) == "10.4"
+    assert s["iterations"] == 12 and "%.2e" % s["residual"] == "5.37e-05" and "%.1f" % (s["evals"] / 1e6) == "0.3"
+    assert "%.1f" % (r["evals"] / 1e6) == "15.3" and round(r["evals"] / (640 * 480)) == 50
+
+
+def test_repeat_runs_are_bit_identical():
+    """ci.yml:95-101 on the GPU: two full runs give the same bytes."""
+    outs = []
+    for _ in range(2):
+        ctx = ia.Context(0)
+        ctx.load_scene(scene_text("product.rad"))
+        ctx.generate_surfels()
+        ctx.transfer(8)
+        ctx.solve()
+        r = ctx.shade("beauty", 320, 240)
+        outs.append((ctx.transfer_arrays(), ctx.radiosity(), r["rgb8"]))
+    (t1, r1, i1), (t2, r2, i2) = outs
+    for a, b in zip(t1 + r1, t2 + r2):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+    assert np.array_equal(i1, i2)

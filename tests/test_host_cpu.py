@@ -1,0 +1,108 @@
+"""CPU checks of the product library: the C ABI loads and exports every
+declared symbol, and its host-side scene loader, BVH build and surfel
+placement (a host-only context, device -1) agree bit for bit with the oracle.
+No kernel runs here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import interflect_amd as ia
+from conftest import ROOT, scene_text
+from oracle import pyoracle as po
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "interflect.h")).read()
+    declared = set(re.findall(r"^(?:ifx_status|const char\*)\s+(ifx_\w+)\(", hdr, re.M))
+    assert declared == set(ia.EXPORTS)
+    L = ia.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def test_product_library_does_not_link_the_oracle():
+    import subprocess
+
+    out = subprocess.run(["ldd", ia.LIB_PATH], capture_output=True, text=True).stdout
+    assert "oracle" not in out
+    syms = subprocess.run(["nm", "-D", "--defined-only", ia.LIB_PATH], capture_output=True, text=True).stdout
+    assert "orc_" not in syms
+
+
+@pytest.mark.parametrize("name", ["product.rad", "cornell_bench.rad", "stress.rad", "closed_box.rad", "alcove.rad",
+                                  "dog.rad", "minimal.rad"])
+def test_loader_matches_oracle(name):
+    src = scene_text(name)
+    ctx = ia.Context(-1)
+    info = ctx.load_scene(src)
+    sc = po.Scene(src)
+    assert info["n_objects"] == sc.n_objects and info["n_planes"] == sc.n_planes
+    assert info["n_quad_lights"] == sc.n_lights and info["n_materials"] == sc.n_materials
+    assert info["n_bvh_nodes"] == sc.n_bvh_nodes
+    assert np.array_equal(info["lo"], sc.lo) and np.array_equal(info["hi"], sc.hi)
+    assert (info["width"], info["height"], info["surfels"], info["bounces"]) == (
+        sc.width, sc.height, sc.surfels_requested, sc.bounces)
+    assert info["exposure"] == np.float32(sc.exposure)
+
+
+@pytest.mark.parametrize("name,target", [("product.rad", None), ("cornell_bench.rad", None), ("stress.rad", None),
+                                         ("closed_box.rad", 3000)])
+def test_surfels_match_oracle_bitwise(name, target):
+    src = scene_text(name)
+    ctx = ia.Context(-1)
+    ctx.load_scene(src)
+    sc = po.Scene(src)
+    n1 = ctx.generate_surfels(target)
+    n2 = sc.generate(target)
+    assert n1 == n2
+    for a, b in zip(ctx.surfels(), sc.surfels()):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+from test_oracle_pins import LIMITS, MALFORMED, BASE_LIGHT  # noqa: E402
+
+
+@pytest.mark.parametrize("src,msg", MALFORMED)
+def test_loader_rejects_malformed_input_like_the_reference(src, msg):
+    ctx = ia.Context(-1)
+    with pytest.raises(ia.IfxError, match=msg) as e:
+        ctx.load_scene(src)
+    assert e.value.status == -2
+
+
+@pytest.mark.parametrize("cfg,msg", LIMITS)
+def test_loader_rejects_absurd_settings(cfg, msg):
+    ctx = ia.Context(-1)
+    with pytest.raises(ia.IfxError, match=msg):
+        ctx.load_scene(cfg + "\n" + BASE_LIGHT + "\n")
+
+
+def test_host_only_context_refuses_kernels():
+    ctx = ia.Context(-1)
+    ctx.load_scene(scene_text("minimal.rad"))
+    ctx.generate_surfels(100)
+    with pytest.raises(ia.IfxError) as e:
+        ctx.transfer(8)
+    assert e.value.status == -3
+    with pytest.raises(ia.IfxError):
+        ctx.dist_mat(np.zeros((4, 3), np.float32))
+
+
+def test_cli_overrides_are_revalidated():
+    """src/main.rs:394-413: --surfels 999999999 is rejected after parsing."""
+    ctx = ia.Context(-1)
+    ctx.load_scene(scene_text("minimal.rad"))
+    with pytest.raises(ia.IfxError, match="surfels"):
+        ctx.set_render(800, 600, 999999999, 12)
+    ctx.set_render(320, 240, 5000, 8)
+    info = ctx.info()
+    assert (info["width"], info["height"], info["surfels"], info["bounces"]) == (320, 240, 5000, 8)
+
+
+def test_stage_order_is_enforced():
+    ctx = ia.Context(-1)
+    with pytest.raises(ia.IfxError) as e:
+        ctx.generate_surfels(10)
+    assert e.value.status == -3
