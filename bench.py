@@ -107,6 +107,14 @@ def main():
     for _ in range(args.warmup):
         run_frame(ctx, cfg["res"], bounces)
 
+    # One untimed instrumented frame: algorithmic FP32 operations per launch
+    # of each tracing kernel (same traversal, counting stack).
+    ctx.reset_kernel_times()
+    ctx.set_counting(True)
+    run_frame(ctx, cfg["res"], bounces)
+    ctx.set_counting(False)
+    ops = {k: ctx.kernel_ops(k) for k in TRACE_KERNELS}
+
     ctx.set_profiling(True)
     ctx.reset_kernel_times()
     stage = {"transfer": 0.0, "solve": 0.0, "shade": 0.0}
@@ -141,6 +149,20 @@ def main():
         spmv_roof = {"kernel": "spmv", "bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": gbs / PEAK_HBM_GBS, "traffic": None, "bytes_per_launch": spmv_bytes}
 
+    roof = None
+    if dominant in TRACE_KERNELS and ops.get(dominant):
+        tfl = ops[dominant] / (ktime[dominant]["avg_ms"] * 1e-3) / 1e12
+        roof = {"kernel": dominant, "bound": "valu", "achieved": tfl, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
+                "frac": tfl / PEAK_VALU_TFLOPS, "traffic": None, "ops_per_launch": ops[dominant]}
+    elif dominant == "spmv":
+        roof = spmv_roof
+    trace_roofs = {}
+    for k in TRACE_KERNELS:
+        if k in ktime and ops.get(k):
+            tfl = ops[k] / (ktime[k]["avg_ms"] * 1e-3) / 1e12
+            trace_roofs[k] = {"achieved_tflops": tfl, "frac": tfl / PEAK_VALU_TFLOPS, "ops_per_launch": ops[k],
+                              "avg_ms": ktime[k]["avg_ms"]}
+
     frame_evals = evals / args.steps
     out = {
         "metric": METRIC,
@@ -160,8 +182,9 @@ def main():
         "stages_ms": {k: v / args.steps for k, v in stage.items()},
         "kernels": ktime,
         "dominant_kernel": dominant,
-        "roofline": None,
+        "roofline": roof,
         "roofline_spmv": spmv_roof,
+        "roofline_tracing": trace_roofs,
     }
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))

@@ -133,6 +133,12 @@ ifx_status ifx_set_profiling(ifx_ctx* ctx, int32_t on);
 /* Kernel `name`'s launch count and summed duration since the last reset. */
 ifx_status ifx_kernel_time(ifx_ctx* ctx, const char* name, uint64_t* launches, double* total_ms);
 ifx_status ifx_reset_kernel_times(ifx_ctx* ctx);
+/* Instrumented mode: the tracing kernels (visibility, closure, direct_e,
+ * shade) run a counting instantiation that tallies algorithmic FP32
+ * operations; results are identical, timings are not representative.
+ * No counterpart in the reference; feeds the roofline report. */
+ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
+ifx_status ifx_kernel_ops(ifx_ctx* ctx, const char* name, uint64_t* ops);
 
 #ifdef __cplusplus
 }

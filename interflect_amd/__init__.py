@@ -32,7 +32,7 @@ EXPORTS = [
     "ifx_set_use_bvh", "ifx_generate_surfels", "ifx_set_surfels", "ifx_num_surfels", "ifx_get_surfels",
     "ifx_transfer", "ifx_get_transfer", "ifx_get_clusters", "ifx_solve", "ifx_get_radiosity", "ifx_set_radiosity",
     "ifx_shade", "ifx_dist_mat", "ifx_trace_rays", "ifx_soft_shadows", "ifx_set_profiling", "ifx_kernel_time",
-    "ifx_reset_kernel_times",
+    "ifx_reset_kernel_times", "ifx_set_counting", "ifx_kernel_ops",
 ]
 
 
@@ -104,6 +104,8 @@ def lib():
         "ifx_set_profiling": [vp, i32],
         "ifx_kernel_time": [vp, C.c_char_p, C.POINTER(u64), C.POINTER(C.c_double)],
         "ifx_reset_kernel_times": [vp],
+        "ifx_set_counting": [vp, i32],
+        "ifx_kernel_ops": [vp, C.c_char_p, C.POINTER(u64)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -314,3 +316,11 @@ class Context:
 
     def reset_kernel_times(self):
         self._check(lib().ifx_reset_kernel_times(self._h))
+
+    def set_counting(self, on):
+        self._check(lib().ifx_set_counting(self._h, 1 if on else 0))
+
+    def kernel_ops(self, name):
+        n = C.c_uint64()
+        self._check(lib().ifx_kernel_ops(self._h, name.encode(), C.byref(n)))
+        return int(n.value)

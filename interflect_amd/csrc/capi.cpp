@@ -331,4 +331,21 @@ ifx_status ifx_reset_kernel_times(ifx_ctx* ctx) {
     });
 }
 
+ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on) {
+    return guarded(ctx, [&]() -> ifx_status {
+        ctx->eng.set_counting(on != 0);
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_kernel_ops(ifx_ctx* ctx, const char* name, uint64_t* ops) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (!name || !ops) return IFX_E_INVALID;
+        const auto& m = ctx->eng.kernel_ops();
+        auto it = m.find(name);
+        *ops = it == m.end() ? 0 : it->second;
+        return IFX_OK;
+    });
+}
+
 }  // extern "C"

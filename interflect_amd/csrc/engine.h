@@ -72,8 +72,14 @@ class Engine {
     int upload_radiosity(const float* direct, const float* irr, const float* rad, std::string& err);
 
     void set_profiling(bool on) { prof_ = on; }
+    // Instrumented launches of the tracing kernels (algorithmic FP32 operation tallies).
+    void set_counting(bool on) { count_ops_ = on; }
+    const std::map<std::string, uint64_t>& kernel_ops() const { return kops_; }
     const std::map<std::string, KernelStat>& kernel_stats() const { return kstats_; }
-    void reset_kernel_stats() { kstats_.clear(); }
+    void reset_kernel_stats() {
+        kstats_.clear();
+        kops_.clear();
+    }
 
     HostScene scene;
     std::vector<HSurfel> surfels;
@@ -95,6 +101,8 @@ class Engine {
     int device_ = 0;
     bool has_device_ = false;
     bool prof_ = false;
+    bool count_ops_ = false;
+    std::map<std::string, uint64_t> kops_;
     std::map<std::string, KernelStat> kstats_;
     struct Pending {
         const char* name;

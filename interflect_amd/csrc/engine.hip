@@ -75,6 +75,30 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     __shared__ uint16_t stk_mem[kStackCap * kBlock];     \
     LdsStack stk{stk_mem + threadIdx.x, (uint32_t)kBlock}
 
+// Timed kernels traverse with LdsStack; the instrumented instantiation
+// (COUNT=true) uses CountStack and adds its operation tally to `ops`.
+template <bool COUNT>
+struct StackOf {
+    using T = LdsStack;
+};
+template <>
+struct StackOf<true> {
+    using T = CountStack;
+};
+__device__ __forceinline__ void stk_init(LdsStack&) {}
+__device__ __forceinline__ void stk_init(CountStack& s) { s.ops = 0ull; }
+__device__ __forceinline__ void stk_flush(LdsStack&, unsigned long long*) {}
+__device__ __forceinline__ void stk_flush(CountStack& s, unsigned long long* ops) {
+    if (s.ops) atomicAdd(ops, s.ops);
+}
+
+#define IFX_STACK(COUNT)                                 \
+    __shared__ uint16_t stk_mem[kStackCap * kBlock];     \
+    typename StackOf<COUNT>::T stk;                      \
+    stk.base = stk_mem + threadIdx.x;                    \
+    stk.stride = (uint32_t)kBlock;                       \
+    stk_init(stk)
+
 // src/formfactor.rs:70-89
 __device__ __forceinline__ uint32_t normal_bucket(f3 n) {
     f3 a = abs3(n);
@@ -242,6 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
 
 // Cone-traced link visibility, one candidate per lane (src/formfactor.rs:194-214).
 // cand_v = ff * vis for a kept link, 0 for a culled one.
+template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_visibility(SceneView S, const float4* __restrict__ spos,
                                                        const float4* __restrict__ snrm,
                                                        const float4* __restrict__ cpos,
@@ -249,8 +274,8 @@ __global__ __launch_bounds__(kBlock) void k_visibility(SceneView S, const float4
                                                        const uint32_t* __restrict__ cand_ci,
                                                        const uint32_t* __restrict__ cand_row, uint32_t M,
                                                        float off, float tmin, float* cand_v,
-                                                       unsigned long long* evals) {
-    IFX_LDS_STACK();
+                                                       unsigned long long* evals, unsigned long long* ops) {
+    IFX_STACK(COUNT);
     uint32_t acc = 0;
     uint32_t stride = gridDim.x * kBlock;
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < M; k += stride) {
@@ -267,15 +292,17 @@ __global__ __launch_bounds__(kBlock) void k_visibility(SceneView S, const float4
         cand_v[k] = vis <= 1e-3f ? 0.0f : ff * vis;
     }
     count_steps(evals, acc);
+    stk_flush(stk, ops);
 }
 
 // Hemisphere closure: 16 lanes per surfel, one Halton direction each
 // (src/trace.rs:165-186).  dirs[k] = (r cos phi, r sin phi, sqrt(1 - u1)).
+template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_closure(SceneView S, const float4* __restrict__ spos,
                                                     const float4* __restrict__ snrm, uint32_t N,
                                                     const float4* __restrict__ dirs, float tmax, float* sky,
-                                                    unsigned long long* evals) {
-    IFX_LDS_STACK();
+                                                    unsigned long long* evals, unsigned long long* ops) {
+    IFX_STACK(COUNT);
     uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
     uint32_t row = tid >> 4, k = tid & 15u;
     bool active = row < N;
@@ -294,6 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_closure(SceneView S, const float4* _
     uint32_t cnt = (uint32_t)__popcll((m >> (lane_id() & 48u)) & 0xffffull);
     if (active && k == 0) sky[row] = (float)cnt / 16.0f;
     count_steps(evals, steps);
+    stk_flush(stk, ops);
 }
 
 // Serial per-row link sum in cluster order plus the closure calibration
@@ -340,11 +368,12 @@ __global__ void k_compact(uint32_t N, const uint32_t* __restrict__ cstart, const
 }
 
 // ====================================================================== solve
+template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_direct_e(SceneView S, LightsView L, const float4* __restrict__ spos,
                                                      const float4* __restrict__ snrm, const float4* __restrict__ salb,
                                                      uint32_t N, float4* direct, float4* rad, float4* irr,
-                                                     unsigned long long* evals) {
-    IFX_LDS_STACK();
+                                                     unsigned long long* evals, unsigned long long* ops) {
+    IFX_STACK(COUNT);
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     uint32_t steps = 0;
     if (i < N) {
@@ -354,6 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_direct_e(SceneView S, LightsView L, 
         irr[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     count_steps(evals, steps);
+    stk_flush(stk, ops);
 }
 
 // Jacobi bounce `it` is a no-op once bounce it-1 converged (residual < 1e-4),
@@ -501,9 +531,10 @@ __device__ f3 shade_pixel(const SceneView& S, const LightsView& L, const ShadeAr
 }
 
 // 16x16 pixel blocks, one 8x8 tile per wavefront.
+template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneView S, LightsView L, ShadeArgs A, float4* film,
-                                                  unsigned long long* evals) {
-    IFX_LDS_STACK();
+                                                  unsigned long long* evals, unsigned long long* ops) {
+    IFX_STACK(COUNT);
     uint32_t wv = threadIdx.x >> 6, l = threadIdx.x & 63u;
     uint32_t x = blockIdx.x * 16u + (wv & 1u) * 8u + (l & 7u);
     uint32_t y = blockIdx.y * 16u + (wv >> 1) * 8u + (l >> 3);
@@ -513,6 +544,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView S, LightsView L, Sha
         film[(size_t)y * A.W + x] = st4(c);
     }
     count_steps(evals, local);
+    stk_flush(stk, ops);
 }
 
 // Film::encode: exposure, ACES fit, sRGB through host-built thresholds
@@ -905,15 +937,24 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     if (M) {
         const uint32_t vis_blocks = std::min<uint32_t>(blocks_for(M, kBlock), 65535u * 4u);
         launch("visibility", [&] {
-            k_visibility<<<vis_blocks, kBlock, 0, stream>>>(S, spos, snrm, d_cpos.as<float4>(), d_cnrm.as<float4>(),
-                                                            d_cand_ci.as<uint32_t>(), d_cand_row.as<uint32_t>(), M,
-                                                            diag * 5e-4f, diag * 1e-3f, d_cand_v.as<float>(), ctr);
+            if (count_ops_)
+                k_visibility<true><<<vis_blocks, kBlock, 0, stream>>>(
+                    S, spos, snrm, d_cpos.as<float4>(), d_cnrm.as<float4>(), d_cand_ci.as<uint32_t>(),
+                    d_cand_row.as<uint32_t>(), M, diag * 5e-4f, diag * 1e-3f, d_cand_v.as<float>(), ctr, ctr + 1);
+            else
+                k_visibility<false><<<vis_blocks, kBlock, 0, stream>>>(
+                    S, spos, snrm, d_cpos.as<float4>(), d_cnrm.as<float4>(), d_cand_ci.as<uint32_t>(),
+                    d_cand_row.as<uint32_t>(), M, diag * 5e-4f, diag * 1e-3f, d_cand_v.as<float>(), ctr, nullptr);
         });
     }
     CK(d_sky.ensure(4ull * N));
     launch("closure", [&] {
-        k_closure<<<blocks_for(16ull * N, kBlock), kBlock, 0, stream>>>(S, spos, snrm, N, d_dirs.as<float4>(),
-                                                                         diag * 2.0f, d_sky.as<float>(), ctr);
+        if (count_ops_)
+            k_closure<true><<<blocks_for(16ull * N, kBlock), kBlock, 0, stream>>>(
+                S, spos, snrm, N, d_dirs.as<float4>(), diag * 2.0f, d_sky.as<float>(), ctr, ctr + 2);
+        else
+            k_closure<false><<<blocks_for(16ull * N, kBlock), kBlock, 0, stream>>>(
+                S, spos, snrm, N, d_dirs.as<float4>(), diag * 2.0f, d_sky.as<float>(), ctr, nullptr);
     });
 
     // ---- calibration, link scan, compaction
@@ -945,10 +986,12 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     });
     // mean_row_sum: serial f32 sum over rows with links, in row order.
     std::vector<float> rs(N);
-    unsigned long long ev = 0;
+    unsigned long long cv[8] = {0};
     CK(hipMemcpyAsync(rs.data(), d_rowsum.p, 4ull * N, hipMemcpyDeviceToHost, stream));
-    CK(hipMemcpyAsync(&ev, ctr, 8, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
+    unsigned long long ev = cv[0];
+    if (count_ops_) kops_["visibility"] += cv[1], kops_["closure"] += cv[2];
     drain_events();
     float acc = -0.0f;
     uint64_t interior = 0;
@@ -982,9 +1025,14 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
     const LightsView Lv = dev_lights();
     if (N) {
         launch("direct_e", [&] {
-            k_direct_e<<<blocks_for(N, kBlock), kBlock, 0, stream>>>(
-                S, Lv, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, d_direct.as<float4>(),
-                d_rad.as<float4>(), d_irr.as<float4>(), ctr);
+            if (count_ops_)
+                k_direct_e<true><<<blocks_for(N, kBlock), kBlock, 0, stream>>>(
+                    S, Lv, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, d_direct.as<float4>(),
+                    d_rad.as<float4>(), d_irr.as<float4>(), ctr, ctr + 3);
+            else
+                k_direct_e<false><<<blocks_for(N, kBlock), kBlock, 0, stream>>>(
+                    S, Lv, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, d_direct.as<float4>(),
+                    d_rad.as<float4>(), d_irr.as<float4>(), ctr, nullptr);
         });
     }
     const uint32_t C = n_clusters;
@@ -1005,10 +1053,12 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
         }
     }
     std::vector<uint32_t> res(bounces + 1, 0);
-    unsigned long long ev = 0;
+    unsigned long long cv[8] = {0};
     CK(hipMemcpyAsync(res.data(), d_resid.p, 4ull * bounces, hipMemcpyDeviceToHost, stream));
-    CK(hipMemcpyAsync(&ev, ctr, 8, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
+    unsigned long long ev = cv[0];
+    if (count_ops_) kops_["direct_e"] += cv[3];
     drain_events();
     out.evals = ev;
     if (C > 0 && N > 0) {
@@ -1124,7 +1174,12 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
     const SceneView S = dev_view();
     const LightsView Lv = dev_lights();
     dim3 grid((W + 15) / 16, (H + 15) / 16);
-    launch("shade", [&] { k_shade<<<grid, kBlock, 0, stream>>>(S, Lv, A, d_film.as<float4>(), ctr); });
+    launch("shade", [&] {
+        if (count_ops_)
+            k_shade<true><<<grid, kBlock, 0, stream>>>(S, Lv, A, d_film.as<float4>(), ctr, ctr + 4);
+        else
+            k_shade<false><<<grid, kBlock, 0, stream>>>(S, Lv, A, d_film.as<float4>(), ctr, nullptr);
+    });
     const bool tonemapped = mode <= M_INDIRECT;
     if (rgb8) {
         launch("encode", [&] {
@@ -1137,11 +1192,12 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
     if (film) {
         CK(hipMemcpy2DAsync(film, 12, d_film.p, 16, 12, npx, hipMemcpyDeviceToHost, stream));
     }
-    unsigned long long ev = 0;
-    CK(hipMemcpyAsync(&ev, ctr, 8, hipMemcpyDeviceToHost, stream));
+    unsigned long long cv[8] = {0};
+    CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
     drain_events();
-    evals = ev;
+    if (count_ops_) kops_["shade"] += cv[4];
+    evals = cv[0];
     return 0;
 }
 

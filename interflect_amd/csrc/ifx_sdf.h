@@ -123,16 +123,19 @@ IFX_HD float prim_dist(float4 g0, float4 g1, float4 g2, f3 p) {
 }
 
 // src/sdf.rs:164-173: base primitive, then each subtraction in order.
+template <class Stk>
 IFX_HD float obj_dist(const float4* __restrict__ rec, const float4* __restrict__ subs, f3 p, uint32_t& mat,
-                      bool& blended) {
+                      bool& blended, Stk& stk) {
     float4 g0 = rec[0], g1 = rec[1], g2 = rec[2], g3 = rec[3];
     mat = fbits(g3.x);
     uint32_t sb = fbits(g3.y), sc = fbits(g3.z);
     blended = fbits(g3.w) != 0u;
     float d = prim_dist(g0, g1, g2, p);
+    stk.count_prim(fbits(g2.z), 0u);
     for (uint32_t s = sb; s < sb + sc; ++s) {
         const float4* q = subs + 4 * s;
         d = fmx(d, -prim_dist(q[0], q[1], q[2], p));
+        stk.count_prim(fbits(q[2].z), 2u);
     }
     return d;
 }
@@ -149,13 +152,26 @@ IFX_HD float smin_poly(float a, float b, float k) {
     return fmn(a, b) - h * h * k * 0.25f;
 }
 
+// Algorithmic FP32 operations (add, sub, mul, div, sqrt, min, max, abs;
+// compares and selects not counted) of one primitive distance, by kind,
+// counted from prim_dist above; an AABB distance test; and the per-sample
+// overhead of a march step (sample point, epsilon / cone update, advance).
+constexpr uint32_t kPrimOps[8] = {10u, 23u, 6u, 24u, 13u, 40u, 57u, 0u};
+constexpr uint32_t kAabbOps = 18u;
+constexpr uint32_t kEvalOps = 15u;
+
 // Per-lane traversal stacks.  Host: a plain array.  Device: a column of a
 // block-wide LDS array (entry i of lane t at base[i * stride]), so a
-// divergent push / pop never spills to scratch memory.
+// divergent push / pop never spills to scratch memory.  The count_* hooks
+// are empty except in CountStack, the instrumented variant that tallies
+// algorithmic operations for the roofline report.
 struct ArrStack {
     uint32_t v[64];
     IFX_HD void put(int i, uint32_t x) { v[i] = x; }
     IFX_HD uint32_t get(int i) const { return v[i]; }
+    IFX_HD void count_prim(uint32_t, uint32_t) {}
+    IFX_HD void count_aabb(uint32_t) {}
+    IFX_HD void count_eval() {}
 };
 
 struct LdsStack {
@@ -163,6 +179,20 @@ struct LdsStack {
     uint32_t stride;
     __device__ __forceinline__ void put(int i, uint32_t x) { base[i * stride] = (uint16_t)x; }
     __device__ __forceinline__ uint32_t get(int i) const { return base[i * stride]; }
+    __device__ __forceinline__ void count_prim(uint32_t, uint32_t) {}
+    __device__ __forceinline__ void count_aabb(uint32_t) {}
+    __device__ __forceinline__ void count_eval() {}
+};
+
+struct CountStack {
+    uint16_t* base;
+    uint32_t stride;
+    unsigned long long ops;
+    __device__ __forceinline__ void put(int i, uint32_t x) { base[i * stride] = (uint16_t)x; }
+    __device__ __forceinline__ uint32_t get(int i) const { return base[i * stride]; }
+    __device__ __forceinline__ void count_prim(uint32_t kind, uint32_t extra) { ops += kPrimOps[kind & 7u] + extra; }
+    __device__ __forceinline__ void count_aabb(uint32_t n) { ops += kAabbOps * n; }
+    __device__ __forceinline__ void count_eval() { ops += kEvalOps; }
 };
 
 // Blend-group state (EXT): nearest hard hit, nearest two blended objects.
@@ -196,13 +226,14 @@ IFX_HD void bvh_query(const SceneView& S, f3 p, BlendState& bs, Stk& stk) {
         uint32_t ni = stk.get(--sp);
         float4 lo = S.nodes[2 * ni], hi = S.nodes[2 * ni + 1];
         float thr = BLEND ? fmn(bs.hard.d, smin_poly(bs.b1.d, bs.b2, S.blend_k)) + S.blend_k : bs.hard.d;
+        stk.count_aabb(1u);
         if (aabb_dist(p, lo, hi) >= thr) continue;
         uint32_t w0 = fbits(lo.w), cnt = fbits(hi.w);
         if (cnt > 0) {
             for (uint32_t k = w0; k < w0 + cnt; ++k) {
                 uint32_t mat;
                 bool bl;
-                float d = obj_dist(S.obj + 4 * k, S.subs, p, mat, bl);
+                float d = obj_dist(S.obj + 4 * k, S.subs, p, mat, bl, stk);
                 if (BLEND) {
                     blend_take(bs, d, mat, bl);
                 } else if (d < bs.hard.d) {
@@ -214,6 +245,7 @@ IFX_HD void bvh_query(const SceneView& S, f3 p, BlendState& bs, Stk& stk) {
             uint32_t l = ni + 1, r = w0;
             float dl = aabb_dist(p, S.nodes[2 * l], S.nodes[2 * l + 1]);
             float dr = aabb_dist(p, S.nodes[2 * r], S.nodes[2 * r + 1]);
+            stk.count_aabb(2u);
             if (dl < dr) {
                 stk.put(sp++, r);
                 stk.put(sp++, l);
@@ -232,10 +264,11 @@ IFX_HD DistHit dist_mat(const SceneView& S, f3 p, Stk& stk) {
     bs.hard = DistHit{kFltMax, 0u};
     bs.b1 = DistHit{kFltMax, 0u};
     bs.b2 = kFltMax;
+    stk.count_eval();
     for (uint32_t i = 0; i < S.n_planes; ++i) {
         uint32_t mat;
         bool bl;
-        float d = obj_dist(S.planes + 4 * i, S.subs, p, mat, bl);
+        float d = obj_dist(S.planes + 4 * i, S.subs, p, mat, bl, stk);
         if (d < bs.hard.d) {
             bs.hard.d = d;
             bs.hard.mat = mat;
@@ -248,7 +281,7 @@ IFX_HD DistHit dist_mat(const SceneView& S, f3 p, Stk& stk) {
             for (uint32_t i = 0; i < S.n_obj; ++i) {
                 uint32_t mat;
                 bool bl;
-                float d = obj_dist(S.obj + 4 * S.lin[i], S.subs, p, mat, bl);
+                float d = obj_dist(S.obj + 4 * S.lin[i], S.subs, p, mat, bl, stk);
                 if (d < bs.hard.d) {
                     bs.hard.d = d;
                     bs.hard.mat = mat;
@@ -263,7 +296,7 @@ IFX_HD DistHit dist_mat(const SceneView& S, f3 p, Stk& stk) {
         for (uint32_t i = 0; i < S.n_obj; ++i) {
             uint32_t mat;
             bool bl;
-            float d = obj_dist(S.obj + 4 * S.lin[i], S.subs, p, mat, bl);
+            float d = obj_dist(S.obj + 4 * S.lin[i], S.subs, p, mat, bl, stk);
             blend_take(bs, d, mat, bl);
         }
     }

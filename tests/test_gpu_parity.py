@@ -204,7 +204,7 @@ def test_shade_modes_match_oracle(product_pair, mode):
     g = ctx.shade(mode, w, h)
     film, ev = sc.render(mode, w, h)
     tonemap = mode in ("beauty", "direct", "indirect")
-    b = po.encode(film, sc.exposure if tonemap else 1.0, tonemap)
+    b = po.encode(film, sc.exposure, tonemap)  # exposure applies in every mode (src/film.rs:51-61)
     frac, worst = image_close(g["rgb8"], b)
     assert frac >= 0.999, (frac, worst)
     assert abs(g["evals"] - ev) <= 1e-4 * ev
