@@ -1,14 +1,17 @@
 """Benchmark of the transfer -> solve -> shade frame on one MI355X.
 
 One step = one frame on resident surfels: transfer (H2D of the surfels,
-clusters, candidate culls, cone-traced link visibility, closure, CSR) +
-solve (direct E, Jacobi) + shade (grid build, per-pixel pass, encode, D2H of
-the RGB8 image).  Surfel placement, parsing and the BVH build precede the
-timed region, as in the reference's stage timers (src/main.rs:432-534).
+clusters, candidate culls, cone-traced link visibility, hemisphere closure,
+CSR) + solve (direct E, Jacobi) + shade (grid build, per-pixel pass, encode,
+D2H of the RGB8 image).  Surfel placement, parsing and the BVH build precede
+the timed region, as in the reference's stage timers (src/main.rs:432-534).
 
 Prints one JSON line.  `value` is Gsdf-evals/s under the reference's
 counting rule (trace step +1, hit normal +4, shadow step +1) over the whole
-frame; ms_per_step is the frame time.
+frame; ms_per_step is the frame time.  The default workload is the literal
+product scene (7 SDF objects incl. a drilled CSG bowl, 2 sphere lights,
+6 Lambertian materials, 640x480, 20000 surfels requested); `--also stress`
+adds the 1920x1080 seeded stress scene as a secondary measurement.
 """
 import argparse
 import json
@@ -22,21 +25,34 @@ sys.path.insert(0, ROOT)
 
 METRIC = "frame ms (transfer+solve+shade) & Gsdf-evals/s at ≤1/255 image err; grads/sec"
 
-# Kernel -> reference stage.  Tracing kernels are FP32-VALU bound; the Jacobi
-# gather is HBM bound.
 TRACE_KERNELS = ["visibility", "closure", "direct_e", "shade"]
 ALL_KERNELS = ["cluster_keys", "seg_flags", "seg_begin", "cluster_build", "cluster_compact", "ff_count", "ff_emit",
-               "visibility", "closure", "row_sum", "compact", "direct_e", "aggregate", "spmv", "grid_keys", "shade",
+               "visibility", "closure", "row_sum", "compact", "direct_e", "direct_sum", "jacobi", "grid_keys", "shade",
                "encode"]
 
 PEAK_VALU_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
-CONFIGS = {
-    "product": {"scene": "tests/golden/scenes/product.rad", "res": 8,
-                "workload": "product.rad (7 SDF objects incl. CSG, 2 quad lights, 6 materials), 640x480, "
-                            "20000 surfels requested, cluster res 8, Jacobi cap 16, beauty"},
-}
+
+def _cfg(name):
+    from interflect_amd import scenes
+
+    if name == "product":
+        return {"text": scenes.product_literal(), "res": 8, "sample": None,
+                "workload": "product scene: sphere, bowl = sphere - sphere - 6 drilled cylinders, torus R.4 r.12, "
+                            "capsule len .8 r.15, box floor + 2 box walls; 2 sphere lights; 6 Lambertian materials; "
+                            "640x480; 20000 surfels requested; cluster res 8; 12 bounces; beauty"}
+    if name == "product_rad":
+        return {"text": open(os.path.join(ROOT, "tests", "golden", "scenes", "product.rad")).read(), "res": 8,
+                "sample": None, "workload": "scenes/product.rad of the reference, 640x480, 20000 surfels requested"}
+    if name == "cornell_unit":
+        return {"text": scenes.cornell_unit(), "res": 8, "sample": None,
+                "workload": "unit Cornell box, 2 spheres r.3, quad light emit 15, 260x260, 20000 surfels"}
+    if name == "stress":
+        return {"text": scenes.stress(), "res": 8, "sample": {"rows": 200, "surfels": 4000, "pixel_rows": 4},
+                "workload": "seeded stress scene (0x5EED): 256 random sphere/box/torus/capsule prims in a 10x4x10 "
+                            "room, 25% smooth-union k=.05, 16 albedos, 4 point lights, 200000 surfels, 1920x1080"}
+    raise SystemExit("unknown config " + name)
 
 
 def cpu_model():
@@ -56,56 +72,72 @@ def run_frame(ctx, res, bounces):
     return t, s, r
 
 
-def cpu_baseline(scene_text, res, threads):
-    """The oracle (C++ restatement of the reference CPU path) on the host cores:
-    one full frame of the same workload, after one untimed warm-up frame."""
+def cpu_baseline(cfg, surfels, threads, gpu_frame):
+    """The oracle (C++ restatement of the reference CPU path) on the host
+    cores, on the GPU run's surfels.  Small scenes: one full frame after one
+    warm-up frame.  Large scenes: a bounded sample of every stage (transfer
+    rows, direct-light surfels, film rows), each stage's rate applied to the
+    GPU frame's evaluation counts for the frame estimate."""
     from oracle import pyoracle as po
 
-    sc = po.Scene(scene_text, threads=threads)
-    sc.generate()
-    sc.transfer(res)
-    sc.solve()
+    sc = po.Scene(cfg["text"], threads=threads)
+    sc.set_surfels(*surfels)
+    smp = cfg["sample"]
+    if smp is None:
+        sc.transfer(cfg["res"])
+        sc.solve()
+        t0 = time.perf_counter()
+        tr = sc.transfer(cfg["res"])
+        t1 = time.perf_counter()
+        so = sc.solve()
+        t2 = time.perf_counter()
+        _, ev = sc.render("beauty")
+        t3 = time.perf_counter()
+        evals = tr["evals"] + so["evals"] + ev
+        return {"value": evals / (t3 - t0) / 1e9, "unit": "Gsdf-evals/s", "cores": threads, "kind": "port",
+                "sample": "one full frame of the same workload (transfer+solve+shade) on %d threads of %s" % (
+                    threads, cpu_model()),
+                "frame_ms": (t3 - t0) * 1e3,
+                "stages_ms": {"transfer": (t1 - t0) * 1e3, "solve": (t2 - t1) * 1e3, "shade": (t3 - t2) * 1e3}}
+    p, n, a, ar = surfels
     t0 = time.perf_counter()
-    tr = sc.transfer(res)
+    tr = sc.transfer(cfg["res"], rows=(0, smp["rows"]))
     t1 = time.perf_counter()
-    so = sc.solve()
+    k = smp["surfels"]
+    _, st = sc.direct_diffuse(p[:k], n[:k], a[:k])
     t2 = time.perf_counter()
-    _, ev = sc.render("beauty")
+    y0 = sc.height // 2
+    _, ev = sc.render("direct", rows=(y0, y0 + smp["pixel_rows"]), use_gi=False)
     t3 = time.perf_counter()
-    evals = tr["evals"] + so["evals"] + ev
-    frame_ms = (t3 - t0) * 1e3
+    rates = {"transfer": tr["evals"] / (t1 - t0), "solve": float(st.sum()) / (t2 - t1), "shade": ev / (t3 - t2)}
+    evals = tr["evals"] + float(st.sum()) + ev
+    est = {s: gpu_frame[s] / rates[s] * 1e3 for s in rates}
     return {"value": evals / (t3 - t0) / 1e9, "unit": "Gsdf-evals/s", "cores": threads, "kind": "port",
-            "sample": "one full frame (transfer+solve+shade) of the same workload on %d threads of %s" % (
-                threads, cpu_model()),
-            "frame_ms": frame_ms, "stages_ms": {"transfer": (t1 - t0) * 1e3, "solve": (t2 - t1) * 1e3,
-                                                "shade": (t3 - t2) * 1e3}}
+            "sample": "%d transfer rows, direct light of %d surfels, %d film rows (direct mode) of the same scene "
+                      "and surfels, %.1f s on %d threads of %s" % (smp["rows"], k, smp["pixel_rows"], t3 - t0,
+                                                                   threads, cpu_model()),
+            "stage_rates_gevals_s": {s: r / 1e9 for s, r in rates.items()},
+            "frame_ms": sum(est.values()), "frame_ms_is_estimate": True,
+            "stages_ms": est}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="product", choices=sorted(CONFIGS))
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU only: this path has no multi-GPU mode (see DESIGN.md, sharding)",
-              file=sys.stderr)
-        sys.exit(2)
+def note(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
-    import numpy as np
 
+def measure(name, steps, warmup, with_cpu):
     import interflect_amd as ia
 
-    cfg = CONFIGS[args.config]
-    text = open(os.path.join(ROOT, cfg["scene"])).read()
+    cfg = _cfg(name)
+    note(name, "placing surfels")
     ctx = ia.Context(0)
-    info = ctx.load_scene(text)
+    info = ctx.load_scene(cfg["text"])
     placed = ctx.generate_surfels()
     bounces = info["bounces"]
-    for _ in range(args.warmup):
+    note(name, "%d surfels placed; warm-up" % placed)
+    for _ in range(warmup):
         run_frame(ctx, cfg["res"], bounces)
+        note(name, "warm-up frame done")
 
     # One untimed instrumented frame: algorithmic FP32 operations per launch
     # of each tracing kernel (same traversal, counting stack).
@@ -114,86 +146,119 @@ def main():
     run_frame(ctx, cfg["res"], bounces)
     ctx.set_counting(False)
     ops = {k: ctx.kernel_ops(k) for k in TRACE_KERNELS}
+    note(name, "instrumented frame done")
 
     ctx.set_profiling(True)
     ctx.reset_kernel_times()
     stage = {"transfer": 0.0, "solve": 0.0, "shade": 0.0}
-    evals = 0
+    sev = {"transfer": 0, "solve": 0, "shade": 0}
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         t, s, r = run_frame(ctx, cfg["res"], bounces)
         stage["transfer"] += t["ms"]
         stage["solve"] += s["ms"]
         stage["shade"] += r["ms"]
-        evals += t["evals"] + s["evals"] + r["evals"]
+        sev["transfer"] += t["evals"]
+        sev["solve"] += s["evals"]
+        sev["shade"] += r["evals"]
+        note(name, "timed frame done")
     wall = time.perf_counter() - t0
     ctx.set_profiling(False)
-    ms_per_step = wall * 1e3 / args.steps
+    ms_per_step = wall * 1e3 / steps
+    evals = sum(sev.values())
 
     ktime = {}
     for k in ALL_KERNELS:
         n, ms = ctx.kernel_time(k)
         if n:
-            ktime[k] = {"launches": n, "avg_ms": ms / n, "total_ms_per_frame": ms / args.steps}
+            ktime[k] = {"launches": n, "avg_ms": ms / n, "total_ms_per_frame": ms / steps}
     dominant = max(ktime, key=lambda k: ktime[k]["total_ms_per_frame"])
 
-    # SpMV roofline: algorithmic bytes per Jacobi bounce 8L + 64N + 12C for the
-    # gather (idx+w per link; starts, direct, albedo, rad in/out, irr out per
-    # row; one Bc read per cluster).
+    # Jacobi roofline: one cooperative launch runs every bounce.  Algorithmic
+    # bytes per bounce 8L + 84N + 24C: idx+w per link; starts, direct,
+    # albedo, rad in/out and irr out per row plus member index, rad and area
+    # for the cluster sums; Bc written and read once per cluster.
     N, L, C = placed, t["links"], t["clusters"]
-    spmv_bytes = 8 * L + 64 * N + 12 * C
-    sp = ktime.get("spmv")
-    spmv_roof = None
-    if sp:
-        gbs = spmv_bytes / (sp["avg_ms"] * 1e-3) / 1e9
-        spmv_roof = {"kernel": "spmv", "bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": gbs / PEAK_HBM_GBS, "traffic": None, "bytes_per_launch": spmv_bytes}
-
+    jac_bytes = (8 * L + 84 * N + 24 * C) * s["iterations"]
+    jk = ktime.get("jacobi")
+    jac_roof = None
+    if jk:
+        gbs = jac_bytes / (jk["avg_ms"] * 1e-3) / 1e9
+        jac_roof = {"kernel": "jacobi", "bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": gbs / PEAK_HBM_GBS, "traffic": None, "bytes_per_launch": jac_bytes}
     roof = None
     if dominant in TRACE_KERNELS and ops.get(dominant):
         tfl = ops[dominant] / (ktime[dominant]["avg_ms"] * 1e-3) / 1e12
         roof = {"kernel": dominant, "bound": "valu", "achieved": tfl, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
                 "frac": tfl / PEAK_VALU_TFLOPS, "traffic": None, "ops_per_launch": ops[dominant]}
-    elif dominant == "spmv":
-        roof = spmv_roof
+    elif dominant == "jacobi":
+        roof = jac_roof
     trace_roofs = {}
     for k in TRACE_KERNELS:
         if k in ktime and ops.get(k):
             tfl = ops[k] / (ktime[k]["avg_ms"] * 1e-3) / 1e12
             trace_roofs[k] = {"achieved_tflops": tfl, "frac": tfl / PEAK_VALU_TFLOPS, "ops_per_launch": ops[k],
                               "avg_ms": ktime[k]["avg_ms"]}
-
-    frame_evals = evals / args.steps
     out = {
-        "metric": METRIC,
         "value": evals / wall / 1e9,
+        "ms_per_step": ms_per_step,
+        "config": {"workload": cfg["workload"], "surfels_placed": placed, "clusters": C, "links": L,
+                   "candidates": t["candidates"], "bounces": s["iterations"], "frame_evals": evals / steps,
+                   "width": info["width"], "height": info["height"]},
+        "stages_ms": {k: v / steps for k, v in stage.items()},
+        "stage_evals": {k: v / steps for k, v in sev.items()},
+        "kernels": ktime,
+        "dominant_kernel": dominant,
+        "roofline": roof,
+        "roofline_jacobi": jac_roof,
+        "roofline_tracing": trace_roofs,
+    }
+    if with_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        note(name, "CPU baseline on %d threads" % threads)
+        cb = cpu_baseline(cfg, ctx.surfels(), threads, out["stage_evals"])
+        cb["speedup_frame"] = cb["frame_ms"] / ms_per_step
+        cb["speedup_throughput"] = out["value"] / cb["value"]
+        out["cpu_baseline"] = cb
+    else:
+        out["cpu_baseline"] = None
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="product", choices=["product", "product_rad", "cornell_unit", "stress"])
+    ap.add_argument("--also", default="", help="comma-separated secondary configs (e.g. stress)")
+    ap.add_argument("--also-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this path has no multi-GPU mode (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    m = measure(args.config, args.steps, args.warmup, not args.no_cpu)
+    line = {
+        "metric": METRIC,
+        "value": m["value"],
         "unit": "Gsdf-evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: scene file + deterministic Halton surfel placement (no dataset)",
-        "config": {"workload": cfg["workload"], "surfels_placed": placed, "clusters": C, "links": L,
-                   "bounces": s["iterations"], "frame_evals": frame_evals},
-        "stages_ms": {k: v / args.steps for k, v in stage.items()},
-        "kernels": ktime,
-        "dominant_kernel": dominant,
-        "roofline": roof,
-        "roofline_spmv": spmv_roof,
-        "roofline_tracing": trace_roofs,
+        "data": "synthetic: seeded scene + deterministic Halton surfel placement (no dataset)",
     }
-    if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        cb = cpu_baseline(text, cfg["res"], threads)
-        cb["speedup_frame"] = cb["frame_ms"] / ms_per_step
-        out["cpu_baseline"] = cb
-    else:
-        out["cpu_baseline"] = None
-    print(json.dumps(out))
+    line.update({k: v for k, v in m.items() if k not in ("value", "ms_per_step")})
+    for extra in [x for x in args.also.split(",") if x]:
+        line.setdefault("secondary", {})[extra] = measure(extra, args.also_steps, 1, not args.no_cpu)
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

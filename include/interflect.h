@@ -138,6 +138,13 @@ ifx_status ifx_reset_kernel_times(ifx_ctx* ctx);
  * operations; results are identical, timings are not representative.
  * No counterpart in the reference; feeds the roofline report. */
 ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
+/* Launch-configuration knobs (no counterpart in the reference; results are
+ * identical for every setting, only speed changes):
+ *   "lds"           1 = stage scene records in LDS when they fit (default), 0 = global
+ *   "vis_blocks"    grid of the link-visibility kernel (0 = one lane per candidate)
+ *   "ff_blocks"     grid of the candidate-cull kernels (0 = automatic)
+ *   "jacobi_blocks" grid of the cooperative Jacobi kernel (0 = automatic) */
+ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value);
 ifx_status ifx_kernel_ops(ifx_ctx* ctx, const char* name, uint64_t* ops);
 
 #ifdef __cplusplus

@@ -32,7 +32,7 @@ EXPORTS = [
     "ifx_set_use_bvh", "ifx_generate_surfels", "ifx_set_surfels", "ifx_num_surfels", "ifx_get_surfels",
     "ifx_transfer", "ifx_get_transfer", "ifx_get_clusters", "ifx_solve", "ifx_get_radiosity", "ifx_set_radiosity",
     "ifx_shade", "ifx_dist_mat", "ifx_trace_rays", "ifx_soft_shadows", "ifx_set_profiling", "ifx_kernel_time",
-    "ifx_reset_kernel_times", "ifx_set_counting", "ifx_kernel_ops",
+    "ifx_reset_kernel_times", "ifx_set_counting", "ifx_kernel_ops", "ifx_set_tuning",
 ]
 
 
@@ -106,6 +106,7 @@ def lib():
         "ifx_reset_kernel_times": [vp],
         "ifx_set_counting": [vp, i32],
         "ifx_kernel_ops": [vp, C.c_char_p, C.POINTER(u64)],
+        "ifx_set_tuning": [vp, C.c_char_p, C.c_int64],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -316,6 +317,9 @@ class Context:
 
     def reset_kernel_times(self):
         self._check(lib().ifx_reset_kernel_times(self._h))
+
+    def set_tuning(self, key, value):
+        self._check(lib().ifx_set_tuning(self._h, key.encode(), int(value)))
 
     def set_counting(self, on):
         self._check(lib().ifx_set_counting(self._h, 1 if on else 0))

@@ -338,6 +338,26 @@ ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on) {
     });
 }
 
+ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (!key) return IFX_E_INVALID;
+        std::string k(key);
+        if (value < 0 || value > (int64_t)1 << 30) {
+            ctx->err = "tuning value out of range";
+            return IFX_E_INVALID;
+        }
+        if (k == "lds") ctx->eng.use_lds = value != 0;
+        else if (k == "vis_blocks") ctx->eng.vis_blocks = (uint32_t)value;
+        else if (k == "ff_blocks") ctx->eng.ff_blocks = (uint32_t)value;
+        else if (k == "jacobi_blocks") ctx->eng.jacobi_blocks = (uint32_t)value;
+        else {
+            ctx->err = "unknown tuning key `" + k + "`";
+            return IFX_E_INVALID;
+        }
+        return IFX_OK;
+    });
+}
+
 ifx_status ifx_kernel_ops(ifx_ctx* ctx, const char* name, uint64_t* ops) {
     return guarded(ctx, [&]() -> ifx_status {
         if (!name || !ops) return IFX_E_INVALID;

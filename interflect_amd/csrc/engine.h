@@ -12,8 +12,12 @@
 
 namespace ifx {
 
+struct LdsPlan {
+    uint32_t scene_f4, light_f4;  // float4s of scene / light records staged in LDS (0 = global)
+};
+
 constexpr int kBlock = 256;       // threads per block of every traversal kernel
-constexpr int kStackCap = 32;     // LDS traversal-stack entries per lane
+constexpr int kStackCap = 64;     // deepest traversal stack the kernels accept (the reference's bound)
 
 // Grow-only device allocation.
 struct DevBuf {
@@ -89,7 +93,16 @@ class Engine {
     uint64_t n_links = 0;
     hipStream_t stream = nullptr;
 
+    bool use_lds = true;  // stage scene records in LDS when they fit
+    uint32_t vis_blocks = 0, ff_blocks = 0, jacobi_blocks = 0;  // 0 = automatic
+
    private:
+    struct Lds {
+        LdsPlan plan;
+        size_t bytes;
+        bool staged;
+    };
+    Lds plan_lds(bool lights) const;
     int check(hipError_t e, const char* what, std::string& err);
     int need_device(std::string& err) const;
     SceneView dev_view() const;
@@ -99,6 +112,7 @@ class Engine {
     void drain_events();
 
     int device_ = 0;
+    int n_cu_ = 1;
     bool has_device_ = false;
     bool prof_ = false;
     bool count_ops_ = false;
@@ -122,7 +136,7 @@ class Engine {
     // candidates and links
     DevBuf d_ccount, d_cstart, d_cand_ci, d_cand_row, d_cand_v, d_sky, d_lcount, d_lstart, d_scale, d_rowsum, d_idx, d_w;
     // solve
-    DevBuf d_direct, d_irr, d_rad, d_bc, d_resid;
+    DevBuf d_direct, d_irr, d_rad, d_bc, d_resid, d_terms, d_sync;
     // shade
     DevBuf d_gkey, d_gkey2, d_gval, d_gval2, d_gstart, d_film, d_rgb8;
     // misc

@@ -71,9 +71,63 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-#define IFX_LDS_STACK()                                  \
-    __shared__ uint16_t stk_mem[kStackCap * kBlock];     \
-    LdsStack stk{stk_mem + threadIdx.x, (uint32_t)kBlock}
+// Dynamic LDS of every traversal kernel: [scene records][lights, materials]
+// [traversal stack].  Staging the scene puts every node / object / light
+// read of the divergent traversal on the LDS path instead of the vector
+// cache; scenes too large for the budget are read from global memory and
+// only the stack lives in LDS.
+constexpr uint32_t kLdsBudget = 64u * 1024u;
+
+__device__ __forceinline__ void copy_f4(float4* dst, const float4* __restrict__ src, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+// Copies the scene (and lights) into LDS and repoints the views.  Every
+// thread of the block must call it (it ends in a barrier).
+template <bool LDS, bool LIGHTS>
+__device__ __forceinline__ void stage_in(float4* lds, SceneView& S, LightsView& L) {
+    if constexpr (LDS) {
+        float4* q = lds;
+        uint32_t n = 4u * S.n_obj;
+        copy_f4(q, S.obj, n);
+        S.obj = q;
+        q += n;
+        n = 4u * S.n_planes;
+        copy_f4(q, S.planes, n);
+        S.planes = q;
+        q += n;
+        n = 4u * S.n_subs;
+        copy_f4(q, S.subs, n);
+        S.subs = q;
+        q += n;
+        n = 2u * S.n_nodes;
+        copy_f4(q, S.nodes, n);
+        S.nodes = q;
+        q += n;
+        n = (S.n_obj + 3u) / 4u;
+        copy_f4(q, (const float4*)S.lin, n);
+        S.lin = (const uint32_t*)q;
+        q += n;
+        if constexpr (LIGHTS) {
+            n = 7u * L.n_quad;
+            copy_f4(q, L.quad, n);
+            L.quad = q;
+            q += n;
+            n = 2u * L.n_sph;
+            copy_f4(q, L.sph, n);
+            L.sph = q;
+            q += n;
+            n = 2u * L.n_pnt;
+            copy_f4(q, L.pnt, n);
+            L.pnt = q;
+            q += n;
+            n = 4u * L.n_mat;
+            copy_f4(q, L.mat, n);
+            L.mat = q;
+        }
+        __syncthreads();
+    }
+}
 
 // Timed kernels traverse with LdsStack; the instrumented instantiation
 // (COUNT=true) uses CountStack and adds its operation tally to `ops`.
@@ -92,12 +146,17 @@ __device__ __forceinline__ void stk_flush(CountStack& s, unsigned long long* ops
     if (s.ops) atomicAdd(ops, s.ops);
 }
 
-#define IFX_STACK(COUNT)                                 \
-    __shared__ uint16_t stk_mem[kStackCap * kBlock];     \
-    typename StackOf<COUNT>::T stk;                      \
-    stk.base = stk_mem + threadIdx.x;                    \
-    stk.stride = (uint32_t)kBlock;                       \
-    stk_init(stk)
+// Kernel prologue: stage, then place this lane's stack column after the staged records.
+#define IFX_SETUP(LDS, LIGHTS, COUNT)                                            \
+    extern __shared__ float4 ifx_lds[];                                         \
+    SceneView S = Sg;                                                           \
+    LightsView L = Lg;                                                          \
+    stage_in<LDS, LIGHTS>(ifx_lds, S, L);                                       \
+    typename StackOf<COUNT>::T stk;                                             \
+    stk.base = (uint16_t*)(ifx_lds + P.scene_f4 + P.light_f4) + threadIdx.x;    \
+    stk.stride = (uint32_t)kBlock;                                              \
+    stk_init(stk);                                                              \
+    (void)L
 
 // src/formfactor.rs:70-89
 __device__ __forceinline__ uint32_t normal_bucket(f3 n) {
@@ -123,9 +182,10 @@ __device__ __forceinline__ bool ff_geom(f3 sp, f3 sn, float4 cp, float4 cn, floa
 }
 
 // ====================================================================== probes
-__global__ __launch_bounds__(kBlock) void k_probe_dist(SceneView S, const float* pts, uint32_t n, float* d,
-                                                      uint32_t* mat, float* nrm) {
-    IFX_LDS_STACK();
+template <bool LDS>
+__global__ __launch_bounds__(kBlock) void k_probe_dist(SceneView Sg, LightsView Lg, LdsPlan P, const float* pts,
+                                                      uint32_t n, float* d, uint32_t* mat, float* nrm) {
+    IFX_SETUP(LDS, false, false);
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     f3 p = mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
@@ -138,10 +198,11 @@ __global__ __launch_bounds__(kBlock) void k_probe_dist(SceneView S, const float*
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_probe_trace(SceneView S, const float* ro, const float* rd, uint32_t n,
-                                                       float tmin, float tmax, float* t, float* nrm, uint32_t* mat,
-                                                       int32_t* hit, uint32_t* steps) {
-    IFX_LDS_STACK();
+template <bool LDS>
+__global__ __launch_bounds__(kBlock) void k_probe_trace(SceneView Sg, LightsView Lg, LdsPlan P, const float* ro,
+                                                       const float* rd, uint32_t n, float tmin, float tmax, float* t,
+                                                       float* nrm, uint32_t* mat, int32_t* hit, uint32_t* steps) {
+    IFX_SETUP(LDS, false, false);
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     uint32_t s = 0;
@@ -154,10 +215,11 @@ __global__ __launch_bounds__(kBlock) void k_probe_trace(SceneView S, const float
     steps[i] = s;
 }
 
-__global__ __launch_bounds__(kBlock) void k_probe_shadow(SceneView S, const float* ro, const float* rd, uint32_t n,
-                                                        float tmin, const float* tmax, const float* k, float* vis,
-                                                        uint32_t* steps) {
-    IFX_LDS_STACK();
+template <bool LDS>
+__global__ __launch_bounds__(kBlock) void k_probe_shadow(SceneView Sg, LightsView Lg, LdsPlan P, const float* ro,
+                                                        const float* rd, uint32_t n, float tmin, const float* tmax,
+                                                        const float* k, float* vis, uint32_t* steps) {
+    IFX_SETUP(LDS, false, false);
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     uint32_t s = 0;
@@ -266,8 +328,8 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
 
 // Cone-traced link visibility, one candidate per lane (src/formfactor.rs:194-214).
 // cand_v = ff * vis for a kept link, 0 for a culled one.
-template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_visibility(SceneView S, const float4* __restrict__ spos,
+template <bool COUNT, bool LDS>
+__global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView Lg, LdsPlan P, const float4* __restrict__ spos,
                                                        const float4* __restrict__ snrm,
                                                        const float4* __restrict__ cpos,
                                                        const float4* __restrict__ cnrm,
@@ -275,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(SceneView S, const float4
                                                        const uint32_t* __restrict__ cand_row, uint32_t M,
                                                        float off, float tmin, float* cand_v,
                                                        unsigned long long* evals, unsigned long long* ops) {
-    IFX_STACK(COUNT);
+    IFX_SETUP(LDS, false, COUNT);
     uint32_t acc = 0;
     uint32_t stride = gridDim.x * kBlock;
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < M; k += stride) {
@@ -297,12 +359,12 @@ __global__ __launch_bounds__(kBlock) void k_visibility(SceneView S, const float4
 
 // Hemisphere closure: 16 lanes per surfel, one Halton direction each
 // (src/trace.rs:165-186).  dirs[k] = (r cos phi, r sin phi, sqrt(1 - u1)).
-template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_closure(SceneView S, const float4* __restrict__ spos,
+template <bool COUNT, bool LDS>
+__global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg, LdsPlan P, const float4* __restrict__ spos,
                                                     const float4* __restrict__ snrm, uint32_t N,
                                                     const float4* __restrict__ dirs, float tmax, float* sky,
                                                     unsigned long long* evals, unsigned long long* ops) {
-    IFX_STACK(COUNT);
+    IFX_SETUP(LDS, false, COUNT);
     uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
     uint32_t row = tid >> 4, k = tid & 15u;
     bool active = row < N;
@@ -368,67 +430,135 @@ __global__ void k_compact(uint32_t N, const uint32_t* __restrict__ cstart, const
 }
 
 // ====================================================================== solve
-template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_direct_e(SceneView S, LightsView L, const float4* __restrict__ spos,
-                                                     const float4* __restrict__ snrm, const float4* __restrict__ salb,
-                                                     uint32_t N, float4* direct, float4* rad, float4* irr,
-                                                     unsigned long long* evals, unsigned long long* ops) {
-    IFX_STACK(COUNT);
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+// Direct light E, one lane per (light, surfel) pair: the cone shadow of each
+// light runs in parallel and k_direct_sum adds the kept terms per surfel in
+// the reference's light order (src/shade.rs:137-162, src/solve.rs:27-39).
+template <bool COUNT, bool LDS>
+__global__ __launch_bounds__(kBlock) void k_direct_terms(SceneView Sg, LightsView Lg, LdsPlan P,
+                                                         const float4* __restrict__ spos,
+                                                         const float4* __restrict__ snrm,
+                                                         const float4* __restrict__ salb, uint32_t N, uint32_t nl,
+                                                         float4* terms, unsigned long long* evals,
+                                                         unsigned long long* ops) {
+    IFX_SETUP(LDS, true, COUNT);
+    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     uint32_t steps = 0;
-    if (i < N) {
-        f3 e = direct_diffuse(S, L, ld3(spos[i]), ld3(snrm[i]), ld3(salb[i]), steps, stk);
-        direct[i] = st4(e);
-        rad[i] = st4(e);
-        irr[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (t < N * nl) {
+        uint32_t li = t / N, i = t - li * N;
+        f3 p = ld3(spos[i]), n = ld3(snrm[i]);
+        f3 tan, bit;
+        frame(n, tan, bit);
+        f3 term = spl(0.0f);
+        bool on = diffuse_term(S, L, li, p, n, tan, bit, ld3(salb[i]), term, steps, stk);
+        terms[t] = on ? st4(term, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     count_steps(evals, steps);
     stk_flush(stk, ops);
 }
 
-// Jacobi bounce `it` is a no-op once bounce it-1 converged (residual < 1e-4),
-// which reproduces the reference's early `break` with a fixed launch count.
-__device__ __forceinline__ bool converged_before(const uint32_t* resid, uint32_t it) {
-    return it > 0 && bitsf(resid[it - 1]) < 1e-4f;
+__global__ void k_direct_sum(uint32_t N, uint32_t nl, const float4* __restrict__ terms, float4* direct, float4* rad,
+                             float4* irr) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    f3 out = spl(0.0f);
+    for (uint32_t li = 0; li < nl; ++li) {
+        float4 t = terms[(size_t)li * N + i];
+        if (t.w != 0.0f) out = out + ld3(t);
+    }
+    direct[i] = st4(out);
+    rad[i] = st4(out);
+    irr[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// Cluster radiosity: serial area-weighted sum over members (src/solve.rs:66-73).
-__global__ void k_aggregate(uint32_t it, const uint32_t* resid, uint32_t C, const uint2* __restrict__ range,
-                            const uint32_t* __restrict__ members, const float4* __restrict__ spos,
-                            const float4* __restrict__ cpos, const float4* __restrict__ rad, float4* bc) {
-    if (converged_before(resid, it)) return;
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C) return;
-    uint2 r = range[c];
-    f3 acc = spl(0.0f);
-    for (uint32_t k = r.x; k < r.y; ++k) {
-        uint32_t m = members[k];
-        acc = acc + ld3(rad[m]) * spos[m].w;
+// Jacobi bounces B = E + rho * W (G B) as one cooperative persistent launch
+// (src/solve.rs:60-102).  Per bounce: cluster radiosity G B, one wavefront
+// per cluster summing its members strictly in member order; a grid barrier;
+// the row gather in link order with the max-norm update folded into an
+// integer max of non-negative float bits; a grid barrier; then every lane
+// reads the same residual and leaves together once it is below 1e-4.
+struct JacobiArgs {
+    uint32_t N, C, bounces;
+    const uint2* range;
+    const uint32_t* members;
+    const float4* spos;
+    const float4* cpos;
+    const uint32_t* lstart;
+    const uint32_t* idx;
+    const float* w;
+    const float4* direct;
+    const float4* salb;
+    float4* rad;
+    float4* irr;
+    float4* bc;
+    uint32_t* resid;  // [bounces] max |delta| bits
+    uint32_t* sync;   // [0] barrier arrivals, [1] fault flag
+};
+
+// Grid-wide barrier for a cooperatively launched (co-resident) grid.  The
+// wait is bounded: on timeout the fault flag is raised and the block goes on,
+// so every wave still reaches the kernel's end.
+__device__ __forceinline__ void grid_barrier(uint32_t* sync, uint32_t& target) {
+    __threadfence();
+    __syncthreads();
+    target += gridDim.x;
+    if (threadIdx.x == 0) {
+        atomicAdd(&sync[0], 1u);
+        uint32_t spins = 0;
+        while (__hip_atomic_load(&sync[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins > (1u << 22)) {
+                atomicOr(&sync[1], 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
     }
-    bc[c] = st4(acc / fmx(cpos[c].w, 1e-12f));
+    __syncthreads();
+    __threadfence();
 }
 
-// Row gather B = E + rho * sum_k w_k Bc[idx_k] in link order, and the
-// max-norm update as an integer max of non-negative float bits (src/solve.rs:76-93).
-__global__ __launch_bounds__(kBlock) void k_spmv(uint32_t it, uint32_t* resid, uint32_t N,
-                                                 const uint32_t* __restrict__ lstart,
-                                                 const uint32_t* __restrict__ idx, const float* __restrict__ w,
-                                                 const float4* __restrict__ bc, const float4* __restrict__ direct,
-                                                 const float4* __restrict__ salb, float4* rad, float4* irr) {
-    if (converged_before(resid, it)) return;
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t delta = 0;
-    if (i < N) {
-        f3 g = spl(0.0f);
-        uint32_t e = lstart[i + 1];
-        for (uint32_t k = lstart[i]; k < e; ++k) g = g + ld3(bc[idx[k]]) * w[k];
-        f3 nr = ld3(direct[i]) + ld3(salb[i]) * g;
-        delta = fbits(maxe(abs3(nr - ld3(rad[i]))));
-        irr[i] = st4(g);
-        rad[i] = st4(nr);
+__global__ __launch_bounds__(kBlock) void k_jacobi(JacobiArgs A) {
+    const uint32_t lane = lane_id();
+    const uint32_t gt = blockIdx.x * kBlock + threadIdx.x, nt = gridDim.x * kBlock;
+    const uint32_t gw = gt >> 6, nw = nt >> 6;
+    uint32_t target = 0;
+    for (uint32_t it = 0; it < A.bounces; ++it) {
+        for (uint32_t c = gw; c < A.C; c += nw) {
+            uint2 r = A.range[c];
+            f3 acc = spl(0.0f);
+            for (uint32_t b = r.x; b < r.y; b += 64u) {
+                uint32_t k = b + lane;
+                f3 v = spl(0.0f);
+                if (k < r.y) {
+                    uint32_t m = A.members[k];
+                    v = ld3(A.rad[m]) * A.spos[m].w;
+                }
+                uint32_t cnt = r.y - b < 64u ? r.y - b : 64u;
+                for (uint32_t j = 0; j < cnt; ++j) {
+                    f3 vj = mk(bitsf(__builtin_amdgcn_readlane(fbits(v.x), j)),
+                               bitsf(__builtin_amdgcn_readlane(fbits(v.y), j)),
+                               bitsf(__builtin_amdgcn_readlane(fbits(v.z), j)));
+                    acc = acc + vj;
+                }
+            }
+            if (lane == 0) A.bc[c] = st4(acc / fmx(A.cpos[c].w, 1e-12f));
+        }
+        grid_barrier(A.sync, target);
+        uint32_t delta = 0;
+        for (uint32_t i = gt; i < A.N; i += nt) {
+            f3 g = spl(0.0f);
+            uint32_t e = A.lstart[i + 1];
+            for (uint32_t k = A.lstart[i]; k < e; ++k) g = g + ld3(A.bc[A.idx[k]]) * A.w[k];
+            f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
+            uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
+            delta = d > delta ? d : delta;
+            A.irr[i] = st4(g);
+            A.rad[i] = st4(nr);
+        }
+        uint32_t m = wave_max(delta);
+        if (lane == 0 && m) atomicMax(&A.resid[it], m);
+        grid_barrier(A.sync, target);
+        if (bitsf(__hip_atomic_load(&A.resid[it], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < 1e-4f) break;
     }
-    uint32_t m = wave_max(delta);
-    if (lane_id() == 0 && m) atomicMax(&resid[it], m);
 }
 
 // ====================================================================== shade
@@ -467,7 +597,7 @@ enum { M_BEAUTY = 0, M_DIRECT = 1, M_INDIRECT = 2, M_NORMALS = 3, M_DEPTH = 4, M
 
 // One pixel of render_frame (src/main.rs:247-361).
 template <class Stk>
-__device__ f3 shade_pixel(const SceneView& S, const LightsView& L, const ShadeArgs& A, uint32_t x, uint32_t y,
+__device__ __forceinline__ f3 shade_pixel(const SceneView& S, const LightsView& L, const ShadeArgs& A, uint32_t x, uint32_t y,
                           uint32_t& local, Stk& stk) {
     const f3 bg = mk(0.02f, 0.03f, 0.05f);
     const CamV& cam = A.cam;
@@ -531,10 +661,10 @@ __device__ f3 shade_pixel(const SceneView& S, const LightsView& L, const ShadeAr
 }
 
 // 16x16 pixel blocks, one 8x8 tile per wavefront.
-template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_shade(SceneView S, LightsView L, ShadeArgs A, float4* film,
+template <bool COUNT, bool LDS>
+__global__ __launch_bounds__(kBlock) void k_shade(SceneView Sg, LightsView Lg, LdsPlan P, ShadeArgs A, float4* film,
                                                   unsigned long long* evals, unsigned long long* ops) {
-    IFX_STACK(COUNT);
+    IFX_SETUP(LDS, true, COUNT);
     uint32_t wv = threadIdx.x >> 6, l = threadIdx.x & 63u;
     uint32_t x = blockIdx.x * 16u + (wv & 1u) * 8u + (l & 7u);
     uint32_t y = blockIdx.y * 16u + (wv >> 1) * 8u + (l >> 3);
@@ -595,6 +725,52 @@ float host_srgb(float c) {
 }
 }  // namespace
 
+#define LAUNCH_CL(K, COUNT, LDS, GRID, DYN, ...)                                          \
+    do {                                                                                 \
+        if (COUNT) {                                                                     \
+            if (LDS)                                                                     \
+                K<true, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);               \
+            else                                                                         \
+                K<true, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);              \
+        } else {                                                                         \
+            if (LDS)                                                                     \
+                K<false, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);              \
+            else                                                                         \
+                K<false, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);             \
+        }                                                                                \
+    } while (0)
+
+#define LAUNCH_L(K, LDS, GRID, DYN, ...)                                                 \
+    do {                                                                                 \
+        if (LDS)                                                                         \
+            K<true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);                         \
+        else                                                                             \
+            K<false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);                        \
+    } while (0)
+
+Engine::Lds Engine::plan_lds(bool lights) const {
+    const HostScene& s = scene;
+    const uint32_t n_obj = (uint32_t)s.objects.size();
+    uint32_t scene_f4 = (uint32_t)(s.obj_rec.size() + s.plane_rec.size() + s.sub_rec.size() + s.node_rec.size()) +
+                        (n_obj + 3u) / 4u;
+    uint32_t light_f4 =
+        lights ? (uint32_t)(s.quad_rec.size() + s.sph_rec.size() + s.pnt_rec.size() + s.mat_rec.size()) : 0u;
+    size_t stack = (size_t)kBlock * (s.max_stack ? s.max_stack : 1u) * 2u;
+    stack = (stack + 15u) & ~(size_t)15u;
+    Lds r;
+    size_t total = (size_t)(scene_f4 + light_f4) * 16u + stack;
+    if (use_lds && total <= kLdsBudget) {
+        r.plan = LdsPlan{scene_f4, light_f4};
+        r.bytes = total;
+        r.staged = true;
+    } else {
+        r.plan = LdsPlan{0u, 0u};
+        r.bytes = stack;
+        r.staged = false;
+    }
+    return r;
+}
+
 int Engine::check(hipError_t e, const char* what, std::string& err) {
     if (e == hipSuccess) return 0;
     err = std::string(what) + ": " + hipGetErrorString(e);
@@ -632,6 +808,11 @@ int Engine::init(int device, std::string& err) {
     device_ = device;
     CK(hipSetDevice(device));
     CK(hipStreamCreate(&stream));
+    {
+        int cus = 0;
+        CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        n_cu_ = cus > 0 ? cus : 1;
+    }
     // Closure directions and sRGB thresholds are scene independent.
     float4 dirs[16];
     for (uint32_t k = 0; k < 16; ++k) {
@@ -758,7 +939,11 @@ int Engine::load_scene(const std::string& src, std::string& err) {
     if ((rc = up(d_planes, s.plane_rec.data(), s.plane_rec.size() * 16))) return rc;
     if ((rc = up(d_subs, s.sub_rec.data(), s.sub_rec.size() * 16))) return rc;
     if ((rc = up(d_nodes, s.node_rec.data(), s.node_rec.size() * 16))) return rc;
-    if ((rc = up(d_lin, s.lin.data(), s.lin.size() * 4))) return rc;
+    {
+        std::vector<uint32_t> lin_pad((s.lin.size() + 3) & ~(size_t)3, 0u);
+        std::copy(s.lin.begin(), s.lin.end(), lin_pad.begin());
+        if ((rc = up(d_lin, lin_pad.data(), lin_pad.size() * 4))) return rc;
+    }
     if ((rc = up(d_quad, s.quad_rec.data(), s.quad_rec.size() * 16))) return rc;
     if ((rc = up(d_sph, s.sph_rec.data(), s.sph_rec.size() * 16))) return rc;
     if ((rc = up(d_pnt, s.pnt_rec.data(), s.pnt_rec.size() * 16))) return rc;
@@ -903,9 +1088,9 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     CK(d_ccount.ensure(4ull * (N + 1)));
     CK(d_cstart.ensure(4ull * (N + 1)));
     CK(hipMemsetAsync(d_ccount.as<uint32_t>() + N, 0, 4, stream));
-    const uint32_t ff_blocks = std::min<uint32_t>(blocks_for(N, 4), 8192);
+    const uint32_t ffb = ff_blocks ? ff_blocks : std::min<uint32_t>(blocks_for(N, 4), 8192);
     launch("ff_count", [&] {
-        k_ff<false><<<ff_blocks, kBlock, 0, stream>>>(spos, snrm, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
+        k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
                                                        d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr);
     });
     tmp = 0;
@@ -926,7 +1111,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     CK(d_cand_row.ensure(4ull * M));
     CK(d_cand_v.ensure(4ull * M));
     launch("ff_emit", [&] {
-        k_ff<true><<<ff_blocks, kBlock, 0, stream>>>(spos, snrm, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
+        k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
                                                       nullptr, d_cstart.as<uint32_t>(), d_cand_ci.as<uint32_t>(),
                                                       d_cand_row.as<uint32_t>());
     });
@@ -935,27 +1120,25 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     const float diag = scene.diag();
     const SceneView S = dev_view();
     if (M) {
-        const uint32_t vis_blocks = std::min<uint32_t>(blocks_for(M, kBlock), 65535u * 4u);
+        const uint32_t visb = vis_blocks ? vis_blocks : std::min<uint32_t>(blocks_for(M, kBlock), 65535u * 4u);
+        const Lds ld = plan_lds(false);
+        const LightsView Lg{};
         launch("visibility", [&] {
-            if (count_ops_)
-                k_visibility<true><<<vis_blocks, kBlock, 0, stream>>>(
-                    S, spos, snrm, d_cpos.as<float4>(), d_cnrm.as<float4>(), d_cand_ci.as<uint32_t>(),
-                    d_cand_row.as<uint32_t>(), M, diag * 5e-4f, diag * 1e-3f, d_cand_v.as<float>(), ctr, ctr + 1);
-            else
-                k_visibility<false><<<vis_blocks, kBlock, 0, stream>>>(
-                    S, spos, snrm, d_cpos.as<float4>(), d_cnrm.as<float4>(), d_cand_ci.as<uint32_t>(),
-                    d_cand_row.as<uint32_t>(), M, diag * 5e-4f, diag * 1e-3f, d_cand_v.as<float>(), ctr, nullptr);
+            LAUNCH_CL(k_visibility, count_ops_, ld.staged, visb, ld.bytes, S, Lg, ld.plan, spos, snrm,
+                      d_cpos.as<float4>(), d_cnrm.as<float4>(), d_cand_ci.as<uint32_t>(), d_cand_row.as<uint32_t>(),
+                      M, diag * 5e-4f, diag * 1e-3f, d_cand_v.as<float>(), ctr, count_ops_ ? ctr + 1 : nullptr);
         });
     }
     CK(d_sky.ensure(4ull * N));
-    launch("closure", [&] {
-        if (count_ops_)
-            k_closure<true><<<blocks_for(16ull * N, kBlock), kBlock, 0, stream>>>(
-                S, spos, snrm, N, d_dirs.as<float4>(), diag * 2.0f, d_sky.as<float>(), ctr, ctr + 2);
-        else
-            k_closure<false><<<blocks_for(16ull * N, kBlock), kBlock, 0, stream>>>(
-                S, spos, snrm, N, d_dirs.as<float4>(), diag * 2.0f, d_sky.as<float>(), ctr, nullptr);
-    });
+    {
+        const Lds ld = plan_lds(false);
+        const LightsView Lg{};
+        launch("closure", [&] {
+            LAUNCH_CL(k_closure, count_ops_, ld.staged, blocks_for(16ull * N, kBlock), ld.bytes, S, Lg, ld.plan, spos,
+                      snrm, N, d_dirs.as<float4>(), diag * 2.0f, d_sky.as<float>(), ctr,
+                      count_ops_ ? ctr + 2 : nullptr);
+        });
+    }
 
     // ---- calibration, link scan, compaction
     CK(d_lcount.ensure(4ull * (N + 1)));
@@ -1013,53 +1196,80 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
     CK(hipSetDevice(device_));
     out = SolveOut();
     const uint32_t N = (uint32_t)surfels.size();
+    const LightsView Lv = dev_lights();
+    const uint32_t nl = n_lights(Lv);
     CK(d_direct.ensure(16ull * N));
     CK(d_irr.ensure(16ull * N));
     CK(d_rad.ensure(16ull * N));
     CK(d_bc.ensure(16ull * (n_clusters + 1)));
     CK(d_resid.ensure(4ull * (bounces + 1)));
+    CK(d_terms.ensure(16ull * N * (nl ? nl : 1)));
+    CK(d_sync.ensure(16));
     unsigned long long* ctr = d_counter.as<unsigned long long>();
     CK(hipMemsetAsync(ctr, 0, 64, stream));
     CK(hipMemsetAsync(d_resid.p, 0, 4ull * (bounces + 1), stream));
+    CK(hipMemsetAsync(d_sync.p, 0, 16, stream));
     const SceneView S = dev_view();
-    const LightsView Lv = dev_lights();
     if (N) {
-        launch("direct_e", [&] {
-            if (count_ops_)
-                k_direct_e<true><<<blocks_for(N, kBlock), kBlock, 0, stream>>>(
-                    S, Lv, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, d_direct.as<float4>(),
-                    d_rad.as<float4>(), d_irr.as<float4>(), ctr, ctr + 3);
-            else
-                k_direct_e<false><<<blocks_for(N, kBlock), kBlock, 0, stream>>>(
-                    S, Lv, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, d_direct.as<float4>(),
-                    d_rad.as<float4>(), d_irr.as<float4>(), ctr, nullptr);
+        const Lds ld = plan_lds(true);
+        if (nl) {
+            launch("direct_e", [&] {
+                LAUNCH_CL(k_direct_terms, count_ops_, ld.staged, blocks_for((uint64_t)N * nl, kBlock), ld.bytes, S,
+                          Lv, ld.plan, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, nl,
+                          d_terms.as<float4>(), ctr, count_ops_ ? ctr + 3 : nullptr);
+            });
+        }
+        launch("direct_sum", [&] {
+            k_direct_sum<<<blocks_for(N, 256), 256, 0, stream>>>(N, nl, d_terms.as<float4>(), d_direct.as<float4>(),
+                                                                 d_rad.as<float4>(), d_irr.as<float4>());
         });
     }
     const uint32_t C = n_clusters;
     if (C > 0 && N > 0) {
-        uint32_t* resid = d_resid.as<uint32_t>();
-        for (uint32_t it = 0; it < bounces; ++it) {
-            launch("aggregate", [&] {
-                k_aggregate<<<blocks_for(C, 64), 64, 0, stream>>>(it, resid, C, d_crange.as<uint2>(),
-                                                                  d_cval2.as<uint32_t>(), d_spos.as<float4>(),
-                                                                  d_cpos.as<float4>(), d_rad.as<float4>(),
-                                                                  d_bc.as<float4>());
-            });
-            launch("spmv", [&] {
-                k_spmv<<<blocks_for(N, kBlock), kBlock, 0, stream>>>(
-                    it, resid, N, d_lstart.as<uint32_t>(), d_idx.as<uint32_t>(), d_w.as<float>(), d_bc.as<float4>(),
-                    d_direct.as<float4>(), d_salb.as<float4>(), d_rad.as<float4>(), d_irr.as<float4>());
-            });
-        }
+        JacobiArgs A;
+        A.N = N;
+        A.C = C;
+        A.bounces = bounces;
+        A.range = d_crange.as<uint2>();
+        A.members = d_cval2.as<uint32_t>();
+        A.spos = d_spos.as<float4>();
+        A.cpos = d_cpos.as<float4>();
+        A.lstart = d_lstart.as<uint32_t>();
+        A.idx = d_idx.as<uint32_t>();
+        A.w = d_w.as<float>();
+        A.direct = d_direct.as<float4>();
+        A.salb = d_salb.as<float4>();
+        A.rad = d_rad.as<float4>();
+        A.irr = d_irr.as<float4>();
+        A.bc = d_bc.as<float4>();
+        A.resid = d_resid.as<uint32_t>();
+        A.sync = d_sync.as<uint32_t>();
+        int per_cu = 0;
+        CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jacobi, kBlock, 0));
+        uint32_t want = std::max<uint32_t>(blocks_for(N, kBlock), blocks_for(64ull * C, kBlock));
+        uint32_t cap = (uint32_t)std::max(1, per_cu) * (uint32_t)n_cu_;
+        uint32_t grid = std::max<uint32_t>(1u, std::min(jacobi_blocks ? jacobi_blocks : want, cap));
+        void* args[] = {&A};
+        hipError_t le = hipSuccess;
+        launch("jacobi", [&] {
+            le = hipLaunchCooperativeKernel((const void*)k_jacobi, dim3(grid), dim3(kBlock), args, 0, stream);
+        });
+        CK(le);
     }
     std::vector<uint32_t> res(bounces + 1, 0);
     unsigned long long cv[8] = {0};
+    uint32_t sync[4] = {0};
     CK(hipMemcpyAsync(res.data(), d_resid.p, 4ull * bounces, hipMemcpyDeviceToHost, stream));
     CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(sync, d_sync.p, 16, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
+    drain_events();
+    if (sync[1]) {
+        err = "jacobi grid barrier timed out";
+        return -4;
+    }
     unsigned long long ev = cv[0];
     if (count_ops_) kops_["direct_e"] += cv[3];
-    drain_events();
     out.evals = ev;
     if (C > 0 && N > 0) {
         out.iterations = bounces;
@@ -1174,11 +1384,10 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
     const SceneView S = dev_view();
     const LightsView Lv = dev_lights();
     dim3 grid((W + 15) / 16, (H + 15) / 16);
+    const Lds ld = plan_lds(true);
     launch("shade", [&] {
-        if (count_ops_)
-            k_shade<true><<<grid, kBlock, 0, stream>>>(S, Lv, A, d_film.as<float4>(), ctr, ctr + 4);
-        else
-            k_shade<false><<<grid, kBlock, 0, stream>>>(S, Lv, A, d_film.as<float4>(), ctr, nullptr);
+        LAUNCH_CL(k_shade, count_ops_, ld.staged, grid, ld.bytes, S, Lv, ld.plan, A, d_film.as<float4>(), ctr,
+                  count_ops_ ? ctr + 4 : nullptr);
     });
     const bool tonemapped = mode <= M_INDIRECT;
     if (rgb8) {
@@ -1213,8 +1422,11 @@ int Engine::probe_dist(const float* pts, uint64_t n, float* d, uint32_t* mat, fl
     uint32_t* om = (uint32_t*)(od + n);
     float* on = nrm ? od + 2 * n : nullptr;
     const SceneView S = dev_view();
+    const Lds ld = plan_lds(false);
+    const LightsView Lg{};
     launch("probe_dist", [&] {
-        k_probe_dist<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(S, d_probe_in.as<float>(), (uint32_t)n, od, om, on);
+        LAUNCH_L(k_probe_dist, ld.staged, blocks_for(n, kBlock), ld.bytes, S, Lg, ld.plan, d_probe_in.as<float>(),
+                 (uint32_t)n, od, om, on);
     });
     CK(hipMemcpyAsync(d, od, 4ull * n, hipMemcpyDeviceToHost, stream));
     CK(hipMemcpyAsync(mat, om, 4ull * n, hipMemcpyDeviceToHost, stream));
@@ -1241,9 +1453,11 @@ int Engine::probe_trace(const float* ro, const float* rd, uint64_t n, float tmin
     int32_t* oh = (int32_t*)(o + 5 * n);
     uint32_t* os = (uint32_t*)(o + 6 * n);
     const SceneView S = dev_view();
+    const Lds ld = plan_lds(false);
+    const LightsView Lg{};
     launch("probe_trace", [&] {
-        k_probe_trace<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(S, in, in + 3 * n, (uint32_t)n, tmin, tmax, ot, on,
-                                                                    om, oh, os);
+        LAUNCH_L(k_probe_trace, ld.staged, blocks_for(n, kBlock), ld.bytes, S, Lg, ld.plan, in, in + 3 * n, (uint32_t)n,
+                 tmin, tmax, ot, on, om, oh, os);
     });
     CK(hipMemcpyAsync(t, ot, 4ull * n, hipMemcpyDeviceToHost, stream));
     CK(hipMemcpyAsync(nrm, on, 12ull * n, hipMemcpyDeviceToHost, stream));
@@ -1270,9 +1484,11 @@ int Engine::probe_shadow(const float* ro, const float* rd, uint64_t n, float tmi
     float* ov = d_probe_out.as<float>();
     uint32_t* os = (uint32_t*)(ov + n);
     const SceneView S = dev_view();
+    const Lds ld = plan_lds(false);
+    const LightsView Lg{};
     launch("probe_shadow", [&] {
-        k_probe_shadow<<<blocks_for(n, kBlock), kBlock, 0, stream>>>(S, in, in + 3 * n, (uint32_t)n, tmin, in + 6 * n,
-                                                                     in + 7 * n, ov, os);
+        LAUNCH_L(k_probe_shadow, ld.staged, blocks_for(n, kBlock), ld.bytes, S, Lg, ld.plan, in, in + 3 * n,
+                 (uint32_t)n, tmin, in + 6 * n, in + 7 * n, ov, os);
     });
     CK(hipMemcpyAsync(vis, ov, 4ull * n, hipMemcpyDeviceToHost, stream));
     CK(hipMemcpyAsync(steps, os, 4ull * n, hipMemcpyDeviceToHost, stream));

@@ -479,6 +479,7 @@ SceneView HostScene::host_view() const {
     v.n_planes = (uint32_t)planes.size();
     v.n_nodes = (uint32_t)(node_rec.size() / 2);
     v.use_bvh = 1;
+    v.n_subs = (uint32_t)(sub_rec.size() / 4);
     v.blend_k = blend_k;
     return v;
 }

@@ -55,6 +55,7 @@ struct SceneView {
     const float4* nodes;
     const uint32_t* lin;   // linear-scan order: lin[i] = slot of object i
     uint32_t n_obj, n_planes, n_nodes, use_bvh;
+    uint32_t n_subs;
     float blend_k;         // EXT blend-group radius, 0 = reference behaviour
 };
 

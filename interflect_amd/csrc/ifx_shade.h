@@ -133,42 +133,64 @@ IFX_HD void frame(f3 n, f3& t, f3& b) {
     b = mk(bb, s + n.y * n.y * a, -n.y);
 }
 
+// One edge term of the Lambert polygon sum (src/ltc.rs:72-80).
+IFX_HD float edge_term(f3 a, f3 b) {
+    float cos_t = clmp(dot3(a, b), -1.0f, 1.0f);
+    float theta = acosf(cos_t);
+    f3 cr = cross3(a, b);
+    float len = len3(cr);
+    return len > 1e-9f ? theta * (cr.z / len) : 0.0f;
+}
+
+IFX_HD f3 unit_or_keep(f3 v) {
+    float len = len3(v);
+    return len > 1e-9f ? v / len : v;
+}
+
 // src/ltc.rs:25-83 on the quad given in world space (to_local, :108-115,
-// reverses the winding on the way in).
+// reverses the winding on the way in).  The horizon-clipped polygon is
+// streamed vertex by vertex instead of stored: each emitted vertex is
+// normalised and closes the edge from its predecessor, and the closing edge
+// back to the first vertex is added last, which is the reference's edge
+// order (edges 0..n-1, wrapping).  No per-lane array, so nothing spills.
 IFX_HD float quad_ff(const float4* L, f3 p, f3 t, f3 b, f3 n) {
-    f3 l[4];
-    for (int i = 0; i < 4; ++i) {
-        f3 d = ld3(L[3 - i]) - p;
-        l[i] = mk(dot3(d, t), dot3(d, b), dot3(d, n));
+    f3 l0, l1, l2, l3;
+    {
+        f3 d = ld3(L[3]) - p;
+        l0 = mk(dot3(d, t), dot3(d, b), dot3(d, n));
+        d = ld3(L[2]) - p;
+        l1 = mk(dot3(d, t), dot3(d, b), dot3(d, n));
+        d = ld3(L[1]) - p;
+        l2 = mk(dot3(d, t), dot3(d, b), dot3(d, n));
+        d = ld3(L[0]) - p;
+        l3 = mk(dot3(d, t), dot3(d, b), dot3(d, n));
     }
-    f3 c[5];
+    f3 first = spl(0.0f), prev = spl(0.0f);
     int m = 0;
-    for (int i = 0; i < 4; ++i) {
-        f3 a = l[i];
-        f3 bb = l[(i + 1) & 3];
-        float da = a.z, db = bb.z;
-        if (da >= 0.0f) c[m++] = a;
-        if ((da >= 0.0f) != (db >= 0.0f)) {
-            float tt = da / (da - db);
-            c[m++] = a + (bb - a) * tt;
-        }
-        if (m >= 5) break;
-    }
-    if (m < 3) return 0.0f;
-    for (int i = 0; i < m; ++i) {
-        float len = len3(c[i]);
-        if (len > 1e-9f) c[i] = c[i] / len;
-    }
     float sum = 0.0f;
-    for (int i = 0; i < m; ++i) {
-        f3 a = c[i];
-        f3 bb = c[i + 1 == m ? 0 : i + 1];
-        float cos_t = clmp(dot3(a, bb), -1.0f, 1.0f);
-        float theta = acosf(cos_t);
-        f3 cr = cross3(a, bb);
-        float len = len3(cr);
-        if (len > 1e-9f) sum += theta * (cr.z / len);
-    }
+    auto emit = [&](f3 v) {
+        v = unit_or_keep(v);
+        if (m == 0)
+            first = v;
+        else
+            sum += edge_term(prev, v);
+        prev = v;
+        ++m;
+    };
+    auto clip_edge = [&](f3 a, f3 c) {
+        float da = a.z, dc = c.z;
+        if (da >= 0.0f) emit(a);
+        if ((da >= 0.0f) != (dc >= 0.0f)) {
+            float tt = da / (da - dc);
+            emit(a + (c - a) * tt);
+        }
+    };
+    clip_edge(l0, l1);
+    clip_edge(l1, l2);
+    clip_edge(l2, l3);
+    clip_edge(l3, l0);
+    if (m < 3) return 0.0f;
+    sum += edge_term(prev, first);
     return fmx(sum * 0.15915494f, 0.0f);
 }
 
@@ -263,6 +285,48 @@ IFX_HD float ext_shadow(const SceneView& S, f3 p, f3 n, const ExtGeo& g, uint32_
     return vis;
 }
 
+// Light `li` (quad lights first, then EXT sphere and point lights) of the
+// diffuse direct term (src/shade.rs:137-162).  Returns false when the
+// reference's loop would `continue` past this light without adding.
+template <class Stk>
+IFX_HD bool diffuse_term(const SceneView& S, const LightsView& Lv, uint32_t li, f3 p, f3 n, f3 tan, f3 bit, f3 albedo,
+                         f3& term, uint32_t& steps, Stk& stk) {
+    if (li < Lv.n_quad) {
+        const float4* L = Lv.quad + 7 * li;
+        f3 lc = ld3(L[5]);
+        f3 to_light = lc - p;
+        if (dot3(ld3(L[6]), -nrm3(to_light)) <= 0.0f) return false;
+        float ff = quad_ff(L, p, tan, bit, n);
+        if (ff <= 1e-6f) return false;
+        float dist = len3(to_light);
+        float radius = L[5].w;
+        float k = clmp(dist / radius, 1.5f, 96.0f);
+        float end = fmx(dist - radius * 1.25f, 0.05f);
+        uint32_t s = 0;
+        float vis = soft_shadow(S, p + n * 3e-3f, to_light / dist, 0.015f, end, k, s, stk);
+        steps += s;
+        if (vis <= 1e-4f) return false;
+        term = (albedo * ld3(L[4])) * (ff * vis);
+        return true;
+    }
+    li -= Lv.n_quad;
+    ExtGeo g;
+    const float4* L;
+    if (li < Lv.n_sph) {  // EXT
+        L = Lv.sph + 2 * li;
+        if (!sphere_geo(L, p, n, g)) return false;
+    } else {  // EXT
+        L = Lv.pnt + 2 * (li - Lv.n_sph);
+        if (!point_geo(L, p, n, g)) return false;
+    }
+    float vis = ext_shadow(S, p, n, g, steps, stk);
+    if (vis <= 1e-4f) return false;
+    term = (albedo * ld3(L[1])) * (g.ff * vis);
+    return true;
+}
+
+IFX_HD uint32_t n_lights(const LightsView& Lv) { return Lv.n_quad + Lv.n_sph + Lv.n_pnt; }
+
 // src/shade.rs:133-164: diffuse direct light, the radiosity E term.
 template <class Stk>
 IFX_HD f3 direct_diffuse(const SceneView& S, const LightsView& Lv, f3 p, f3 n, f3 albedo, uint32_t& steps_out,
@@ -271,38 +335,10 @@ IFX_HD f3 direct_diffuse(const SceneView& S, const LightsView& Lv, f3 p, f3 n, f
     uint32_t steps = 0;
     f3 tan, bit;
     frame(n, tan, bit);
-    for (uint32_t li = 0; li < Lv.n_quad; ++li) {
-        const float4* L = Lv.quad + 7 * li;
-        f3 lc = ld3(L[5]);
-        f3 to_light = lc - p;
-        if (dot3(ld3(L[6]), -nrm3(to_light)) <= 0.0f) continue;
-        float ff = quad_ff(L, p, tan, bit, n);
-        if (ff <= 1e-6f) continue;
-        float dist = len3(to_light);
-        float radius = L[5].w;
-        float k = clmp(dist / radius, 1.5f, 96.0f);
-        float end = fmx(dist - radius * 1.25f, 0.05f);
-        uint32_t s = 0;
-        float vis = soft_shadow(S, p + n * 3e-3f, to_light / dist, 0.015f, end, k, s, stk);
-        steps += s;
-        if (vis <= 1e-4f) continue;
-        out = out + (albedo * ld3(L[4])) * (ff * vis);
-    }
-    for (uint32_t li = 0; li < Lv.n_sph; ++li) {  // EXT
-        const float4* L = Lv.sph + 2 * li;
-        ExtGeo g;
-        if (!sphere_geo(L, p, n, g)) continue;
-        float vis = ext_shadow(S, p, n, g, steps, stk);
-        if (vis <= 1e-4f) continue;
-        out = out + (albedo * ld3(L[1])) * (g.ff * vis);
-    }
-    for (uint32_t li = 0; li < Lv.n_pnt; ++li) {  // EXT
-        const float4* L = Lv.pnt + 2 * li;
-        ExtGeo g;
-        if (!point_geo(L, p, n, g)) continue;
-        float vis = ext_shadow(S, p, n, g, steps, stk);
-        if (vis <= 1e-4f) continue;
-        out = out + (albedo * ld3(L[1])) * (g.ff * vis);
+    const uint32_t nl = n_lights(Lv);
+    for (uint32_t li = 0; li < nl; ++li) {
+        f3 t;
+        if (diffuse_term(S, Lv, li, p, n, tan, bit, albedo, t, steps, stk)) out = out + t;
     }
     steps_out = steps;
     return out;

@@ -486,3 +486,85 @@ def test_every_scene_fixture_renders_small():
         sc.solve(4)
         film, ev = sc.render("beauty", 40, 30)
         assert np.all(np.isfinite(film)) and ev > 0, name
+
+
+# --------------------------------------------------------------------------- EXT closed forms
+FLOOR = ('material "w" { albedo: [0.5, 0.5, 0.5], roughness: 1.0 }\n'
+         'box { center: [0, -0.1, 0], size: [40, 0.2, 40], mat: "w" }\n')
+
+
+def test_point_light_irradiance_closed_form():
+    """EXT point light over an open floor: E = albedo * I * cos / (pi d^2), unshadowed."""
+    sc = po.Scene(FLOOR + "point_light { pos: [0.3, 2.0, -0.4], emit: [5, 6, 7] }\n")
+    p = np.array([[0, 0, 0], [1.0, 0, 0.5], [-2, 0, 1]], np.float32)
+    n = np.tile([0, 1, 0], (3, 1)).astype(np.float32)
+    e, steps = sc.direct_diffuse(p, n, np.full((3, 3), 0.5, np.float32))
+    to = np.array([0.3, 2.0, -0.4]) - p
+    d2 = (to ** 2).sum(1)
+    cos = to[:, 1] / np.sqrt(d2)
+    want = 0.5 * np.array([5, 6, 7])[None, :] * (cos / (np.pi * d2))[:, None]
+    assert np.allclose(e, want, rtol=1e-5) and np.all(steps > 0)
+
+
+def test_sphere_light_irradiance_closed_form():
+    """EXT sphere light wholly above the horizon: E = albedo * L * cos * (R/d)^2."""
+    sc = po.Scene(FLOOR + "sphere_light { center: [0.0, 3.0, 0.0], radius: 0.4, emit: [2, 3, 4] }\n")
+    p = np.array([[0, 0, 0], [1.5, 0, 0]], np.float32)
+    n = np.tile([0, 1, 0], (2, 1)).astype(np.float32)
+    e, _ = sc.direct_diffuse(p, n, np.full((2, 3), 0.5, np.float32))
+    to = np.array([0, 3.0, 0]) - p
+    d2 = (to ** 2).sum(1)
+    cos = to[:, 1] / np.sqrt(d2)
+    want = 0.5 * np.array([2, 3, 4])[None, :] * (cos * 0.16 / d2)[:, None]
+    assert np.allclose(e, want, rtol=1e-5)
+
+
+BLEND2 = ('material "w" { albedo: [0.5, 0.5, 0.5] }\n'
+          'sphere { center: [0, 0, 0], radius: 0.5, mat: "w", blend: 0.2 }\n'
+          'sphere { center: [1.1, 0, 0], radius: 0.5, mat: "w", blend: 0.2 }\n'
+          'light { verts: [[-1,4,-1],[1,4,-1],[1,4,1],[-1,4,1]], emit: [1,1,1] }\n')
+
+
+def test_blend_group_smooth_union():
+    """EXT blend: d = smin(d1, d2, k) = min - h^2 k / 4 with h = max(k - |d1-d2|, 0) / k;
+    equal to the hard minimum once |d1 - d2| >= k, never above it, and the
+    BVH traversal with its k-widened prune agrees with the linear scan."""
+    sc = po.Scene(BLEND2)
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(-2, 3, (4000, 3)).astype(np.float32)
+    d, _ = sc.dist_mat(pts)
+    d1 = np.linalg.norm(pts - [0, 0, 0], axis=1) - 0.5
+    d2 = np.linalg.norm(pts - [1.1, 0, 0], axis=1) - 0.5
+    h = np.maximum(0.2 - np.abs(d1 - d2), 0) / 0.2
+    want = np.minimum(d1, d2) - h * h * 0.2 * 0.25
+    assert np.allclose(d, want, atol=2e-6)
+    assert np.all(d <= np.minimum(d1, d2) + 1e-6)
+    sc.set_use_bvh(False)
+    dl, _ = sc.dist_mat(pts)
+    assert np.array_equal(d.view(np.uint32), dl.view(np.uint32))
+
+
+def test_blend_field_is_lipschitz():
+    """The smooth union never overstates distance: stepping d along any
+    direction lands no deeper than -1e-3 (cf. src/sdf.rs:254-296)."""
+    sc = po.Scene(BLEND2)
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(-2, 3, (3000, 3)).astype(np.float32)
+    d, _ = sc.dist_mat(pts)
+    dirs = rng.normal(size=(3000, 3)).astype(np.float32)
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    ok = d > 0
+    d2, _ = sc.dist_mat(pts[ok] + dirs[ok] * d[ok, None])
+    assert np.all(d2 > -1e-3)
+
+
+def test_cylinder_subtraction():
+    """EXT subtract_cylinder: max(d, -cyl) carves a hole through a sphere."""
+    src = ('material "w" { albedo: [0.5, 0.5, 0.5] }\n'
+           'sphere { center: [0, 0, 0], radius: 1.0, mat: "w", subtract_cylinder: [[0, -2, 0], [0, 2, 0], 0.25] }\n'
+           'light { verts: [[-1,4,-1],[1,4,-1],[1,4,1],[-1,4,1]], emit: [1,1,1] }\n')
+    sc = po.Scene(src)
+    d, _ = sc.dist_mat(np.array([[0, 0, 0], [0.6, 0, 0], [0, 0.9, 0]], np.float32))
+    assert d[0] == pytest.approx(0.25, abs=1e-6)   # axis: inside the drilled hole
+    assert d[1] == pytest.approx(-0.35, abs=1e-6)  # solid shell between hole and surface
+    assert d[2] == pytest.approx(0.25, abs=1e-6)
