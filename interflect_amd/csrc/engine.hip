@@ -153,7 +153,9 @@ __device__ __forceinline__ void stk_flush(CountStack& s, unsigned long long* ops
     LightsView L = Lg;                                                          \
     stage_in<LDS, LIGHTS>(ifx_lds, S, L);                                       \
     typename StackOf<COUNT>::T stk;                                             \
-    stk.base = (uint16_t*)(ifx_lds + P.scene_f4 + P.light_f4) + threadIdx.x;    \
+    stk.dbase = (float*)(ifx_lds + P.scene_f4 + P.light_f4) + threadIdx.x;      \
+    stk.base = (uint16_t*)(stk.dbase - threadIdx.x + P.stack_depth * kBlock) +  \
+               threadIdx.x;                                                     \
     stk.stride = (uint32_t)kBlock;                                              \
     stk_init(stk);                                                              \
     (void)L
@@ -298,13 +300,15 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
                                                uint32_t N, const float4* __restrict__ cpos,
                                                const float4* __restrict__ cnrm, uint32_t C, uint32_t* count,
                                                const uint32_t* __restrict__ start, uint32_t* cand_ci,
-                                               uint32_t* cand_row) {
+                                               uint32_t* cand_row, const uint32_t* __restrict__ morton,
+                                               uint32_t* key, uint32_t* val) {
     uint32_t lane = lane_id();
     uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     uint32_t nwaves = (gridDim.x * kBlock) >> 6;
     for (uint32_t i = wave; i < N; i += nwaves) {
         f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
         uint32_t base = EMIT ? start[i] : 0u;
+        uint32_t mc = (EMIT && key) ? morton[i] : 0u;
         uint32_t run = 0;
         for (uint32_t c0 = 0; c0 < C; c0 += 64) {
             uint32_t c = c0 + lane;
@@ -319,6 +323,10 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
                 uint32_t o = base + run + lanes_below(m);
                 cand_ci[o] = c;
                 cand_row[o] = i;
+                if (key) {
+                    key[o] = (c << 15) | mc;
+                    val[o] = o;
+                }
             }
             run += (uint32_t)__popcll(m);
         }
@@ -326,70 +334,166 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
     }
 }
 
-// Cone-traced link visibility, one candidate per lane (src/formfactor.rs:194-214).
-// cand_v = ff * vis for a kept link, 0 for a culled one.
+// 15-bit Morton code (5 bits per axis) of each surfel's cell in the scene
+// box; it only orders work (coherent wavefronts), never enters a result.
+__device__ __forceinline__ uint32_t spread5(uint32_t v) {
+    uint32_t r = 0;
+    for (int b = 0; b < 5; ++b) r |= ((v >> b) & 1u) << (3 * b);
+    return r;
+}
+
+__global__ void k_morton(const float4* __restrict__ spos, uint32_t N, f3 lo, f3 scale, uint32_t* code,
+                         uint32_t* iota) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    f3 q = (ld3(spos[i]) - lo) * scale;
+    uint32_t x = (uint32_t)clampi(sat_i32(q.x), 0, 31), y = (uint32_t)clampi(sat_i32(q.y), 0, 31),
+             z = (uint32_t)clampi(sat_i32(q.z), 0, 31);
+    code[i] = spread5(x) | (spread5(y) << 1) | (spread5(z) << 2);
+    iota[i] = i;
+}
+
+// Wavefront-private slice of a global work queue.  A wave takes kChunk
+// consecutive items per atomic and hands them to its idle lanes in order;
+// all fields are wave-uniform.
+struct WaveQueue {
+    uint32_t next, end, chunk;
+    bool drained;
+};
+
+__device__ __forceinline__ bool refill(bool live, WaveQueue& wq, uint32_t* queue, uint32_t total, uint32_t& item) {
+    uint64_t idle = __ballot(!live);
+    if (!idle) return false;
+    if (wq.next >= wq.end) {
+        if (wq.drained) return false;
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(queue, wq.chunk);
+        base = __shfl(base, 0, 64);
+        if (base >= total) {
+            wq.drained = true;
+            return false;
+        }
+        wq.next = base;
+        wq.end = total - base < wq.chunk ? total : base + wq.chunk;
+    }
+    uint32_t nidle = (uint32_t)__popcll(idle);
+    uint32_t avail = wq.end - wq.next;
+    uint32_t take = nidle < avail ? nidle : avail;
+    uint32_t rank = lanes_below(idle);
+    item = wq.next + rank;
+    wq.next += take;
+    return !live && rank < take;
+}
+
+// Cone-traced link visibility (src/formfactor.rs:194-214) as a persistent
+// kernel: every lane marches one candidate's shadow ray a sample at a time
+// and takes the next candidate from the queue as soon as its ray ends, so
+// short and long marches no longer idle each other.  Candidates are taken
+// in `order` (cluster-major, surfels in Morton order: near-parallel rays
+// from nearby origins share a wavefront).  cand_v[slot] = ff * vis for a
+// kept link, 0 for a culled one; slots are fixed, so the work order never
+// changes a result.
 template <bool COUNT, bool LDS>
-__global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView Lg, LdsPlan P, const float4* __restrict__ spos,
+__global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView Lg, LdsPlan P,
+                                                       const float4* __restrict__ spos,
                                                        const float4* __restrict__ snrm,
                                                        const float4* __restrict__ cpos,
                                                        const float4* __restrict__ cnrm,
                                                        const uint32_t* __restrict__ cand_ci,
-                                                       const uint32_t* __restrict__ cand_row, uint32_t M,
-                                                       float off, float tmin, float* cand_v,
+                                                       const uint32_t* __restrict__ cand_row,
+                                                       const uint32_t* __restrict__ order, uint32_t M, float off,
+                                                       float tmin, float* cand_v, uint32_t* queue, uint32_t chunk,
                                                        unsigned long long* evals, unsigned long long* ops) {
     IFX_SETUP(LDS, false, COUNT);
     uint32_t acc = 0;
-    uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < M; k += stride) {
-        uint32_t i = cand_row[k], c = cand_ci[k];
-        f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
-        float4 CN = cnrm[c];
-        float r = 0.0f, ff = 0.0f;
-        f3 dir = spl(0.0f);
-        ff_geom(sp, sn, cpos[c], CN, r, dir, ff);
-        float kk = clmp(r / fmx(CN.w, 1e-4f), 1.5f, 32.0f);
-        uint32_t s = 0;
-        float vis = soft_shadow(S, sp + sn * off, dir, tmin, r - CN.w * 0.75f, kk, s, stk);
-        acc += s;
-        cand_v[k] = vis <= 1e-3f ? 0.0f : ff * vis;
+    bool live = false;
+    WaveQueue wq{0u, 0u, chunk, false};
+    uint32_t slot = 0;
+    float ff = 0.0f;
+    ShadowMarch m;
+    m.begin(spl(0.0f), spl(0.0f), 0.0f, 0.0f, 0.0f);
+    for (;;) {
+        uint32_t q;
+        if (refill(live, wq, queue, M, q)) {
+            slot = order ? order[q] : q;
+            uint32_t i = cand_row[slot], c = cand_ci[slot];
+            f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
+            float4 CN = cnrm[c];
+            float r = 0.0f;
+            f3 dir = spl(0.0f);
+            ff_geom(sp, sn, cpos[c], CN, r, dir, ff);
+            float kk = clmp(r / fmx(CN.w, 1e-4f), 1.5f, 32.0f);
+            m.begin(sp + sn * off, dir, tmin, r - CN.w * 0.75f, kk);
+            live = true;
+        }
+        if (__ballot(live) == 0) break;
+        if (live) {
+            float vis;
+            if (m.step(S, vis, stk)) {
+                cand_v[slot] = vis <= 1e-3f ? 0.0f : ff * vis;
+                acc += m.steps;
+                live = false;
+            }
+        }
     }
     count_steps(evals, acc);
     stk_flush(stk, ops);
 }
 
-// Hemisphere closure: 16 lanes per surfel, one Halton direction each
-// (src/trace.rs:165-186).  dirs[k] = (r cos phi, r sin phi, sqrt(1 - u1)).
+// Hemisphere closure (src/trace.rs:165-186), persistent like k_visibility.
+// Item q = direction-major over surfels in Morton order, so a wavefront
+// traces one Halton direction from 64 neighbouring surfels; a missed ray
+// adds one to its surfel's integer open count.  dirs[k] = (r cos phi,
+// r sin phi, sqrt(1 - u1)).
 template <bool COUNT, bool LDS>
-__global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg, LdsPlan P, const float4* __restrict__ spos,
+__global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg, LdsPlan P,
+                                                    const float4* __restrict__ spos,
                                                     const float4* __restrict__ snrm, uint32_t N,
-                                                    const float4* __restrict__ dirs, float tmax, float* sky,
-                                                    unsigned long long* evals, unsigned long long* ops) {
+                                                    const uint32_t* __restrict__ row_order,
+                                                    const float4* __restrict__ dirs, float tmax, uint32_t* open,
+                                                    uint32_t* queue, uint32_t chunk, unsigned long long* evals,
+                                                    unsigned long long* ops) {
     IFX_SETUP(LDS, false, COUNT);
-    uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t row = tid >> 4, k = tid & 15u;
-    bool active = row < N;
-    uint32_t steps = 0;
-    bool open = false;
-    if (active) {
-        f3 p = ld3(spos[row]), n = ld3(snrm[row]);
-        f3 t, b;
-        frame(n, t, b);
-        float4 D = dirs[k];
-        f3 d = (t * D.x + b * D.y) + n * D.z;
-        Hit h = trace<false>(S, p + n * 1e-3f, nrm3(d), 1e-3f, tmax, steps, stk);
-        open = !h.hit;
+    uint32_t acc = 0;
+    bool live = false;
+    WaveQueue wq{0u, 0u, chunk, false};
+    uint32_t row = 0;
+    TraceMarch m;
+    m.begin(spl(0.0f), spl(0.0f), 0.0f, 0.0f);
+    const uint32_t total = 16u * N;
+    for (;;) {
+        uint32_t q;
+        if (refill(live, wq, queue, total, q)) {
+            uint32_t k = q / N;
+            row = row_order[q - k * N];
+            f3 p = ld3(spos[row]), n = ld3(snrm[row]);
+            f3 t, b;
+            frame(n, t, b);
+            float4 D = dirs[k];
+            f3 d = (t * D.x + b * D.y) + n * D.z;
+            m.begin(p + n * 1e-3f, nrm3(d), 1e-3f, tmax);
+            live = true;
+        }
+        if (__ballot(live) == 0) break;
+        if (live) {
+            DistHit h;
+            int st = m.step(S, h, stk);
+            if (st) {
+                if (st == 2) atomicAdd(&open[row], 1u);
+                acc += m.steps;
+                live = false;
+            }
+        }
     }
-    uint64_t m = __ballot(open);
-    uint32_t cnt = (uint32_t)__popcll((m >> (lane_id() & 48u)) & 0xffffull);
-    if (active && k == 0) sky[row] = (float)cnt / 16.0f;
-    count_steps(evals, steps);
+    count_steps(evals, acc);
     stk_flush(stk, ops);
 }
 
 // Serial per-row link sum in cluster order plus the closure calibration
 // (src/formfactor.rs:207-240).  row_sum = -1 marks a row without links.
 __global__ void k_row_sum(uint32_t N, const uint32_t* __restrict__ cstart, const float* __restrict__ cand_v,
-                          const float* __restrict__ sky, uint32_t* lcount, float* scale, float* row_sum) {
+                          const uint32_t* __restrict__ open, float* sky_out, uint32_t* lcount, float* scale,
+                          float* row_sum) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     float sum = 0.0f;
@@ -401,7 +505,9 @@ __global__ void k_row_sum(uint32_t N, const uint32_t* __restrict__ cstart, const
             ++cnt;
         }
     }
-    float target = clmp(1.0f - sky[i], 0.0f, 1.0f);
+    float sky = (float)open[i] / 16.0f;
+    sky_out[i] = sky;
+    float target = clmp(1.0f - sky, 0.0f, 1.0f);
     float sc = 1.0f;
     if (sum > 1e-5f) {
         sc = fmn(target / sum, 8.0f);
@@ -711,6 +817,14 @@ __global__ void k_encode(const float4* __restrict__ film, uint32_t npx, float ex
 namespace {
 uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
+// Items a wavefront takes per queue atomic: about an eighth of its even
+// share of the work, between one and four wavefront widths.
+uint32_t wave_chunk(uint64_t items, uint32_t blocks) {
+    uint64_t waves = (uint64_t)blocks * (kBlock / 64);
+    uint64_t c = items / (waves * 8 + 1);
+    return (uint32_t)(c < 64 ? 64 : (c > 256 ? 256 : c));
+}
+
 uint32_t bits_for(uint64_t maxv) {
     uint32_t b = 1;
     while (b < 64 && (maxv >> b) != 0) ++b;
@@ -755,16 +869,17 @@ Engine::Lds Engine::plan_lds(bool lights) const {
                         (n_obj + 3u) / 4u;
     uint32_t light_f4 =
         lights ? (uint32_t)(s.quad_rec.size() + s.sph_rec.size() + s.pnt_rec.size() + s.mat_rec.size()) : 0u;
-    size_t stack = (size_t)kBlock * (s.max_stack ? s.max_stack : 1u) * 2u;
+    const uint32_t depth = s.max_stack ? s.max_stack : 1u;
+    size_t stack = (size_t)kBlock * depth * 6u;  // f32 distances, then u16 node indices
     stack = (stack + 15u) & ~(size_t)15u;
     Lds r;
     size_t total = (size_t)(scene_f4 + light_f4) * 16u + stack;
     if (use_lds && total <= kLdsBudget) {
-        r.plan = LdsPlan{scene_f4, light_f4};
+        r.plan = LdsPlan{scene_f4, light_f4, depth};
         r.bytes = total;
         r.staged = true;
     } else {
-        r.plan = LdsPlan{0u, 0u};
+        r.plan = LdsPlan{0u, 0u, depth};
         r.bytes = stack;
         r.staged = false;
     }
@@ -1080,6 +1195,46 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     n_clusters = C;
     out.clusters = C;
 
+    // ---- work order: Morton cell of every surfel (coherent wavefronts only)
+    const float diag = scene.diag();
+    const SceneView S = dev_view();
+    CK(d_morton.ensure(4ull * N));
+    CK(d_mval.ensure(4ull * N));
+    CK(d_mkey_s.ensure(4ull * N));
+    CK(d_rorder.ensure(4ull * N));
+    {
+        const f3 sc = mk(32.0f / fmx(ext.x, 1e-6f), 32.0f / fmx(ext.y, 1e-6f), 32.0f / fmx(ext.z, 1e-6f));
+        launch("morton", [&] {
+            k_morton<<<blocks_for(N, 256), 256, 0, stream>>>(spos, N, lo, sc, d_morton.as<uint32_t>(),
+                                                             d_mval.as<uint32_t>());
+        });
+        tmp = 0;
+        CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_morton.as<uint32_t>(), d_mkey_s.as<uint32_t>(),
+                                              d_mval.as<uint32_t>(), d_rorder.as<uint32_t>(), (int)N, 0, 15, stream));
+        CK(d_tmp.ensure(tmp));
+        CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_morton.as<uint32_t>(), d_mkey_s.as<uint32_t>(),
+                                              d_mval.as<uint32_t>(), d_rorder.as<uint32_t>(), (int)N, 0, 15, stream));
+    }
+    CK(d_queue.ensure(16));
+    CK(hipMemsetAsync(d_queue.p, 0, 16, stream));
+    uint32_t* queue = d_queue.as<uint32_t>();
+    const uint32_t persist = (uint32_t)n_cu_ * 8u;
+
+    // ---- closure: independent of the links, launched first
+    CK(d_sky.ensure(4ull * N));
+    CK(d_open.ensure(4ull * N));
+    CK(hipMemsetAsync(d_open.p, 0, 4ull * N, stream));
+    {
+        const Lds ld = plan_lds(false);
+        const LightsView Lg{};
+        const uint32_t grid = std::max(1u, std::min<uint32_t>(persist, blocks_for(16ull * N, kBlock)));
+        launch("closure", [&] {
+            LAUNCH_CL(k_closure, count_ops_, ld.staged, grid, ld.bytes, S, Lg, ld.plan, spos, snrm, N,
+                      d_rorder.as<uint32_t>(), d_dirs.as<float4>(), diag * 2.0f, d_open.as<uint32_t>(), queue + 1,
+                      wave_chunk(16ull * N, grid), ctr, count_ops_ ? ctr + 2 : nullptr);
+        });
+    }
+
     // ---- candidates: count, scan, emit
     if ((uint64_t)N * C >= (1ull << 32)) {
         err = "surfels x clusters exceeds 2^32 candidate pairs";
@@ -1091,7 +1246,8 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     const uint32_t ffb = ff_blocks ? ff_blocks : std::min<uint32_t>(blocks_for(N, 4), 8192);
     launch("ff_count", [&] {
         k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
-                                                       d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr);
+                                                 d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                 nullptr);
     });
     tmp = 0;
     CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_ccount.as<uint32_t>(), d_cstart.as<uint32_t>(), (int)(N + 1),
@@ -1107,36 +1263,47 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         err = "candidate count exceeds 2^31";
         return -6;
     }
+    // Cluster-major, Morton-minor visiting order of the candidates.
+    const uint32_t vkey_bits = bits_for(C ? C - 1 : 0) + 15u;
+    const bool sort_cand = vkey_bits <= 32u && M > 0;
     CK(d_cand_ci.ensure(4ull * M));
     CK(d_cand_row.ensure(4ull * M));
     CK(d_cand_v.ensure(4ull * M));
+    if (sort_cand) {
+        CK(d_vkey.ensure(4ull * M));
+        CK(d_vval.ensure(4ull * M));
+        CK(d_vkey_s.ensure(4ull * M));
+        CK(d_vorder.ensure(4ull * M));
+    }
     launch("ff_emit", [&] {
-        k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
-                                                      nullptr, d_cstart.as<uint32_t>(), d_cand_ci.as<uint32_t>(),
-                                                      d_cand_row.as<uint32_t>());
+        k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C, nullptr,
+                                                d_cstart.as<uint32_t>(), d_cand_ci.as<uint32_t>(),
+                                                d_cand_row.as<uint32_t>(), d_morton.as<uint32_t>(),
+                                                sort_cand ? d_vkey.as<uint32_t>() : nullptr,
+                                                sort_cand ? d_vval.as<uint32_t>() : nullptr);
     });
+    if (sort_cand) {
+        tmp = 0;
+        CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_vkey.as<uint32_t>(), d_vkey_s.as<uint32_t>(),
+                                              d_vval.as<uint32_t>(), d_vorder.as<uint32_t>(), (int)M, 0,
+                                              (int)vkey_bits, stream));
+        CK(d_tmp.ensure(tmp));
+        CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_vkey.as<uint32_t>(), d_vkey_s.as<uint32_t>(),
+                                              d_vval.as<uint32_t>(), d_vorder.as<uint32_t>(), (int)M, 0,
+                                              (int)vkey_bits, stream));
+    }
 
-    // ---- visibility and closure
-    const float diag = scene.diag();
-    const SceneView S = dev_view();
+    // ---- link visibility
     if (M) {
-        const uint32_t visb = vis_blocks ? vis_blocks : std::min<uint32_t>(blocks_for(M, kBlock), 65535u * 4u);
+        const uint32_t visb =
+            vis_blocks ? vis_blocks : std::max(1u, std::min<uint32_t>(persist, blocks_for(M, kBlock)));
         const Lds ld = plan_lds(false);
         const LightsView Lg{};
         launch("visibility", [&] {
             LAUNCH_CL(k_visibility, count_ops_, ld.staged, visb, ld.bytes, S, Lg, ld.plan, spos, snrm,
                       d_cpos.as<float4>(), d_cnrm.as<float4>(), d_cand_ci.as<uint32_t>(), d_cand_row.as<uint32_t>(),
-                      M, diag * 5e-4f, diag * 1e-3f, d_cand_v.as<float>(), ctr, count_ops_ ? ctr + 1 : nullptr);
-        });
-    }
-    CK(d_sky.ensure(4ull * N));
-    {
-        const Lds ld = plan_lds(false);
-        const LightsView Lg{};
-        launch("closure", [&] {
-            LAUNCH_CL(k_closure, count_ops_, ld.staged, blocks_for(16ull * N, kBlock), ld.bytes, S, Lg, ld.plan, spos,
-                      snrm, N, d_dirs.as<float4>(), diag * 2.0f, d_sky.as<float>(), ctr,
-                      count_ops_ ? ctr + 2 : nullptr);
+                      sort_cand ? d_vorder.as<uint32_t>() : nullptr, M, diag * 5e-4f, diag * 1e-3f,
+                      d_cand_v.as<float>(), queue, wave_chunk(M, visb), ctr, count_ops_ ? ctr + 1 : nullptr);
         });
     }
 
@@ -1148,8 +1315,9 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     CK(hipMemsetAsync(d_lcount.as<uint32_t>() + N, 0, 4, stream));
     launch("row_sum", [&] {
         k_row_sum<<<blocks_for(N, 256), 256, 0, stream>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
-                                                          d_sky.as<float>(), d_lcount.as<uint32_t>(),
-                                                          d_scale.as<float>(), d_rowsum.as<float>());
+                                                          d_open.as<uint32_t>(), d_sky.as<float>(),
+                                                          d_lcount.as<uint32_t>(), d_scale.as<float>(),
+                                                          d_rowsum.as<float>());
     });
     tmp = 0;
     CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_lcount.as<uint32_t>(), d_lstart.as<uint32_t>(), (int)(N + 1),

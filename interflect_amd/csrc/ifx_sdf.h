@@ -166,10 +166,20 @@ constexpr uint32_t kEvalOps = 15u;
 // divergent push / pop never spills to scratch memory.  The count_* hooks
 // are empty except in CountStack, the instrumented variant that tallies
 // algorithmic operations for the roofline report.
+// Each entry is a node index and that node's AABB distance, computed by its
+// parent when it was pushed (the reference recomputes the same value on pop,
+// src/bvh.rs:116; keeping it is bit-identical and skips the node's loads).
 struct ArrStack {
     uint32_t v[64];
-    IFX_HD void put(int i, uint32_t x) { v[i] = x; }
-    IFX_HD uint32_t get(int i) const { return v[i]; }
+    float dv[64];
+    IFX_HD void put(int i, uint32_t x, float d) {
+        v[i] = x;
+        dv[i] = d;
+    }
+    IFX_HD uint32_t get(int i, float& d) const {
+        d = dv[i];
+        return v[i];
+    }
     IFX_HD void count_prim(uint32_t, uint32_t) {}
     IFX_HD void count_aabb(uint32_t) {}
     IFX_HD void count_eval() {}
@@ -177,9 +187,16 @@ struct ArrStack {
 
 struct LdsStack {
     uint16_t* base;
+    float* dbase;
     uint32_t stride;
-    __device__ __forceinline__ void put(int i, uint32_t x) { base[i * stride] = (uint16_t)x; }
-    __device__ __forceinline__ uint32_t get(int i) const { return base[i * stride]; }
+    __device__ __forceinline__ void put(int i, uint32_t x, float d) {
+        base[i * stride] = (uint16_t)x;
+        dbase[i * stride] = d;
+    }
+    __device__ __forceinline__ uint32_t get(int i, float& d) const {
+        d = dbase[i * stride];
+        return base[i * stride];
+    }
     __device__ __forceinline__ void count_prim(uint32_t, uint32_t) {}
     __device__ __forceinline__ void count_aabb(uint32_t) {}
     __device__ __forceinline__ void count_eval() {}
@@ -187,10 +204,17 @@ struct LdsStack {
 
 struct CountStack {
     uint16_t* base;
+    float* dbase;
     uint32_t stride;
     unsigned long long ops;
-    __device__ __forceinline__ void put(int i, uint32_t x) { base[i * stride] = (uint16_t)x; }
-    __device__ __forceinline__ uint32_t get(int i) const { return base[i * stride]; }
+    __device__ __forceinline__ void put(int i, uint32_t x, float d) {
+        base[i * stride] = (uint16_t)x;
+        dbase[i * stride] = d;
+    }
+    __device__ __forceinline__ uint32_t get(int i, float& d) const {
+        d = dbase[i * stride];
+        return base[i * stride];
+    }
     __device__ __forceinline__ void count_prim(uint32_t kind, uint32_t extra) { ops += kPrimOps[kind & 7u] + extra; }
     __device__ __forceinline__ void count_aabb(uint32_t n) { ops += kAabbOps * n; }
     __device__ __forceinline__ void count_eval() { ops += kEvalOps; }
@@ -222,13 +246,14 @@ template <bool BLEND, class Stk>
 IFX_HD void bvh_query(const SceneView& S, f3 p, BlendState& bs, Stk& stk) {
     if (S.n_nodes == 0) return;
     int sp = 0;
-    stk.put(sp++, 0u);
+    stk.count_aabb(1u);
+    stk.put(sp++, 0u, aabb_dist(p, S.nodes[0], S.nodes[1]));
     while (sp > 0) {
-        uint32_t ni = stk.get(--sp);
-        float4 lo = S.nodes[2 * ni], hi = S.nodes[2 * ni + 1];
+        float dn;
+        uint32_t ni = stk.get(--sp, dn);
         float thr = BLEND ? fmn(bs.hard.d, smin_poly(bs.b1.d, bs.b2, S.blend_k)) + S.blend_k : bs.hard.d;
-        stk.count_aabb(1u);
-        if (aabb_dist(p, lo, hi) >= thr) continue;
+        if (dn >= thr) continue;
+        float4 lo = S.nodes[2 * ni], hi = S.nodes[2 * ni + 1];
         uint32_t w0 = fbits(lo.w), cnt = fbits(hi.w);
         if (cnt > 0) {
             for (uint32_t k = w0; k < w0 + cnt; ++k) {
@@ -248,11 +273,11 @@ IFX_HD void bvh_query(const SceneView& S, f3 p, BlendState& bs, Stk& stk) {
             float dr = aabb_dist(p, S.nodes[2 * r], S.nodes[2 * r + 1]);
             stk.count_aabb(2u);
             if (dl < dr) {
-                stk.put(sp++, r);
-                stk.put(sp++, l);
+                stk.put(sp++, r, dr);
+                stk.put(sp++, l, dl);
             } else {
-                stk.put(sp++, l);
-                stk.put(sp++, r);
+                stk.put(sp++, l, dl);
+                stk.put(sp++, r, dr);
             }
         }
     }

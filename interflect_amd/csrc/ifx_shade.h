@@ -35,54 +35,93 @@ struct Hit {
     bool hit;
 };
 
+// Sphere-tracing march of src/trace.rs:27-58 as a resumable state machine:
+// begin() then step() until it reports the end.  trace() below is the plain
+// loop over it; persistent kernels interleave marches of many rays per lane.
+struct TraceMarch {
+    f3 ro, rd;
+    float t, tmax;
+    uint32_t steps;
+    IFX_HD void begin(f3 ro_, f3 rd_, float tmin, float tmax_) {
+        ro = ro_;
+        rd = rd_;
+        t = tmin;
+        tmax = tmax_;
+        steps = 0;
+    }
+    // One sample.  Returns 0 while marching, 1 on a hit (h.mat, point at t),
+    // 2 on a miss.  On a hit `steps` already includes the normal's 4 taps.
+    template <class Stk>
+    IFX_HD int step(const SceneView& S, DistHit& h, Stk& stk) {
+        if (steps >= kMaxSteps) return 2;
+        f3 p = ro + rd * t;
+        h = dist_mat(S, p, stk);
+        steps += 1;
+        float eps = 1e-4f * fmx(t, 1.0f);
+        if (h.d < eps) {
+            steps += 4;
+            return 1;
+        }
+        t += h.d;
+        return t > tmax ? 2 : 0;
+    }
+};
+
 // src/trace.rs:27-58.  NORMAL=false skips the 4-tap normal (closure rays
 // discard it); the step count still includes its 4 evaluations.
 template <bool NORMAL, class Stk>
 IFX_HD Hit trace(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, uint32_t& steps_out, Stk& stk) {
-    float t = tmin;
-    uint32_t steps = 0;
-    while (steps < kMaxSteps) {
-        f3 p = ro + rd * t;
-        DistHit h = dist_mat(S, p, stk);
-        steps += 1;
-        float eps = 1e-4f * fmx(t, 1.0f);
-        if (h.d < eps) {
-            Hit r;
-            r.t = t;
-            r.p = p;
-            r.n = NORMAL ? scene_normal(S, p, stk) : spl(0.0f);
-            r.mat = h.mat;
-            r.hit = true;
-            steps_out = steps + 4;
-            return r;
-        }
-        t += h.d;
-        if (t > tmax) break;
+    TraceMarch m;
+    m.begin(ro, rd, tmin, tmax);
+    DistHit h;
+    int st;
+    while ((st = m.step(S, h, stk)) == 0) {
     }
-    steps_out = steps;
+    steps_out = m.steps;
     Hit r;
-    r.t = 0.0f;
-    r.p = spl(0.0f);
-    r.n = spl(0.0f);
-    r.mat = 0;
-    r.hit = false;
+    if (st == 1) {
+        r.t = m.t;
+        r.p = ro + rd * m.t;
+        r.n = NORMAL ? scene_normal(S, r.p, stk) : spl(0.0f);
+        r.mat = h.mat;
+        r.hit = true;
+    } else {
+        r.t = 0.0f;
+        r.p = spl(0.0f);
+        r.n = spl(0.0f);
+        r.mat = 0;
+        r.hit = false;
+    }
     return r;
 }
 
-// src/trace.rs:84-124
-template <class Stk>
-IFX_HD float soft_shadow(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, float k, uint32_t& steps_out,
-                         Stk& stk) {
-    float res = 1.0f;
-    float t = tmin;
-    uint32_t steps = 0;
-    float ph = 1e20f;
-    while (steps < kShadowSteps && t < tmax) {
+// Cone soft shadow of src/trace.rs:84-124 as a resumable state machine.
+struct ShadowMarch {
+    f3 ro, rd;
+    float t, tmax, k, ph, res;
+    uint32_t steps;
+    IFX_HD void begin(f3 ro_, f3 rd_, float tmin, float tmax_, float k_) {
+        ro = ro_;
+        rd = rd_;
+        t = tmin;
+        tmax = tmax_;
+        k = k_;
+        ph = 1e20f;
+        res = 1.0f;
+        steps = 0;
+    }
+    // One sample.  Returns true when the march has ended, with `vis` set.
+    template <class Stk>
+    IFX_HD bool step(const SceneView& S, float& vis, Stk& stk) {
+        if (!(steps < kShadowSteps && t < tmax)) {
+            vis = clmp(res, 0.0f, 1.0f);
+            return true;
+        }
         float h = dist(S, ro + rd * t, stk);
         steps += 1;
         if (h < 1e-4f) {
-            steps_out = steps;
-            return 0.0f;
+            vis = 0.0f;
+            return true;
         }
         float y, d;
         if (h < ph) {
@@ -94,14 +133,26 @@ IFX_HD float soft_shadow(const SceneView& S, f3 ro, f3 rd, float tmin, float tma
         }
         res = fmn(res, k * d / fmx(t - y, 1e-4f));
         if (res < 1e-3f) {
-            steps_out = steps;
-            return 0.0f;
+            vis = 0.0f;
+            return true;
         }
         ph = h;
         t += clmp(h, 0.006f, 0.12f);
+        return false;
     }
-    steps_out = steps;
-    return clmp(res, 0.0f, 1.0f);
+};
+
+// src/trace.rs:84-124
+template <class Stk>
+IFX_HD float soft_shadow(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, float k, uint32_t& steps_out,
+                         Stk& stk) {
+    ShadowMarch m;
+    m.begin(ro, rd, tmin, tmax, k);
+    float vis;
+    while (!m.step(S, vis, stk)) {
+    }
+    steps_out = m.steps;
+    return vis;
 }
 
 // src/trace.rs:128-148
