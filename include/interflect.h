@@ -143,9 +143,48 @@ ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
  *   "lds"           1 = stage scene records in LDS when they fit (default), 0 = global
  *   "vis_blocks"    grid of the link-visibility kernel (0 = one lane per candidate)
  *   "ff_blocks"     grid of the candidate-cull kernels (0 = automatic)
- *   "jacobi_blocks" grid of the cooperative Jacobi kernel (0 = automatic) */
+ *   "jacobi_blocks" grid of the cooperative Jacobi kernel (0 = automatic)
+ *   "jacobi_rows"   0 = automatic (lane per row), 1 = one lane per row, 2 = one wavefront per row */
 ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value);
 ifx_status ifx_kernel_ops(ifx_ctx* ctx, const char* name, uint64_t* ops);
+
+/* ---- Adjoint (no counterpart in the reference, which has no backward pass).
+ * Gradients of a loss L on the linear beauty film of the last beauty
+ * ifx_shade (before exposure and tone mapping) with respect to each
+ * material's albedo and each light's emission, geometry frozen: transfer
+ * weights, light visibility and gather weights are held fixed and the solve
+ * is differentiated at its fixed point (implicit adjoint).  Diffuse
+ * materials only (metallic 0, roughness >= 0.45); surfels must carry their
+ * material (ifx_generate_surfels).  Materials are indexed as the scene
+ * indexes them: 0 is the implicit default material (src/scene.rs:373), k the
+ * k-th material block of the file, so there are n_materials + 1 of them.
+ * Lights are indexed quads first, then sphere lights, then point lights. */
+typedef struct {
+    uint32_t iterations; /* adjoint sweeps */
+    float residual;      /* max |delta mu| / max |mu| of the last sweep */
+    float ms;            /* wall time of the call */
+    float reserved;
+} ifx_backward_stats;
+
+/* Parameter updates that keep the transfer (albedo and emission do not enter it). */
+ifx_status ifx_set_material_albedo(ifx_ctx* ctx, uint32_t material, const float* rgb);
+ifx_status ifx_set_light_emit(ifx_ctx* ctx, uint32_t light, const float* rgb);
+/* Jacobi stopping threshold of ifx_solve (default 1e-4, src/solve.rs:98). */
+ifx_status ifx_set_solve_tolerance(ifx_ctx* ctx, float tol);
+/* Adjoint sweeps: stop once max |delta mu| <= rel_tol * max |mu| or after max_iters. */
+ifx_status ifx_set_adjoint(ifx_ctx* ctx, uint32_t max_iters, float rel_tol);
+/* dL_dfilm: width*height*3 floats of the last beauty shade.  Outputs (either may
+ * be NULL): dL_dalbedo[(n_materials + 1)*3], dL_demit[lights*3]. */
+ifx_status ifx_backward(ifx_ctx* ctx, const float* dL_dfilm, float* dL_dalbedo, float* dL_demit,
+                        ifx_backward_stats* out);
+/* The solve's share alone, for a given dL/dirr (N*3 floats of surfel
+ * irradiance); mu (N*3, may be NULL) receives the adjoint state. */
+ifx_status ifx_backward_surfels(ifx_ctx* ctx, const float* dL_dirr, float* dL_dalbedo, float* dL_demit, float* mu,
+                                ifx_backward_stats* out);
+/* Per-surfel material index (N u32) and the direct-light factors
+ * g[light*N + i] = form factor x visibility of the last solve. */
+ifx_status ifx_get_surfel_materials(ifx_ctx* ctx, uint32_t* mat);
+ifx_status ifx_get_direct_geometry(ifx_ctx* ctx, float* g);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,9 @@ of the reference's stage functions (Cherie05/interflect v0.1.3, src/*.rs):
     solve.light_surfels + solve        -> Context.solve    (src/solve.rs:27,47)
     render_frame + Film.encode         -> Context.shade    (src/main.rs:229, src/film.rs:51)
 
+Beyond the reference: Context.backward (implicit adjoint of the frozen-geometry
+pipeline: gradients of a film loss w.r.t. material albedo and light emission).
+
 There is no CPU fallback: if the library is missing, every entry raises.
 """
 import ctypes as C
@@ -33,6 +36,8 @@ EXPORTS = [
     "ifx_transfer", "ifx_get_transfer", "ifx_get_clusters", "ifx_solve", "ifx_get_radiosity", "ifx_set_radiosity",
     "ifx_shade", "ifx_dist_mat", "ifx_trace_rays", "ifx_soft_shadows", "ifx_set_profiling", "ifx_kernel_time",
     "ifx_reset_kernel_times", "ifx_set_counting", "ifx_kernel_ops", "ifx_set_tuning",
+    "ifx_set_material_albedo", "ifx_set_light_emit", "ifx_set_solve_tolerance", "ifx_set_adjoint", "ifx_backward",
+    "ifx_backward_surfels", "ifx_get_surfel_materials", "ifx_get_direct_geometry",
 ]
 
 
@@ -68,6 +73,10 @@ class SolveStats(C.Structure):
 
 class ShadeStats(C.Structure):
     _fields_ = [("evals", C.c_uint64), ("ms", C.c_float), ("reserved", C.c_float)]
+
+
+class BackwardStats(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("residual", C.c_float), ("ms", C.c_float), ("reserved", C.c_float)]
 
 
 def lib():
@@ -107,6 +116,14 @@ def lib():
         "ifx_set_counting": [vp, i32],
         "ifx_kernel_ops": [vp, C.c_char_p, C.POINTER(u64)],
         "ifx_set_tuning": [vp, C.c_char_p, C.c_int64],
+        "ifx_set_material_albedo": [vp, u32, P],
+        "ifx_set_light_emit": [vp, u32, P],
+        "ifx_set_solve_tolerance": [vp, f32],
+        "ifx_set_adjoint": [vp, u32, f32],
+        "ifx_backward": [vp, P, P, P, C.POINTER(BackwardStats)],
+        "ifx_backward_surfels": [vp, P, P, P, P, C.POINTER(BackwardStats)],
+        "ifx_get_surfel_materials": [vp, P],
+        "ifx_get_direct_geometry": [vp, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -328,3 +345,56 @@ class Context:
         n = C.c_uint64()
         self._check(lib().ifx_kernel_ops(self._h, name.encode(), C.byref(n)))
         return int(n.value)
+
+    # ---- adjoint (no counterpart in the reference)
+    def set_material_albedo(self, material, rgb):
+        a = np.ascontiguousarray(rgb, np.float32).reshape(3)
+        self._check(lib().ifx_set_material_albedo(self._h, int(material), _ptr(a)))
+
+    def set_light_emit(self, light, rgb):
+        a = np.ascontiguousarray(rgb, np.float32).reshape(3)
+        self._check(lib().ifx_set_light_emit(self._h, int(light), _ptr(a)))
+
+    def set_solve_tolerance(self, tol):
+        self._check(lib().ifx_set_solve_tolerance(self._h, float(tol)))
+
+    def set_adjoint(self, max_iters, rel_tol):
+        self._check(lib().ifx_set_adjoint(self._h, int(max_iters), float(rel_tol)))
+
+    def _grad_shapes(self):
+        info = self.info()
+        nl = info["n_quad_lights"] + info["n_sphere_lights"] + info["n_point_lights"]
+        # row 0: the implicit default material; row k: the k-th material of the file
+        return np.zeros((info["n_materials"] + 1, 3), np.float32), np.zeros((nl, 3), np.float32)
+
+    def backward(self, dL_dfilm):
+        """Gradients of L w.r.t. albedo ((n_materials + 1) x 3, row 0 the implicit
+        default material) and emission (lights x 3) for dL/dfilm (H x W x 3) of
+        the last beauty shade."""
+        u = np.ascontiguousarray(dL_dfilm, np.float32)
+        da, de = self._grad_shapes()
+        st = BackwardStats()
+        self._check(lib().ifx_backward(self._h, _ptr(u), _ptr(da), _ptr(de), C.byref(st)))
+        return {"albedo": da, "emit": de, "iterations": st.iterations, "residual": st.residual, "ms": st.ms}
+
+    def backward_surfels(self, dL_dirr):
+        """The solve's share for dL/dirr (N x 3): gradients and the adjoint state mu."""
+        u = _f3(dL_dirr)
+        da, de = self._grad_shapes()
+        mu = np.zeros_like(u)
+        st = BackwardStats()
+        self._check(lib().ifx_backward_surfels(self._h, _ptr(u), _ptr(da), _ptr(de), _ptr(mu), C.byref(st)))
+        return {"albedo": da, "emit": de, "mu": mu, "iterations": st.iterations, "residual": st.residual,
+                "ms": st.ms}
+
+    def surfel_materials(self):
+        m = np.zeros(self.num_surfels(), np.uint32)
+        self._check(lib().ifx_get_surfel_materials(self._h, _ptr(m)))
+        return m
+
+    def direct_geometry(self):
+        info = self.info()
+        nl = info["n_quad_lights"] + info["n_sphere_lights"] + info["n_point_lights"]
+        g = np.zeros((nl, self.num_surfels()), np.float32)
+        self._check(lib().ifx_get_direct_geometry(self._h, _ptr(g)))
+        return g

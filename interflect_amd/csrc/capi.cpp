@@ -350,11 +350,102 @@ ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value) {
         else if (k == "vis_blocks") ctx->eng.vis_blocks = (uint32_t)value;
         else if (k == "ff_blocks") ctx->eng.ff_blocks = (uint32_t)value;
         else if (k == "jacobi_blocks") ctx->eng.jacobi_blocks = (uint32_t)value;
+        else if (k == "jacobi_rows") ctx->eng.jacobi_rows = (uint32_t)value;
+
         else {
             ctx->err = "unknown tuning key `" + k + "`";
             return IFX_E_INVALID;
         }
         return IFX_OK;
+    });
+}
+
+ifx_status ifx_set_material_albedo(ifx_ctx* ctx, uint32_t material, const float* rgb) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        return ctx->eng.set_material_albedo(material, rgb, ctx->err);
+    });
+}
+
+ifx_status ifx_set_light_emit(ifx_ctx* ctx, uint32_t light, const float* rgb) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        return ctx->eng.set_light_emit(light, rgb, ctx->err);
+    });
+}
+
+ifx_status ifx_set_solve_tolerance(ifx_ctx* ctx, float tol) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (!(tol >= 0.0f) || !(tol < 1e30f)) {
+            ctx->err = "tolerance must be finite and >= 0";
+            return IFX_E_INVALID;
+        }
+        ctx->eng.solve_tol = tol;
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_set_adjoint(ifx_ctx* ctx, uint32_t max_iters, float rel_tol) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (max_iters == 0 || max_iters > 100000 || !(rel_tol >= 0.0f) || !(rel_tol < 1.0f)) {
+            ctx->err = "adjoint settings out of range (1..100000 sweeps, 0 <= rel_tol < 1)";
+            return IFX_E_INVALID;
+        }
+        ctx->eng.adj_max_iters = max_iters;
+        ctx->eng.adj_tol = rel_tol;
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_backward(ifx_ctx* ctx, const float* dL_dfilm, float* dL_dalbedo, float* dL_demit,
+                        ifx_backward_stats* o) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        auto t0 = std::chrono::steady_clock::now();
+        ifx::BackwardOut r;
+        int rc = ctx->eng.backward(dL_dfilm, dL_dalbedo, dL_demit, r, ctx->err);
+        if (rc) return rc;
+        if (o) {
+            std::memset(o, 0, sizeof(*o));
+            o->iterations = r.iterations;
+            o->residual = r.residual;
+            o->ms = (float)ms_since(t0);
+        }
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_backward_surfels(ifx_ctx* ctx, const float* dL_dirr, float* dL_dalbedo, float* dL_demit, float* mu,
+                                ifx_backward_stats* o) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        auto t0 = std::chrono::steady_clock::now();
+        ifx::BackwardOut r;
+        int rc = ctx->eng.backward_surfels(dL_dirr, dL_dalbedo, dL_demit, mu, r, ctx->err);
+        if (rc) return rc;
+        if (o) {
+            std::memset(o, 0, sizeof(*o));
+            o->iterations = r.iterations;
+            o->residual = r.residual;
+            o->ms = (float)ms_since(t0);
+        }
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_get_surfel_materials(ifx_ctx* ctx, uint32_t* mat) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (!mat) return IFX_E_INVALID;
+        const auto& s = ctx->eng.surfels;
+        for (size_t i = 0; i < s.size(); ++i) mat[i] = s[i].mat;
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_get_direct_geometry(ifx_ctx* ctx, float* g) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        return ctx->eng.download_direct_geometry(g, ctx->err);
     });
 }
 

@@ -45,6 +45,31 @@ struct SolveOut {
     uint64_t evals = 0;
 };
 
+struct BackwardOut {
+    uint32_t iterations = 0;  // adjoint Jacobi sweeps
+    float residual = 0;       // last max |delta mu| / max |mu|
+};
+
+struct CamV {
+    f3 origin, fwd, right, up;
+    float tan_half, aspect;
+};
+
+// Arguments of one render_frame pass (src/main.rs:229-367).
+struct ShadeArgs {
+    CamV cam;
+    uint32_t W, H;
+    int32_t mode;
+    int32_t use_gi;
+    float tmax, radius;
+    GridView grid;
+    const float4* spos;
+    const float4* snrm;
+    const float4* irr;
+};
+
+enum { M_BEAUTY = 0, M_DIRECT = 1, M_INDIRECT = 2, M_NORMALS = 3, M_DEPTH = 4, M_STEPS = 5 };
+
 struct KernelStat {
     uint64_t launches = 0;
     double ms = 0;
@@ -69,6 +94,14 @@ class Engine {
                     uint32_t* mat, int32_t* hit, uint32_t* steps, std::string& err);
     int probe_shadow(const float* ro, const float* rd, uint64_t n, float tmin, const float* tmax, const float* k,
                      float* vis, uint32_t* steps, std::string& err);
+
+    // Adjoint of the frozen-geometry pipeline (surfel albedo, light emission).
+    int set_material_albedo(uint32_t m, const float* rgb, std::string& err);
+    int set_light_emit(uint32_t li, const float* rgb, std::string& err);
+    int backward(const float* dL_dfilm, float* d_albedo, float* d_emit, BackwardOut& out, std::string& err);
+    int backward_surfels(const float* dL_dirr, float* d_albedo, float* d_emit, float* mu, BackwardOut& out,
+                         std::string& err);
+    int download_direct_geometry(float* g, std::string& err);
 
     int download_transfer(uint32_t* starts, uint32_t* idx, float* w, float* sky, float* row_sum, std::string& err);
     int download_clusters(float* p, float* n, float* area, float* radius, uint32_t* mstart, uint32_t* members,
@@ -96,6 +129,11 @@ class Engine {
 
     bool use_lds = true;  // stage scene records in LDS when they fit
     uint32_t vis_blocks = 0, ff_blocks = 0, jacobi_blocks = 0;  // 0 = automatic
+    uint32_t jacobi_rows = 0;  // 0 automatic, 1 lane per row, 2 wavefront per row
+    float solve_tol = 1e-4f;   // Jacobi stops once max |delta| < solve_tol (src/solve.rs:98)
+    uint32_t adj_max_iters = 1000;  // adjoint sweeps at most
+    float adj_tol = 1e-6f;          // adjoint stops once max |delta mu| <= adj_tol * max |mu|
+    uint32_t adj_seg = 1024;        // transposed-link entries per partial sum (fixes the summation order)
 
    private:
     struct Lds {
@@ -104,6 +142,13 @@ class Engine {
         bool staged;
     };
     Lds plan_lds(bool lights) const;
+    int shade_setup(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, ShadeArgs& A, std::string& err);
+    int adjoint_prepare(std::string& err);
+    int adjoint_solve(BackwardOut& out, std::string& err);
+    int adjoint_params(bool pixels, uint64_t npx, float* d_albedo, float* d_emit, std::string& err);
+    int check_backward(std::string& err) const;
+    void upload_materials();
+    void upload_lights();
     int check(hipError_t e, const char* what, std::string& err);
     int need_device(std::string& err) const;
     SceneView dev_view() const;
@@ -139,11 +184,21 @@ class Engine {
     // candidates and links
     DevBuf d_ccount, d_cstart, d_cand_ci, d_cand_row, d_cand_v, d_sky, d_lcount, d_lstart, d_scale, d_rowsum, d_idx, d_w;
     // solve
-    DevBuf d_direct, d_irr, d_rad, d_bc, d_resid, d_terms, d_sync;
+    DevBuf d_direct, d_irr, d_rad, d_bc, d_resid, d_terms, d_gterm, d_sync;
     // shade
     DevBuf d_gkey, d_gkey2, d_gval, d_gval2, d_gstart, d_film, d_rgb8;
     // misc
     DevBuf d_counter, d_tmp, d_probe_in, d_probe_out;
+    // adjoint: surfel materials, transposed links (cluster-major), segments, iterates, parameter partials
+    DevBuf d_smat, d_scl, d_sscale, d_lrow, d_perm, d_perm2, d_ckeyt, d_ckeyt2, d_crow, d_cw, d_colcnt, d_colstart,
+        d_segcnt, d_segfirst, d_seg, d_segsum, d_ubar, d_uacc, d_umax, d_mu, d_v, d_dmax, d_pmat, d_pdr, d_pde, d_sdr,
+        d_sde, d_partial;
+    DevBuf d_ufilm;
+    uint64_t transfer_gen_ = 0, adj_gen_ = ~0ull;
+    uint32_t adj_nseg_ = 0;
+    bool have_beauty_ = false, have_gterm_ = false;
+    HCamera beauty_cam_;
+    uint32_t beauty_w_ = 0, beauty_h_ = 0;
     uint32_t gdim_[3] = {1, 1, 1};
 };
 

@@ -18,7 +18,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "engine.h"
+#include "ifx_kern.h"
 
 namespace ifx {
 
@@ -43,122 +43,6 @@ void DevBuf::release() {
     p = nullptr;
     cap = 0;
 }
-
-// ====================================================================== device helpers
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        uint32_t u = __shfl_xor(v, o, 64);
-        v = u > v ? u : v;
-    }
-    return v;
-}
-
-// Adds a per-lane step count to a 64-bit counter, one atomic per wave.
-__device__ __forceinline__ void count_steps(unsigned long long* ctr, uint32_t steps) {
-    uint32_t t = wave_sum(steps);
-    if (lane_id() == 0 && t) atomicAdd(ctr, (unsigned long long)t);
-}
-
-// Number of set bits of `m` at lanes below this one.
-__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// Dynamic LDS of every traversal kernel: [scene records][lights, materials]
-// [traversal stack].  Staging the scene puts every node / object / light
-// read of the divergent traversal on the LDS path instead of the vector
-// cache; scenes too large for the budget are read from global memory and
-// only the stack lives in LDS.
-constexpr uint32_t kLdsBudget = 64u * 1024u;
-
-__device__ __forceinline__ void copy_f4(float4* dst, const float4* __restrict__ src, uint32_t n) {
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
-}
-
-// Copies the scene (and lights) into LDS and repoints the views.  Every
-// thread of the block must call it (it ends in a barrier).
-template <bool LDS, bool LIGHTS>
-__device__ __forceinline__ void stage_in(float4* lds, SceneView& S, LightsView& L) {
-    if constexpr (LDS) {
-        float4* q = lds;
-        uint32_t n = 4u * S.n_obj;
-        copy_f4(q, S.obj, n);
-        S.obj = q;
-        q += n;
-        n = 4u * S.n_planes;
-        copy_f4(q, S.planes, n);
-        S.planes = q;
-        q += n;
-        n = 4u * S.n_subs;
-        copy_f4(q, S.subs, n);
-        S.subs = q;
-        q += n;
-        n = 2u * S.n_nodes;
-        copy_f4(q, S.nodes, n);
-        S.nodes = q;
-        q += n;
-        n = (S.n_obj + 3u) / 4u;
-        copy_f4(q, (const float4*)S.lin, n);
-        S.lin = (const uint32_t*)q;
-        q += n;
-        if constexpr (LIGHTS) {
-            n = 7u * L.n_quad;
-            copy_f4(q, L.quad, n);
-            L.quad = q;
-            q += n;
-            n = 2u * L.n_sph;
-            copy_f4(q, L.sph, n);
-            L.sph = q;
-            q += n;
-            n = 2u * L.n_pnt;
-            copy_f4(q, L.pnt, n);
-            L.pnt = q;
-            q += n;
-            n = 4u * L.n_mat;
-            copy_f4(q, L.mat, n);
-            L.mat = q;
-        }
-        __syncthreads();
-    }
-}
-
-// Timed kernels traverse with LdsStack; the instrumented instantiation
-// (COUNT=true) uses CountStack and adds its operation tally to `ops`.
-template <bool COUNT>
-struct StackOf {
-    using T = LdsStack;
-};
-template <>
-struct StackOf<true> {
-    using T = CountStack;
-};
-__device__ __forceinline__ void stk_init(LdsStack&) {}
-__device__ __forceinline__ void stk_init(CountStack& s) { s.ops = 0ull; }
-__device__ __forceinline__ void stk_flush(LdsStack&, unsigned long long*) {}
-__device__ __forceinline__ void stk_flush(CountStack& s, unsigned long long* ops) {
-    if (s.ops) atomicAdd(ops, s.ops);
-}
-
-// Kernel prologue: stage, then place this lane's stack column after the staged records.
-#define IFX_SETUP(LDS, LIGHTS, COUNT)                                            \
-    extern __shared__ float4 ifx_lds[];                                         \
-    SceneView S = Sg;                                                           \
-    LightsView L = Lg;                                                          \
-    stage_in<LDS, LIGHTS>(ifx_lds, S, L);                                       \
-    typename StackOf<COUNT>::T stk;                                             \
-    stk.dbase = (float*)(ifx_lds + P.scene_f4 + P.light_f4) + threadIdx.x;      \
-    stk.base = (uint16_t*)(stk.dbase - threadIdx.x + P.stack_depth * kBlock) +  \
-               threadIdx.x;                                                     \
-    stk.stride = (uint32_t)kBlock;                                              \
-    stk_init(stk);                                                              \
-    (void)L
 
 // src/formfactor.rs:70-89
 __device__ __forceinline__ uint32_t normal_bucket(f3 n) {
@@ -544,19 +428,22 @@ __global__ __launch_bounds__(kBlock) void k_direct_terms(SceneView Sg, LightsVie
                                                          const float4* __restrict__ spos,
                                                          const float4* __restrict__ snrm,
                                                          const float4* __restrict__ salb, uint32_t N, uint32_t nl,
-                                                         float4* terms, unsigned long long* evals,
+                                                         const uint32_t* __restrict__ row_order, float4* terms,
+                                                         float* gterm, unsigned long long* evals,
                                                          unsigned long long* ops) {
     IFX_SETUP(LDS, true, COUNT);
     uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     uint32_t steps = 0;
     if (t < N * nl) {
-        uint32_t li = t / N, i = t - li * N;
+        uint32_t li = t / N, i = row_order[t - li * N];
         f3 p = ld3(spos[i]), n = ld3(snrm[i]);
         f3 tan, bit;
         frame(n, tan, bit);
         f3 term = spl(0.0f);
-        bool on = diffuse_term(S, L, li, p, n, tan, bit, ld3(salb[i]), term, steps, stk);
-        terms[t] = on ? st4(term, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float g = 0.0f;
+        bool on = diffuse_term(S, L, li, p, n, tan, bit, ld3(salb[i]), term, g, steps, stk);
+        terms[(size_t)li * N + i] = on ? st4(term, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        gterm[(size_t)li * N + i] = on ? g : 0.0f;
     }
     count_steps(evals, steps);
     stk_flush(stk, ops);
@@ -583,7 +470,8 @@ __global__ void k_direct_sum(uint32_t N, uint32_t nl, const float4* __restrict__
 // integer max of non-negative float bits; a grid barrier; then every lane
 // reads the same residual and leaves together once it is below 1e-4.
 struct JacobiArgs {
-    uint32_t N, C, bounces;
+    uint32_t N, C, bounces, wave_rows;
+    float tol;  // leave once the max-norm update is below tol (the reference's 1e-4)
     const uint2* range;
     const uint32_t* members;
     const float4* spos;
@@ -599,28 +487,6 @@ struct JacobiArgs {
     uint32_t* resid;  // [bounces] max |delta| bits
     uint32_t* sync;   // [0] barrier arrivals, [1] fault flag
 };
-
-// Grid-wide barrier for a cooperatively launched (co-resident) grid.  The
-// wait is bounded: on timeout the fault flag is raised and the block goes on,
-// so every wave still reaches the kernel's end.
-__device__ __forceinline__ void grid_barrier(uint32_t* sync, uint32_t& target) {
-    __threadfence();
-    __syncthreads();
-    target += gridDim.x;
-    if (threadIdx.x == 0) {
-        atomicAdd(&sync[0], 1u);
-        uint32_t spins = 0;
-        while (__hip_atomic_load(&sync[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (++spins > (1u << 22)) {
-                atomicOr(&sync[1], 1u);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __syncthreads();
-    __threadfence();
-}
 
 __global__ __launch_bounds__(kBlock) void k_jacobi(JacobiArgs A) {
     const uint32_t lane = lane_id();
@@ -650,20 +516,48 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(JacobiArgs A) {
         }
         grid_barrier(A.sync, target);
         uint32_t delta = 0;
-        for (uint32_t i = gt; i < A.N; i += nt) {
-            f3 g = spl(0.0f);
-            uint32_t e = A.lstart[i + 1];
-            for (uint32_t k = A.lstart[i]; k < e; ++k) g = g + ld3(A.bc[A.idx[k]]) * A.w[k];
-            f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
-            uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
-            delta = d > delta ? d : delta;
-            A.irr[i] = st4(g);
-            A.rad[i] = st4(nr);
+        if (A.wave_rows) {
+            // Long rows: one wavefront per row, coalesced link loads, the
+            // products summed in link order through readlane.
+            for (uint32_t i = gw; i < A.N; i += nw) {
+                const uint32_t s0 = A.lstart[i], e = A.lstart[i + 1];
+                f3 g = spl(0.0f);
+                for (uint32_t b = s0; b < e; b += 64u) {
+                    uint32_t k = b + lane;
+                    f3 term = spl(0.0f);
+                    if (k < e) term = ld3(A.bc[A.idx[k]]) * A.w[k];
+                    uint32_t cnt = e - b < 64u ? e - b : 64u;
+                    for (uint32_t j = 0; j < cnt; ++j) {
+                        f3 tj = mk(bitsf(__builtin_amdgcn_readlane(fbits(term.x), j)),
+                                   bitsf(__builtin_amdgcn_readlane(fbits(term.y), j)),
+                                   bitsf(__builtin_amdgcn_readlane(fbits(term.z), j)));
+                        g = g + tj;
+                    }
+                }
+                if (lane == 0) {
+                    f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
+                    uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
+                    delta = d > delta ? d : delta;
+                    A.irr[i] = st4(g);
+                    A.rad[i] = st4(nr);
+                }
+            }
+        } else {
+            for (uint32_t i = gt; i < A.N; i += nt) {
+                f3 g = spl(0.0f);
+                uint32_t e = A.lstart[i + 1];
+                for (uint32_t k = A.lstart[i]; k < e; ++k) g = g + ld3(A.bc[A.idx[k]]) * A.w[k];
+                f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
+                uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
+                delta = d > delta ? d : delta;
+                A.irr[i] = st4(g);
+                A.rad[i] = st4(nr);
+            }
         }
         uint32_t m = wave_max(delta);
         if (lane == 0 && m) atomicMax(&A.resid[it], m);
         grid_barrier(A.sync, target);
-        if (bitsf(__hip_atomic_load(&A.resid[it], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < 1e-4f) break;
+        if (bitsf(__hip_atomic_load(&A.resid[it], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < A.tol) break;
     }
 }
 
@@ -681,25 +575,6 @@ __global__ void k_grid_keys(const float4* __restrict__ spos, uint32_t N, f3 lo, 
     vals[i] = i;
     atomicAdd(&counts[c], 1u);
 }
-
-struct CamV {
-    f3 origin, fwd, right, up;
-    float tan_half, aspect;
-};
-
-struct ShadeArgs {
-    CamV cam;
-    uint32_t W, H;
-    int32_t mode;
-    int32_t use_gi;
-    float tmax, radius;
-    GridView grid;
-    const float4* spos;
-    const float4* snrm;
-    const float4* irr;
-};
-
-enum { M_BEAUTY = 0, M_DIRECT = 1, M_INDIRECT = 2, M_NORMALS = 3, M_DEPTH = 4, M_STEPS = 5 };
 
 // One pixel of render_frame (src/main.rs:247-361).
 template <class Stk>
@@ -815,22 +690,6 @@ __global__ void k_encode(const float4* __restrict__ film, uint32_t npx, float ex
 
 // ====================================================================== host side
 namespace {
-uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
-
-// Items a wavefront takes per queue atomic: about an eighth of its even
-// share of the work, between one and four wavefront widths.
-uint32_t wave_chunk(uint64_t items, uint32_t blocks) {
-    uint64_t waves = (uint64_t)blocks * (kBlock / 64);
-    uint64_t c = items / (waves * 8 + 1);
-    return (uint32_t)(c < 64 ? 64 : (c > 256 ? 256 : c));
-}
-
-uint32_t bits_for(uint64_t maxv) {
-    uint32_t b = 1;
-    while (b < 64 && (maxv >> b) != 0) ++b;
-    return b;
-}
-
 float host_srgb(float c) {
     c = clmp(c, 0.0f, 1.0f);
     float v = c <= 0.0031308f ? c * 12.92f : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
@@ -838,29 +697,6 @@ float host_srgb(float c) {
     return (float)(!(q > 0.0f) ? 0 : (q >= 255.0f ? 255 : (int)q));
 }
 }  // namespace
-
-#define LAUNCH_CL(K, COUNT, LDS, GRID, DYN, ...)                                          \
-    do {                                                                                 \
-        if (COUNT) {                                                                     \
-            if (LDS)                                                                     \
-                K<true, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);               \
-            else                                                                         \
-                K<true, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);              \
-        } else {                                                                         \
-            if (LDS)                                                                     \
-                K<false, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);              \
-            else                                                                         \
-                K<false, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);             \
-        }                                                                                \
-    } while (0)
-
-#define LAUNCH_L(K, LDS, GRID, DYN, ...)                                                 \
-    do {                                                                                 \
-        if (LDS)                                                                         \
-            K<true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);                         \
-        else                                                                             \
-            K<false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);                        \
-    } while (0)
 
 Engine::Lds Engine::plan_lds(bool lights) const {
     const HostScene& s = scene;
@@ -893,11 +729,6 @@ int Engine::check(hipError_t e, const char* what, std::string& err) {
     return e == hipErrorOutOfMemory ? -5 : -4;
 }
 
-#define CK(call)                                       \
-    do {                                               \
-        int rc_ = check((call), #call, err);           \
-        if (rc_) return rc_;                           \
-    } while (0)
 
 Engine::~Engine() {
     if (stream) {
@@ -968,28 +799,6 @@ int Engine::need_device(std::string& err) const {
     return -3;
 }
 
-template <class F>
-void Engine::launch(const char* name, F&& f) {
-    if (!prof_) {
-        f();
-        return;
-    }
-    hipEvent_t a, b;
-    if (event_pool_.size() >= 2) {
-        a = event_pool_.back();
-        event_pool_.pop_back();
-        b = event_pool_.back();
-        event_pool_.pop_back();
-    } else {
-        (void)hipEventCreate(&a);
-        (void)hipEventCreate(&b);
-    }
-    (void)hipEventRecord(a, stream);
-    f();
-    (void)hipEventRecord(b, stream);
-    pending_.push_back(Pending{name, a, b});
-}
-
 void Engine::drain_events() {
     for (Pending& p : pending_) {
         float ms = 0.0f;
@@ -1040,7 +849,7 @@ int Engine::load_scene(const std::string& src, std::string& err) {
         scene = std::move(s);
         surfels.clear();
         surfels_dirty = true;
-        have_transfer = have_solve = false;
+        have_transfer = have_solve = have_gterm_ = false;
         return 0;
     }
     CK(hipSetDevice(device_));
@@ -1066,7 +875,7 @@ int Engine::load_scene(const std::string& src, std::string& err) {
     scene = std::move(s);
     surfels.clear();
     surfels_dirty = true;
-    have_transfer = have_solve = false;
+    have_transfer = have_solve = have_gterm_ = false;
     return 0;
 }
 
@@ -1077,7 +886,7 @@ int Engine::set_surfels(const std::vector<HSurfel>& s, std::string& err) {
     }
     surfels = s;
     surfels_dirty = true;
-    have_transfer = have_solve = false;
+    have_transfer = have_solve = have_gterm_ = false;
     (void)err;
     return 0;
 }
@@ -1090,23 +899,28 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     }
     CK(hipSetDevice(device_));
     const uint32_t N = (uint32_t)surfels.size();
-    have_transfer = have_solve = false;
+    have_transfer = have_solve = have_beauty_ = have_gterm_ = false;
+    ++transfer_gen_;
     out = TransferOut();
     // H2D of the surfel records (part of the transfer stage).
     if (surfels_dirty) {
         std::vector<float4> hp(N), hn(N), ha(N);
+        std::vector<uint32_t> hm(N);
         for (uint32_t i = 0; i < N; ++i) {
             hp[i] = st4(surfels[i].p, surfels[i].area);
             hn[i] = st4(surfels[i].n);
             ha[i] = st4(surfels[i].albedo);
+            hm[i] = surfels[i].mat;
         }
         CK(d_spos.ensure(16ull * N));
         CK(d_snrm.ensure(16ull * N));
         CK(d_salb.ensure(16ull * N));
+        CK(d_smat.ensure(4ull * N));
         if (N) {
             CK(hipMemcpyAsync(d_spos.p, hp.data(), 16ull * N, hipMemcpyHostToDevice, stream));
             CK(hipMemcpyAsync(d_snrm.p, hn.data(), 16ull * N, hipMemcpyHostToDevice, stream));
             CK(hipMemcpyAsync(d_salb.p, ha.data(), 16ull * N, hipMemcpyHostToDevice, stream));
+            CK(hipMemcpyAsync(d_smat.p, hm.data(), 4ull * N, hipMemcpyHostToDevice, stream));
             CK(hipStreamSynchronize(stream));
         }
         surfels_dirty = false;
@@ -1372,6 +1186,7 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
     CK(d_bc.ensure(16ull * (n_clusters + 1)));
     CK(d_resid.ensure(4ull * (bounces + 1)));
     CK(d_terms.ensure(16ull * N * (nl ? nl : 1)));
+    CK(d_gterm.ensure(4ull * N * (nl ? nl : 1)));
     CK(d_sync.ensure(16));
     unsigned long long* ctr = d_counter.as<unsigned long long>();
     CK(hipMemsetAsync(ctr, 0, 64, stream));
@@ -1384,7 +1199,8 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
             launch("direct_e", [&] {
                 LAUNCH_CL(k_direct_terms, count_ops_, ld.staged, blocks_for((uint64_t)N * nl, kBlock), ld.bytes, S,
                           Lv, ld.plan, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, nl,
-                          d_terms.as<float4>(), ctr, count_ops_ ? ctr + 3 : nullptr);
+                          d_rorder.as<uint32_t>(), d_terms.as<float4>(), d_gterm.as<float>(), ctr,
+                          count_ops_ ? ctr + 3 : nullptr);
             });
         }
         launch("direct_sum", [&] {
@@ -1398,6 +1214,9 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
         A.N = N;
         A.C = C;
         A.bounces = bounces;
+        A.tol = solve_tol;
+        // Rows of >= 16 links on average take a wavefront each (tunable).
+        A.wave_rows = jacobi_rows == 2 ? 1u : 0u;  // lane per row measured faster at 51 and 237 links/row
         A.range = d_crange.as<uint2>();
         A.members = d_cval2.as<uint32_t>();
         A.spos = d_spos.as<float4>();
@@ -1414,7 +1233,8 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
         A.sync = d_sync.as<uint32_t>();
         int per_cu = 0;
         CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jacobi, kBlock, 0));
-        uint32_t want = std::max<uint32_t>(blocks_for(N, kBlock), blocks_for(64ull * C, kBlock));
+        uint32_t want = std::max<uint32_t>(blocks_for(A.wave_rows ? 64ull * N : N, kBlock),
+                                           blocks_for(64ull * C, kBlock));
         uint32_t cap = (uint32_t)std::max(1, per_cu) * (uint32_t)n_cu_;
         uint32_t grid = std::max<uint32_t>(1u, std::min(jacobi_blocks ? jacobi_blocks : want, cap));
         void* args[] = {&A};
@@ -1439,22 +1259,24 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
     unsigned long long ev = cv[0];
     if (count_ops_) kops_["direct_e"] += cv[3];
     out.evals = ev;
+    have_beauty_ = false;
     if (C > 0 && N > 0) {
         out.iterations = bounces;
         for (uint32_t it = 0; it < bounces; ++it) {
-            if (bitsf(res[it]) < 1e-4f) {
+            if (bitsf(res[it]) < solve_tol) {
                 out.iterations = it + 1;
                 break;
             }
         }
         out.residual = bitsf(res[out.iterations - 1]);
     }
-    have_solve = true;
+    have_solve = have_gterm_ = true;
     return 0;
 }
 
-int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, float* film, uint8_t* rgb8,
-                  uint64_t& evals, std::string& err) {
+// Camera, gather radius and (for the GI modes) the surfel grid of one
+// render_frame call; shared by shade() and backward().
+int Engine::shade_setup(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, ShadeArgs& A, std::string& err) {
     if (int rc0 = need_device(err)) return rc0;
     if (mode < 0 || mode > 5) {
         err = "unknown shading mode";
@@ -1472,10 +1294,6 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
     }
     const uint32_t N = (uint32_t)surfels.size();
     const float diag = scene.diag();
-    unsigned long long* ctr = d_counter.as<unsigned long long>();
-    CK(hipMemsetAsync(ctr, 0, 64, stream));
-
-    ShadeArgs A;
     // Camera basis (src/main.rs:199-211), evaluated on the host like the reference.
     A.cam.origin = cam.pos;
     A.cam.fwd = nrm3(cam.look - cam.pos);
@@ -1546,6 +1364,16 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
         A.grid.starts = d_gstart.as<uint32_t>();
         A.grid.items = d_gval2.as<uint32_t>();
     }
+    return 0;
+}
+
+int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, float* film, uint8_t* rgb8,
+                  uint64_t& evals, std::string& err) {
+    ShadeArgs A;
+    if (int rc = shade_setup(cam, W, H, mode, A, err)) return rc;
+    have_beauty_ = false;
+    unsigned long long* ctr = d_counter.as<unsigned long long>();
+    CK(hipMemsetAsync(ctr, 0, 64, stream));
     const uint64_t npx = (uint64_t)W * H;
     CK(d_film.ensure(16ull * npx));
     CK(d_rgb8.ensure(3ull * npx));
@@ -1575,6 +1403,12 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
     drain_events();
     if (count_ops_) kops_["shade"] += cv[4];
     evals = cv[0];
+    if (mode == M_BEAUTY) {
+        have_beauty_ = true;
+        beauty_cam_ = cam;
+        beauty_w_ = W;
+        beauty_h_ = H;
+    }
     return 0;
 }
 

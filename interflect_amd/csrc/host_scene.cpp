@@ -461,11 +461,17 @@ void finalize_scene(HostScene& s) {
     for (const HPointLight& q : s.points) s.pnt_rec.push_back(st4(q.pos)), s.pnt_rec.push_back(st4(q.emit));
     s.mat_rec.clear();
     for (const HMaterial& m : s.mats) {
-        s.mat_rec.push_back(st4(m.albedo, m.roughness));
-        s.mat_rec.push_back(st4(m.emissive, m.metallic));
-        s.mat_rec.push_back(st4(lerp3(spl(0.04f), m.albedo, m.metallic)));
-        s.mat_rec.push_back(st4(m.albedo * (1.0f - m.metallic)));
+        float4 r[4];
+        material_records(m, r);
+        s.mat_rec.insert(s.mat_rec.end(), r, r + 4);
     }
+}
+
+void material_records(const HMaterial& m, float4 out[4]) {
+    out[0] = st4(m.albedo, m.roughness);
+    out[1] = st4(m.emissive, m.metallic);
+    out[2] = st4(lerp3(spl(0.04f), m.albedo, m.metallic));  // f0 (src/shade.rs:36-38)
+    out[3] = st4(m.albedo * (1.0f - m.metallic));          // kd (src/shade.rs:41-43)
 }
 
 SceneView HostScene::host_view() const {

@@ -89,5 +89,7 @@ bool load_rad(const std::string& src, HostScene& out, std::string& err);
 bool validate_cfg(const HCfg& c, std::string& err);
 // Rebuilds the BVH, bounds and device records from the object lists.
 void finalize_scene(HostScene& s);
+// The four device records of one material (albedo+roughness, emissive+metallic, f0, kd).
+void material_records(const HMaterial& m, float4 out[4]);
 
 }  // namespace ifx

@@ -112,7 +112,7 @@ std::vector<HSurfel> generate_surfels(const HostScene& sc, uint64_t target, int 
             if (close) continue;
             occ[(size_t)(c[0] + dim[0] * (c[1] + dim[1] * c[2]))].push_back((uint32_t)out.size());
             MatV m = load_mat(L, sd.mat);
-            out.push_back(HSurfel{sd.p, sd.n, m.kd, area});
+            out.push_back(HSurfel{sd.p, sd.n, m.kd, area, sd.mat});
         }
     }
     return out;

@@ -341,7 +341,7 @@ IFX_HD float ext_shadow(const SceneView& S, f3 p, f3 n, const ExtGeo& g, uint32_
 // reference's loop would `continue` past this light without adding.
 template <class Stk>
 IFX_HD bool diffuse_term(const SceneView& S, const LightsView& Lv, uint32_t li, f3 p, f3 n, f3 tan, f3 bit, f3 albedo,
-                         f3& term, uint32_t& steps, Stk& stk) {
+                         f3& term, float& g, uint32_t& steps, Stk& stk) {
     if (li < Lv.n_quad) {
         const float4* L = Lv.quad + 7 * li;
         f3 lc = ld3(L[5]);
@@ -357,22 +357,24 @@ IFX_HD bool diffuse_term(const SceneView& S, const LightsView& Lv, uint32_t li, 
         float vis = soft_shadow(S, p + n * 3e-3f, to_light / dist, 0.015f, end, k, s, stk);
         steps += s;
         if (vis <= 1e-4f) return false;
-        term = (albedo * ld3(L[4])) * (ff * vis);
+        g = ff * vis;
+        term = (albedo * ld3(L[4])) * g;
         return true;
     }
     li -= Lv.n_quad;
-    ExtGeo g;
+    ExtGeo geo;
     const float4* L;
     if (li < Lv.n_sph) {  // EXT
         L = Lv.sph + 2 * li;
-        if (!sphere_geo(L, p, n, g)) return false;
+        if (!sphere_geo(L, p, n, geo)) return false;
     } else {  // EXT
         L = Lv.pnt + 2 * (li - Lv.n_sph);
-        if (!point_geo(L, p, n, g)) return false;
+        if (!point_geo(L, p, n, geo)) return false;
     }
-    float vis = ext_shadow(S, p, n, g, steps, stk);
+    float vis = ext_shadow(S, p, n, geo, steps, stk);
     if (vis <= 1e-4f) return false;
-    term = (albedo * ld3(L[1])) * (g.ff * vis);
+    g = geo.ff * vis;
+    term = (albedo * ld3(L[1])) * g;
     return true;
 }
 
@@ -389,7 +391,8 @@ IFX_HD f3 direct_diffuse(const SceneView& S, const LightsView& Lv, f3 p, f3 n, f
     const uint32_t nl = n_lights(Lv);
     for (uint32_t li = 0; li < nl; ++li) {
         f3 t;
-        if (diffuse_term(S, Lv, li, p, n, tan, bit, albedo, t, steps, stk)) out = out + t;
+        float g;
+        if (diffuse_term(S, Lv, li, p, n, tan, bit, albedo, t, g, steps, stk)) out = out + t;
     }
     steps_out = steps;
     return out;
@@ -505,8 +508,97 @@ IFX_HD f3 direct(const SceneView& S, const LightsView& Lv, f3 p, f3 n, f3 v, con
     return out;
 }
 
+// One light's share of direct() split into the factors the backward pass
+// needs: direct() adds (kd * emit) * a + sp * emit for this light (equal in
+// exact arithmetic; the forward keeps the reference's rounding order).
+// Returns false where direct() skips the light.
+template <class Stk>
+IFX_HD bool light_coeff(const SceneView& S, const LightsView& Lv, uint32_t li, f3 p, f3 n, f3 tan, f3 bit, f3 v,
+                        float n_dot_v, float alpha, f3 f0, float& a, f3& sp, f3& emit, Stk& stk) {
+    uint32_t steps = 0;
+    a = 0.0f;
+    sp = spl(0.0f);
+    if (li < Lv.n_quad) {
+        const float4* L = Lv.quad + 7 * li;
+        f3 lc = ld3(L[5]);
+        f3 to_light = lc - p;
+        f3 plane_n = ld3(L[6]);
+        if (dot3(plane_n, -nrm3(to_light)) <= 0.0f) return false;
+        float ff = quad_ff(L, p, tan, bit, n);
+        if (ff <= 1e-6f) return false;
+        float dist = len3(to_light);
+        float radius = L[5].w;
+        float k = clmp(dist / radius, 1.5f, 96.0f);
+        float end = fmx(dist - radius * 1.25f, 0.05f);
+        float vis = soft_shadow(S, p + n * 3e-3f, to_light / dist, 0.015f, end, k, steps, stk);
+        if (vis <= 1e-4f) return false;
+        a = ff * vis;
+        emit = ld3(L[4]);
+        f3 r = (-v) - n * (2.0f * dot3(n, -v));
+        float denom = dot3(r, plane_n);
+        f3 rep = lc;
+        if (fabsf(denom) > 1e-6f) {
+            float t = dot3(ld3(L[0]) - p, plane_n) / denom;
+            if (t > 0.0f) rep = closest_on_quad(L, p + r * t);
+        }
+        f3 l = nrm3(rep - p);
+        float n_dot_l = dot3(n, l);
+        if (n_dot_l <= 0.0f) return true;
+        float rep_dist = fmx(len3(rep - p), 1e-4f);
+        float wide = clmp(alpha + radius / (2.0f * rep_dist), alpha, 1.0f);
+        float energy = (alpha / wide) * (alpha / wide);
+        sp = ggx_lobe(n, v, l, n_dot_v, n_dot_l, wide, energy, f0) * vis;
+        return true;
+    }
+    li -= Lv.n_quad;
+    ExtGeo g;
+    if (li < Lv.n_sph) {  // EXT
+        const float4* L = Lv.sph + 2 * li;
+        if (!sphere_geo(L, p, n, g)) return false;
+        float vis = ext_shadow(S, p, n, g, steps, stk);
+        if (vis <= 1e-4f) return false;
+        a = g.ff * vis;
+        emit = ld3(L[1]);
+        f3 r = (-v) - n * (2.0f * dot3(n, -v));
+        f3 to = ld3(L[0]) - p;
+        f3 ctr = r * dot3(to, r) - to;
+        float lc = len3(ctr);
+        f3 rep_v = lc > 1e-9f ? to + ctr * clmp(L[0].w / lc, 0.0f, 1.0f) : to;
+        f3 l = nrm3(rep_v);
+        float n_dot_l = dot3(n, l);
+        if (n_dot_l <= 0.0f) return true;
+        float rep_dist = fmx(len3(rep_v), 1e-4f);
+        float wide = clmp(alpha + g.radius / (2.0f * rep_dist), alpha, 1.0f);
+        float energy = (alpha / wide) * (alpha / wide);
+        sp = ggx_lobe(n, v, l, n_dot_v, n_dot_l, wide, energy, f0) * vis;
+        return true;
+    }
+    const float4* L = Lv.pnt + 2 * (li - Lv.n_sph);  // EXT
+    if (!point_geo(L, p, n, g)) return false;
+    float vis = ext_shadow(S, p, n, g, steps, stk);
+    if (vis <= 1e-4f) return false;
+    a = g.ff * vis;
+    emit = ld3(L[1]);
+    f3 l = g.ldir;
+    float n_dot_l = dot3(n, l);
+    f3 h = nrm3(l + v);
+    float d = d_ggx(fmx(dot3(n, h), 0.0f), alpha);
+    float vis_s = v_smith(n_dot_v, n_dot_l, alpha);
+    f3 f = fresnel_schlick(fmx(dot3(l, h), 0.0f), f0);
+    sp = f * ((d * vis_s * n_dot_l) * (vis * g.inv_d2));
+    return true;
+}
+
+// Emission of light li (quads, then sphere lights, then point lights).
+IFX_HD f3 light_emit(const LightsView& Lv, uint32_t li) {
+    if (li < Lv.n_quad) return ld3(Lv.quad[7 * li + 4]);
+    li -= Lv.n_quad;
+    if (li < Lv.n_sph) return ld3(Lv.sph[2 * li + 1]);
+    return ld3(Lv.pnt[2 * (li - Lv.n_sph) + 1]);
+}
+
 // src/shade.rs:168-181 (+ EXT visible sphere emitters).
-IFX_HD bool emitter_hit(const LightsView& Lv, f3 ro, f3 rd, float tmax, f3& emit) {
+IFX_HD bool emitter_hit(const LightsView& Lv, f3 ro, f3 rd, float tmax, f3& emit, uint32_t* which = nullptr) {
     bool found = false;
     float bt = 0.0f;
     for (uint32_t li = 0; li < Lv.n_quad; ++li) {
@@ -517,6 +609,7 @@ IFX_HD bool emitter_hit(const LightsView& Lv, f3 ro, f3 rd, float tmax, f3& emit
                 found = true;
                 bt = t;
                 emit = ld3(L[4]);
+                if (which) *which = li;
             }
         }
     }
@@ -533,6 +626,7 @@ IFX_HD bool emitter_hit(const LightsView& Lv, f3 ro, f3 rd, float tmax, f3& emit
             found = true;
             bt = t;
             emit = ld3(L[1]);
+            if (which) *which = Lv.n_quad + li;
         }
     }
     return found;
