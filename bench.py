@@ -11,7 +11,11 @@ counting rule (trace step +1, hit normal +4, shadow step +1) over the whole
 frame; ms_per_step is the frame time.  The default workload is the literal
 product scene (7 SDF objects incl. a drilled CSG bowl, 2 sphere lights,
 6 Lambertian materials, 640x480, 20000 surfels requested); `--also stress`
-adds the 1920x1080 seeded stress scene as a secondary measurement.
+adds the 1920x1080 seeded stress scene as a secondary measurement, and
+`--also diff` the gradient workload: grads/s of the implicit adjoint
+(forward solve + beauty shade + backward to 24 albedo and 3 emission
+parameters) on the 4.4-unit Cornell room with 8 materials and 3 quad lights,
+50000 surfels, 512x512, geometry frozen.
 """
 import argparse
 import json
@@ -174,18 +178,20 @@ def measure(name, steps, warmup, with_cpu):
             ktime[k] = {"launches": n, "avg_ms": ms / n, "total_ms_per_frame": ms / steps}
     dominant = max(ktime, key=lambda k: ktime[k]["total_ms_per_frame"])
 
-    # Jacobi roofline: one cooperative launch runs every bounce.  Algorithmic
-    # bytes per bounce 8L + 84N + 24C: idx+w per link; starts, direct,
-    # albedo, rad in/out and irr out per row plus member index, rad and area
-    # for the cluster sums; Bc written and read once per cluster.
+    # Jacobi roofline over one solve (two launches per bounce, timed as
+    # groups of 8 bounces).  Algorithmic bytes per bounce 8L + 84N + 24C:
+    # idx+w per link; starts, direct, albedo, rad in/out and irr out per row
+    # plus member index, rad and area for the cluster sums; Bc written and
+    # read once per cluster.
     N, L, C = placed, t["links"], t["clusters"]
     jac_bytes = (8 * L + 84 * N + 24 * C) * s["iterations"]
     jk = ktime.get("jacobi")
     jac_roof = None
     if jk:
-        gbs = jac_bytes / (jk["avg_ms"] * 1e-3) / 1e9
+        gbs = jac_bytes / (jk["total_ms_per_frame"] * 1e-3) / 1e9
         jac_roof = {"kernel": "jacobi", "bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": gbs / PEAK_HBM_GBS, "traffic": None, "bytes_per_launch": jac_bytes}
+                    "frac": gbs / PEAK_HBM_GBS, "traffic": None, "bytes_per_solve": jac_bytes,
+                    "ms_per_solve": jk["total_ms_per_frame"], "bounces": s["iterations"]}
     roof = None
     if dominant in TRACE_KERNELS and ops.get(dominant):
         tfl = ops[dominant] / (ktime[dominant]["avg_ms"] * 1e-3) / 1e12
@@ -225,6 +231,99 @@ def measure(name, steps, warmup, with_cpu):
     return out
 
 
+def measure_grads(steps, warmup, with_cpu):
+    """Gradient steps of the frozen-geometry adjoint (one step = solve to
+    tolerance 1e-4 + beauty shade + dL/dfilm on the host + backward)."""
+    import numpy as np
+
+    import interflect_amd as ia
+    from interflect_amd import scenes
+
+    surfels, side, res = 50000, 512, 8
+    note("diff", "target render (seed 7)")
+    tgt = ia.Context(0)
+    tgt.load_scene(scenes.diff_cornell(7, surfels=surfels, width=side, height=side))
+    tgt.generate_surfels()
+    tgt.transfer(res)
+    tgt.solve(64)
+    target = tgt.shade("beauty", rgb8=False)["film"]
+    del tgt
+    ctx = ia.Context(0)
+    text = scenes.diff_cornell(8, surfels=surfels, width=side, height=side)
+    info = ctx.load_scene(text)
+    placed = ctx.generate_surfels()
+    t = ctx.transfer(res)
+    note("diff", "%d surfels, %d links; warm-up" % (placed, t["links"]))
+
+    def step():
+        s = ctx.solve(64)
+        f = ctx.shade("beauty", rgb8=False)["film"]
+        u = 2.0 * (f - target)
+        g = ctx.backward(u)
+        return s, g, float(np.sum((f.astype(np.float64) - target) ** 2))
+
+    for _ in range(warmup):
+        step()
+    ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s, g, loss = step()
+        note("diff", "gradient step done (loss %.4g, %d forward bounces, %d adjoint sweeps)" % (
+            loss, s["iterations"], g["iterations"]))
+    wall = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    ktime = {}
+    for k in ("direct_e", "direct_sum", "jacobi", "grid_keys", "shade", "shade_bwd", "adjoint", "adj_params"):
+        n, ms = ctx.kernel_time(k)
+        if n:
+            ktime[k] = {"launches": n, "avg_ms": ms / n, "total_ms_per_step": ms / steps}
+    N, L = placed, t["links"]
+    out = {"value": steps / wall, "unit": "grads/s", "ms_per_step": wall * 1e3 / steps,
+           "config": {"workload": "4.4-unit Cornell room, 8 materials (albedo U[.1,.9]^3, 24 params), 3 quad "
+                                  "lights (scalar emission U[5,20], 3 params), start seed 8, target seed 7, "
+                                  "50000 surfels requested, 512x512, cluster res 8, loss sum (film - target)^2 "
+                                  "on the linear film",
+                      "surfels_placed": N, "links": L, "clusters": t["clusters"],
+                      "forward_bounces": s["iterations"], "adjoint_sweeps": g["iterations"]},
+           "kernels": ktime}
+    ak = ktime.get("adjoint")
+    if ak:
+        # Algorithmic bytes per sweep: row+weight per transposed link (8L);
+        # per surfel cluster id, scale, ubar, rho, mu in/out, v out and one
+        # read of v (80N); per segment its range and partial out/in (32).
+        nseg = (L + 1023) // 1024 + t["clusters"]
+        sweeps = max(1, g["iterations"])
+        byt = (8 * L + 80 * N + 32 * nseg) * sweeps
+        gbs = byt / (ak["total_ms_per_step"] * 1e-3) / 1e9
+        out["roofline_adjoint"] = {"kernel": "adjoint", "bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS,
+                                   "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS, "traffic": None,
+                                   "bytes_per_backward": byt, "ms_per_backward": ak["total_ms_per_step"],
+                                   "sweeps": sweeps}
+    if with_cpu:
+        # The reference has no backward pass: its gradient is 2*27 + 1
+        # forward evaluations (central differences).  One oracle forward
+        # (light_surfels + solve + beauty render) on the same surfels is timed.
+        from oracle import pyoracle as po
+
+        threads = max(1, min(16, os.cpu_count() or 1))
+        note("diff", "CPU forward on %d threads" % threads)
+        sc = po.Scene(text, threads=threads)
+        sc.set_surfels(*ctx.surfels())
+        sc.transfer(res)
+        t1 = time.perf_counter()
+        sc.solve(64)
+        sc.render("beauty")
+        fwd = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": 1.0 / (fwd * 55), "unit": "grads/s", "cores": threads, "kind": "port",
+                               "sample": "one oracle forward (light_surfels + solve + beauty render, %.2f s on %d "
+                                         "threads of %s) x 55 evaluations of central differences over 27 "
+                                         "parameters" % (fwd, threads, cpu_model()),
+                               "forward_ms": fwd * 1e3}
+        out["cpu_baseline"]["speedup"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,7 +356,10 @@ def main():
     }
     line.update({k: v for k, v in m.items() if k not in ("value", "ms_per_step")})
     for extra in [x for x in args.also.split(",") if x]:
-        line.setdefault("secondary", {})[extra] = measure(extra, args.also_steps, 1, not args.no_cpu)
+        if extra == "diff":
+            line.setdefault("secondary", {})[extra] = measure_grads(args.also_steps, 1, not args.no_cpu)
+        else:
+            line.setdefault("secondary", {})[extra] = measure(extra, args.also_steps, 1, not args.no_cpu)
     print(json.dumps(line))
 
 

@@ -143,7 +143,7 @@ ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
  *   "lds"           1 = stage scene records in LDS when they fit (default), 0 = global
  *   "vis_blocks"    grid of the link-visibility kernel (0 = one lane per candidate)
  *   "ff_blocks"     grid of the candidate-cull kernels (0 = automatic)
- *   "jacobi_blocks" grid of the cooperative Jacobi kernel (0 = automatic)
+ *   "jacobi_blocks" grid of the Jacobi and adjoint sweep kernels (0 = automatic)
  *   "jacobi_rows"   0 = automatic (lane per row), 1 = one lane per row, 2 = one wavefront per row */
 ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value);
 ifx_status ifx_kernel_ops(ifx_ctx* ctx, const char* name, uint64_t* ops);

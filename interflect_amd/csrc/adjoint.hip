@@ -258,61 +258,60 @@ struct AdjArgs {
     float4* v;                 // [N] ubar + rho (.) mu
     float4* segsum;            // [nseg] partial sums
     uint32_t* dmax;            // [2*max_iters] max |delta mu| and max |mu| bits per sweep
-    uint32_t* sync;            // [0] barrier arrivals, [1] fault flag
 };
 
-// mu_{k+1} = P^T W^T (ubar + rho (.) mu_k), mu_0 = 0, as one cooperative
-// persistent launch.  Per sweep: each surfel sums its cluster's segment
-// partials in segment order (t_c = (W^T v)_c), forms mu and v, and folds
-// max |delta| / max |mu| into integer maxima; a grid barrier; every lane
-// reads the same maxima and leaves together once max |delta| <= tol max |mu|;
-// otherwise one wavefront per segment sums w * v[row] over its entries
-// (lane-strided, then a fixed xor tree); a grid barrier.
-__global__ __launch_bounds__(kBlock) void k_adjoint(AdjArgs A) {
-    const uint32_t lane = lane_id();
+// Sweep `it` has converged once max |delta mu| <= tol * max |mu| (sweep 0,
+// from mu = 0, never counts).  A sweep that did not run left both maxima
+// zero, which reads as converged, so later launches fall through.
+__device__ __forceinline__ bool adj_converged(const AdjArgs& A, uint32_t it) {
+    return it > 0 && bitsf(A.dmax[2 * it]) <= A.tol * bitsf(A.dmax[2 * it + 1]);
+}
+
+// mu_{k+1} = P^T W^T (ubar + rho (.) mu_k), mu_0 = 0, two launches per sweep.
+// Sweep part 1: each surfel sums its cluster's segment partials in segment
+// order (t_c = (W^T v)_c), forms mu and v, and folds max |delta| and max |mu|
+// into integer maxima.
+__global__ __launch_bounds__(kBlock) void k_adj_surfels(AdjArgs A, uint32_t it) {
+    if (it > 0 && adj_converged(A, it - 1)) return;
     const uint32_t gt = blockIdx.x * kBlock + threadIdx.x, nt = gridDim.x * kBlock;
-    const uint32_t gw = gt >> 6, nw = nt >> 6;
-    uint32_t target = 0;
-    for (uint32_t it = 0; it < A.max_iters; ++it) {
-        uint32_t dm = 0, mm = 0;
-        for (uint32_t i = gt; i < A.N; i += nt) {
-            const uint32_t c = A.scl[i];
-            f3 t = spl(0.0f);
-            if (it > 0 && c < A.C)
-                for (uint32_t s = A.segfirst[c], e = A.segfirst[c + 1]; s < e; ++s) t = t + ld3(A.segsum[s]);
-            const f3 m = t * A.sscale[i];
-            const f3 old = it > 0 ? ld3(A.mu[i]) : spl(0.0f);
-            const uint32_t d = fbits(maxe(abs3(m - old))), a = fbits(maxe(abs3(m)));
-            dm = d > dm ? d : dm;
-            mm = a > mm ? a : mm;
-            A.mu[i] = st4(m);
-            A.v[i] = st4(ld3(A.ubar[i]) + ld3(A.salb[i]) * m);
+    uint32_t dm = 0, mm = 0;
+    for (uint32_t i = gt; i < A.N; i += nt) {
+        const uint32_t c = A.scl[i];
+        f3 t = spl(0.0f);
+        if (it > 0 && c < A.C)
+            for (uint32_t s = A.segfirst[c], e = A.segfirst[c + 1]; s < e; ++s) t = t + ld3(A.segsum[s]);
+        const f3 m = t * A.sscale[i];
+        const f3 old = it > 0 ? ld3(A.mu[i]) : spl(0.0f);
+        const uint32_t d = fbits(maxe(abs3(m - old))), a = fbits(maxe(abs3(m)));
+        dm = d > dm ? d : dm;
+        mm = a > mm ? a : mm;
+        A.mu[i] = st4(m);
+        A.v[i] = st4(ld3(A.ubar[i]) + ld3(A.salb[i]) * m);
+    }
+    dm = wave_max(dm);
+    mm = wave_max(mm);
+    if (lane_id() == 0) {
+        if (dm) atomicMax(&A.dmax[2 * it], dm);
+        if (mm) atomicMax(&A.dmax[2 * it + 1], mm);
+    }
+}
+
+// Sweep part 2: one wavefront per segment sums w * v[row] over its entries,
+// lane-strided, then a fixed xor tree.
+__global__ __launch_bounds__(kBlock) void k_adj_segments(AdjArgs A, uint32_t it) {
+    if (adj_converged(A, it)) return;
+    const uint32_t lane = lane_id();
+    const uint32_t gw = (blockIdx.x * kBlock + threadIdx.x) >> 6, nw = (gridDim.x * kBlock) >> 6;
+    for (uint32_t s = gw; s < A.nseg; s += nw) {
+        const uint2 r = A.seg[s];
+        f3 acc = spl(0.0f);
+        for (uint32_t k = r.x + lane; k < r.y; k += 64u) acc = acc + ld3(A.v[A.crow[k]]) * A.cw[k];
+        for (int o = 32; o > 0; o >>= 1) {
+            acc.x += __shfl_xor(acc.x, o, 64);
+            acc.y += __shfl_xor(acc.y, o, 64);
+            acc.z += __shfl_xor(acc.z, o, 64);
         }
-        dm = wave_max(dm);
-        mm = wave_max(mm);
-        if (lane == 0) {
-            if (dm) atomicMax(&A.dmax[2 * it], dm);
-            if (mm) atomicMax(&A.dmax[2 * it + 1], mm);
-        }
-        grid_barrier(A.sync, target);
-        if (it > 0) {
-            float d = bitsf(__hip_atomic_load(&A.dmax[2 * it], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            float m = bitsf(__hip_atomic_load(&A.dmax[2 * it + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            if (d <= A.tol * m) break;
-        }
-        if (it + 1 == A.max_iters) break;
-        for (uint32_t s = gw; s < A.nseg; s += nw) {
-            const uint2 r = A.seg[s];
-            f3 acc = spl(0.0f);
-            for (uint32_t k = r.x + lane; k < r.y; k += 64u) acc = acc + ld3(A.v[A.crow[k]]) * A.cw[k];
-            for (int o = 32; o > 0; o >>= 1) {
-                acc.x += __shfl_xor(acc.x, o, 64);
-                acc.y += __shfl_xor(acc.y, o, 64);
-                acc.z += __shfl_xor(acc.z, o, 64);
-            }
-            if (lane == 0) A.segsum[s] = st4(acc);
-        }
-        grid_barrier(A.sync, target);
+        if (lane == 0) A.segsum[s] = st4(acc);
     }
 }
 
@@ -537,9 +536,7 @@ int Engine::adjoint_solve(BackwardOut& out, std::string& err) {
     CK(d_mu.ensure(16ull * N));
     CK(d_v.ensure(16ull * N));
     CK(d_dmax.ensure(8ull * iters));
-    CK(d_sync.ensure(16));
     CK(hipMemsetAsync(d_dmax.p, 0, 8ull * iters, stream));
-    CK(hipMemsetAsync(d_sync.p, 0, 16, stream));
     out = BackwardOut();
     if (N == 0) return 0;
     AdjArgs A;
@@ -560,33 +557,33 @@ int Engine::adjoint_solve(BackwardOut& out, std::string& err) {
     A.v = d_v.as<float4>();
     A.segsum = d_segsum.as<float4>();
     A.dmax = d_dmax.as<uint32_t>();
-    A.sync = d_sync.as<uint32_t>();
-    int per_cu = 0;
-    CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_adjoint, kBlock, 0));
-    const uint32_t want = std::max<uint32_t>(blocks_for(N, kBlock), blocks_for(64ull * adj_nseg_, kBlock));
-    const uint32_t cap = (uint32_t)std::max(1, per_cu) * (uint32_t)n_cu_;
-    const uint32_t grid = std::max<uint32_t>(1u, std::min(jacobi_blocks ? jacobi_blocks : want, cap));
-    void* args[] = {&A};
-    hipError_t le = hipSuccess;
-    launch("adjoint", [&] {
-        le = hipLaunchCooperativeKernel((const void*)k_adjoint, dim3(grid), dim3(kBlock), args, 0, stream);
-    });
-    CK(le);
-    std::vector<uint32_t> dm(2ull * iters);
-    uint32_t sync[4] = {0};
-    CK(hipMemcpyAsync(dm.data(), d_dmax.p, 8ull * iters, hipMemcpyDeviceToHost, stream));
-    CK(hipMemcpyAsync(sync, d_sync.p, 16, hipMemcpyDeviceToHost, stream));
-    CK(hipStreamSynchronize(stream));
-    if (sync[1]) {
-        err = "adjoint grid barrier timed out";
-        return -4;
-    }
+    const uint32_t g_s = jacobi_blocks ? jacobi_blocks : blocks_for(N, kBlock);
+    const uint32_t g_seg = jacobi_blocks ? jacobi_blocks : std::max<uint32_t>(1u, blocks_for(64ull * adj_nseg_, kBlock));
+    // Sweeps are enqueued in groups; between groups the host reads the
+    // maxima and stops enqueuing once the adjoint has converged.
+    std::vector<uint32_t> dm(2ull * iters, 0u);
+    const uint32_t group = 8;
     out.iterations = iters;
-    for (uint32_t it = 1; it < iters; ++it) {
-        if (bitsf(dm[2 * it]) <= adj_tol * bitsf(dm[2 * it + 1])) {
-            out.iterations = it;
-            break;
+    for (uint32_t it0 = 0; it0 < iters; it0 += group) {
+        const uint32_t it1 = std::min(iters, it0 + group);
+        launch("adjoint", [&] {
+            for (uint32_t it = it0; it < it1; ++it) {
+                k_adj_surfels<<<g_s, kBlock, 0, stream>>>(A, it);
+                if (it + 1 < iters) k_adj_segments<<<g_seg, kBlock, 0, stream>>>(A, it);
+            }
+        });
+        CK(hipMemcpyAsync(dm.data() + 2ull * it0, A.dmax + 2ull * it0, 8ull * (it1 - it0), hipMemcpyDeviceToHost,
+                          stream));
+        CK(hipStreamSynchronize(stream));
+        bool done = false;
+        for (uint32_t it = std::max(it0, 1u); it < it1; ++it) {
+            if (bitsf(dm[2 * it]) <= adj_tol * bitsf(dm[2 * it + 1])) {
+                out.iterations = it;
+                done = true;
+                break;
+            }
         }
+        if (done) break;
     }
     const uint32_t last = out.iterations < iters ? out.iterations : iters - 1;
     const float mmax = bitsf(dm[2 * last + 1]);
@@ -704,12 +701,12 @@ int Engine::backward_surfels(const float* dL_dirr, float* d_albedo, float* d_emi
     CK(d_ubar.ensure(16ull * N));
     if (N) {
         CK(hipMemsetAsync(d_ubar.p, 0, 16ull * N, stream));
-        CK(hipMemcpy2DAsync(d_ubar.p, 16, dL_dirr, 12, 12, N, hipMemcpyHostToDevice, stream));
+        if (int rc = copy3_h2d(d_ubar.p, dL_dirr, N, err)) return rc;
     }
     if (int rc = adjoint_solve(out, err)) return rc;
     if (int rc = adjoint_params(false, 0, d_albedo, d_emit, err)) return rc;
     if (mu && N) {
-        CK(hipMemcpy2DAsync(mu, 12, d_mu.p, 16, 12, N, hipMemcpyDeviceToHost, stream));
+        if (int rc = copy3_d2h(mu, d_mu.p, N, err)) return rc;
         CK(hipStreamSynchronize(stream));
     }
     return 0;

@@ -147,6 +147,8 @@ class Engine {
     int adjoint_solve(BackwardOut& out, std::string& err);
     int adjoint_params(bool pixels, uint64_t npx, float* d_albedo, float* d_emit, std::string& err);
     int check_backward(std::string& err) const;
+    int copy3_d2h(float* host, const void* dev, uint64_t n, std::string& err);
+    int copy3_h2d(void* dev, const float* host, uint64_t n, std::string& err);
     void upload_materials();
     void upload_lights();
     int check(hipError_t e, const char* what, std::string& err);
@@ -184,7 +186,7 @@ class Engine {
     // candidates and links
     DevBuf d_ccount, d_cstart, d_cand_ci, d_cand_row, d_cand_v, d_sky, d_lcount, d_lstart, d_scale, d_rowsum, d_idx, d_w;
     // solve
-    DevBuf d_direct, d_irr, d_rad, d_bc, d_resid, d_terms, d_gterm, d_sync;
+    DevBuf d_direct, d_irr, d_rad, d_bc, d_resid, d_terms, d_gterm;
     // shade
     DevBuf d_gkey, d_gkey2, d_gval, d_gval2, d_gstart, d_film, d_rgb8;
     // misc
@@ -193,7 +195,7 @@ class Engine {
     DevBuf d_smat, d_scl, d_sscale, d_lrow, d_perm, d_perm2, d_ckeyt, d_ckeyt2, d_crow, d_cw, d_colcnt, d_colstart,
         d_segcnt, d_segfirst, d_seg, d_segsum, d_ubar, d_uacc, d_umax, d_mu, d_v, d_dmax, d_pmat, d_pdr, d_pde, d_sdr,
         d_sde, d_partial;
-    DevBuf d_ufilm;
+    DevBuf d_ufilm, d_stage;
     uint64_t transfer_gen_ = 0, adj_gen_ = ~0ull;
     uint32_t adj_nseg_ = 0;
     bool have_beauty_ = false, have_gterm_ = false;

@@ -7,8 +7,8 @@
 // Stage map (reference):
 //   transfer: build_clusters  src/formfactor.rs:91-147   -> k_cluster_*
 //             build           src/formfactor.rs:153-277  -> k_ff<>, k_visibility, k_closure, k_row_sum, k_compact
-//   solve:    light_surfels   src/solve.rs:27-39         -> k_direct_e
-//             solve           src/solve.rs:47-108        -> k_aggregate, k_spmv
+//   solve:    light_surfels   src/solve.rs:27-39         -> k_direct_terms, k_direct_sum
+//             solve           src/solve.rs:47-108        -> k_jacobi_agg, k_jacobi_rows
 //   shade:    Grid::build     src/surfel.rs:71-114       -> k_grid_keys + radix sort
 //             render_frame    src/main.rs:229-367        -> k_shade
 //             Film::encode    src/film.rs:51-61          -> k_encode
@@ -463,15 +463,17 @@ __global__ void k_direct_sum(uint32_t N, uint32_t nl, const float4* __restrict__
     irr[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// Jacobi bounces B = E + rho * W (G B) as one cooperative persistent launch
-// (src/solve.rs:60-102).  Per bounce: cluster radiosity G B, one wavefront
-// per cluster summing its members strictly in member order; a grid barrier;
-// the row gather in link order with the max-norm update folded into an
-// integer max of non-negative float bits; a grid barrier; then every lane
-// reads the same residual and leaves together once it is below 1e-4.
+// Jacobi bounces B = E + rho * W (G B) (src/solve.rs:60-102), two launches
+// per bounce: the cluster radiosity G B, one wavefront per cluster summing
+// its members strictly in member order; then the row gather in link order
+// with the max-norm update folded into an integer max of non-negative float
+// bits.  Kernel boundaries order the phases (on this multi-XCD part a grid
+// barrier would need an L2 write-back and invalidate per block).  Bounce
+// `it` runs only while every earlier bounce's update was >= tol, so the
+// host may enqueue bounces ahead of the convergence test.
 struct JacobiArgs {
-    uint32_t N, C, bounces, wave_rows;
-    float tol;  // leave once the max-norm update is below tol (the reference's 1e-4)
+    uint32_t N, C, wave_rows;
+    float tol;  // stop once the max-norm update is below tol (the reference's 1e-4)
     const uint2* range;
     const uint32_t* members;
     const float4* spos;
@@ -485,68 +487,63 @@ struct JacobiArgs {
     float4* irr;
     float4* bc;
     uint32_t* resid;  // [bounces] max |delta| bits
-    uint32_t* sync;   // [0] barrier arrivals, [1] fault flag
 };
 
-__global__ __launch_bounds__(kBlock) void k_jacobi(JacobiArgs A) {
+__device__ __forceinline__ bool jacobi_done(const JacobiArgs& A, uint32_t it) {
+    return it > 0 && bitsf(A.resid[it - 1]) < A.tol;
+}
+
+__global__ __launch_bounds__(kBlock) void k_jacobi_agg(JacobiArgs A, uint32_t it) {
+    if (jacobi_done(A, it)) return;
+    const uint32_t lane = lane_id();
+    const uint32_t gw = (blockIdx.x * kBlock + threadIdx.x) >> 6, nw = (gridDim.x * kBlock) >> 6;
+    for (uint32_t c = gw; c < A.C; c += nw) {
+        uint2 r = A.range[c];
+        f3 acc = spl(0.0f);
+        for (uint32_t b = r.x; b < r.y; b += 64u) {
+            uint32_t k = b + lane;
+            f3 v = spl(0.0f);
+            if (k < r.y) {
+                uint32_t m = A.members[k];
+                v = ld3(A.rad[m]) * A.spos[m].w;
+            }
+            uint32_t cnt = r.y - b < 64u ? r.y - b : 64u;
+            for (uint32_t j = 0; j < cnt; ++j) {
+                f3 vj = mk(bitsf(__builtin_amdgcn_readlane(fbits(v.x), j)),
+                           bitsf(__builtin_amdgcn_readlane(fbits(v.y), j)),
+                           bitsf(__builtin_amdgcn_readlane(fbits(v.z), j)));
+                acc = acc + vj;
+            }
+        }
+        if (lane == 0) A.bc[c] = st4(acc / fmx(A.cpos[c].w, 1e-12f));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_jacobi_rows(JacobiArgs A, uint32_t it) {
+    if (jacobi_done(A, it)) return;
     const uint32_t lane = lane_id();
     const uint32_t gt = blockIdx.x * kBlock + threadIdx.x, nt = gridDim.x * kBlock;
     const uint32_t gw = gt >> 6, nw = nt >> 6;
-    uint32_t target = 0;
-    for (uint32_t it = 0; it < A.bounces; ++it) {
-        for (uint32_t c = gw; c < A.C; c += nw) {
-            uint2 r = A.range[c];
-            f3 acc = spl(0.0f);
-            for (uint32_t b = r.x; b < r.y; b += 64u) {
+    uint32_t delta = 0;
+    if (A.wave_rows) {
+        // One wavefront per row: coalesced link loads, the products summed
+        // in link order through readlane.
+        for (uint32_t i = gw; i < A.N; i += nw) {
+            const uint32_t s0 = A.lstart[i], e = A.lstart[i + 1];
+            f3 g = spl(0.0f);
+            for (uint32_t b = s0; b < e; b += 64u) {
                 uint32_t k = b + lane;
-                f3 v = spl(0.0f);
-                if (k < r.y) {
-                    uint32_t m = A.members[k];
-                    v = ld3(A.rad[m]) * A.spos[m].w;
-                }
-                uint32_t cnt = r.y - b < 64u ? r.y - b : 64u;
+                f3 term = spl(0.0f);
+                if (k < e) term = ld3(A.bc[A.idx[k]]) * A.w[k];
+                uint32_t cnt = e - b < 64u ? e - b : 64u;
                 for (uint32_t j = 0; j < cnt; ++j) {
-                    f3 vj = mk(bitsf(__builtin_amdgcn_readlane(fbits(v.x), j)),
-                               bitsf(__builtin_amdgcn_readlane(fbits(v.y), j)),
-                               bitsf(__builtin_amdgcn_readlane(fbits(v.z), j)));
-                    acc = acc + vj;
+                    f3 tj = mk(bitsf(__builtin_amdgcn_readlane(fbits(term.x), j)),
+                               bitsf(__builtin_amdgcn_readlane(fbits(term.y), j)),
+                               bitsf(__builtin_amdgcn_readlane(fbits(term.z), j)));
+                    g = g + tj;
                 }
             }
-            if (lane == 0) A.bc[c] = st4(acc / fmx(A.cpos[c].w, 1e-12f));
-        }
-        grid_barrier(A.sync, target);
-        uint32_t delta = 0;
-        if (A.wave_rows) {
-            // Long rows: one wavefront per row, coalesced link loads, the
-            // products summed in link order through readlane.
-            for (uint32_t i = gw; i < A.N; i += nw) {
-                const uint32_t s0 = A.lstart[i], e = A.lstart[i + 1];
-                f3 g = spl(0.0f);
-                for (uint32_t b = s0; b < e; b += 64u) {
-                    uint32_t k = b + lane;
-                    f3 term = spl(0.0f);
-                    if (k < e) term = ld3(A.bc[A.idx[k]]) * A.w[k];
-                    uint32_t cnt = e - b < 64u ? e - b : 64u;
-                    for (uint32_t j = 0; j < cnt; ++j) {
-                        f3 tj = mk(bitsf(__builtin_amdgcn_readlane(fbits(term.x), j)),
-                                   bitsf(__builtin_amdgcn_readlane(fbits(term.y), j)),
-                                   bitsf(__builtin_amdgcn_readlane(fbits(term.z), j)));
-                        g = g + tj;
-                    }
-                }
-                if (lane == 0) {
-                    f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
-                    uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
-                    delta = d > delta ? d : delta;
-                    A.irr[i] = st4(g);
-                    A.rad[i] = st4(nr);
-                }
-            }
-        } else {
-            for (uint32_t i = gt; i < A.N; i += nt) {
-                f3 g = spl(0.0f);
-                uint32_t e = A.lstart[i + 1];
-                for (uint32_t k = A.lstart[i]; k < e; ++k) g = g + ld3(A.bc[A.idx[k]]) * A.w[k];
+            if (lane == 0) {
                 f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
                 uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
                 delta = d > delta ? d : delta;
@@ -554,11 +551,20 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(JacobiArgs A) {
                 A.rad[i] = st4(nr);
             }
         }
-        uint32_t m = wave_max(delta);
-        if (lane == 0 && m) atomicMax(&A.resid[it], m);
-        grid_barrier(A.sync, target);
-        if (bitsf(__hip_atomic_load(&A.resid[it], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < A.tol) break;
+    } else {
+        for (uint32_t i = gt; i < A.N; i += nt) {
+            f3 g = spl(0.0f);
+            uint32_t e = A.lstart[i + 1];
+            for (uint32_t k = A.lstart[i]; k < e; ++k) g = g + ld3(A.bc[A.idx[k]]) * A.w[k];
+            f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
+            uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
+            delta = d > delta ? d : delta;
+            A.irr[i] = st4(g);
+            A.rad[i] = st4(nr);
+        }
     }
+    uint32_t m = wave_max(delta);
+    if (lane == 0 && m) atomicMax(&A.resid[it], m);
 }
 
 // ====================================================================== shade
@@ -686,6 +692,22 @@ __global__ void k_encode(const float4* __restrict__ film, uint32_t npx, float ex
     out[3 * i] = srgb_lut(x.x, thr);
     out[3 * i + 1] = srgb_lut(x.y, thr);
     out[3 * i + 2] = srgb_lut(x.z, thr);
+}
+
+// Packed xyz triples <-> float4 records, so host copies stay contiguous.
+__global__ void k_pack3(const float4* __restrict__ src, uint64_t n, float* dst) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 v = src[i];
+    dst[3 * i] = v.x;
+    dst[3 * i + 1] = v.y;
+    dst[3 * i + 2] = v.z;
+}
+
+__global__ void k_unpack3(const float* __restrict__ src, uint64_t n, float4* dst) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[i] = make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.0f);
 }
 
 // ====================================================================== host side
@@ -831,6 +853,24 @@ LightsView Engine::dev_lights() const {
     l.pnt = d_pnt.as<float4>();
     l.mat = d_mat.as<float4>();
     return l;
+}
+
+// xyz triples of n float4 records to the host through a packed staging buffer
+// (asynchronous on the stream; the caller synchronizes).
+int Engine::copy3_d2h(float* host, const void* dev, uint64_t n, std::string& err) {
+    if (!n || !host) return 0;
+    CK(d_stage.ensure(12ull * n));
+    k_pack3<<<blocks_for(n, 256), 256, 0, stream>>>((const float4*)dev, n, d_stage.as<float>());
+    CK(hipMemcpyAsync(host, d_stage.p, 12ull * n, hipMemcpyDeviceToHost, stream));
+    return 0;
+}
+
+int Engine::copy3_h2d(void* dev, const float* host, uint64_t n, std::string& err) {
+    if (!n || !host) return 0;
+    CK(d_stage.ensure(12ull * n));
+    CK(hipMemcpyAsync(d_stage.p, host, 12ull * n, hipMemcpyHostToDevice, stream));
+    k_unpack3<<<blocks_for(n, 256), 256, 0, stream>>>(d_stage.as<float>(), n, (float4*)dev);
+    return 0;
 }
 
 int Engine::load_scene(const std::string& src, std::string& err) {
@@ -1187,11 +1227,9 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
     CK(d_resid.ensure(4ull * (bounces + 1)));
     CK(d_terms.ensure(16ull * N * (nl ? nl : 1)));
     CK(d_gterm.ensure(4ull * N * (nl ? nl : 1)));
-    CK(d_sync.ensure(16));
     unsigned long long* ctr = d_counter.as<unsigned long long>();
     CK(hipMemsetAsync(ctr, 0, 64, stream));
     CK(hipMemsetAsync(d_resid.p, 0, 4ull * (bounces + 1), stream));
-    CK(hipMemsetAsync(d_sync.p, 0, 16, stream));
     const SceneView S = dev_view();
     if (N) {
         const Lds ld = plan_lds(true);
@@ -1213,9 +1251,7 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
         JacobiArgs A;
         A.N = N;
         A.C = C;
-        A.bounces = bounces;
         A.tol = solve_tol;
-        // Rows of >= 16 links on average take a wavefront each (tunable).
         A.wave_rows = jacobi_rows == 2 ? 1u : 0u;  // lane per row measured faster at 51 and 237 links/row
         A.range = d_crange.as<uint2>();
         A.members = d_cval2.as<uint32_t>();
@@ -1230,32 +1266,35 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
         A.irr = d_irr.as<float4>();
         A.bc = d_bc.as<float4>();
         A.resid = d_resid.as<uint32_t>();
-        A.sync = d_sync.as<uint32_t>();
-        int per_cu = 0;
-        CK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_jacobi, kBlock, 0));
-        uint32_t want = std::max<uint32_t>(blocks_for(A.wave_rows ? 64ull * N : N, kBlock),
-                                           blocks_for(64ull * C, kBlock));
-        uint32_t cap = (uint32_t)std::max(1, per_cu) * (uint32_t)n_cu_;
-        uint32_t grid = std::max<uint32_t>(1u, std::min(jacobi_blocks ? jacobi_blocks : want, cap));
-        void* args[] = {&A};
-        hipError_t le = hipSuccess;
-        launch("jacobi", [&] {
-            le = hipLaunchCooperativeKernel((const void*)k_jacobi, dim3(grid), dim3(kBlock), args, 0, stream);
-        });
-        CK(le);
+        const uint32_t g_agg = jacobi_blocks ? jacobi_blocks : blocks_for(64ull * C, kBlock);
+        const uint32_t g_rows =
+            jacobi_blocks ? jacobi_blocks : blocks_for(A.wave_rows ? 64ull * N : (uint64_t)N, kBlock);
+        // Bounces are enqueued in groups; between groups the host reads the
+        // last residual and stops enqueuing once the solve has converged.
+        const uint32_t group = 8;
+        uint32_t done_at = 0;
+        for (uint32_t it0 = 0; it0 < bounces && !done_at; it0 += group) {
+            const uint32_t it1 = std::min(bounces, it0 + group);
+            launch("jacobi", [&] {
+                for (uint32_t it = it0; it < it1; ++it) {
+                    k_jacobi_agg<<<g_agg, kBlock, 0, stream>>>(A, it);
+                    k_jacobi_rows<<<g_rows, kBlock, 0, stream>>>(A, it);
+                }
+            });
+            if (it1 < bounces) {
+                uint32_t last = 0;
+                CK(hipMemcpyAsync(&last, A.resid + it1 - 1, 4, hipMemcpyDeviceToHost, stream));
+                CK(hipStreamSynchronize(stream));
+                if (bitsf(last) < solve_tol) done_at = it1;
+            }
+        }
     }
     std::vector<uint32_t> res(bounces + 1, 0);
     unsigned long long cv[8] = {0};
-    uint32_t sync[4] = {0};
     CK(hipMemcpyAsync(res.data(), d_resid.p, 4ull * bounces, hipMemcpyDeviceToHost, stream));
     CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
-    CK(hipMemcpyAsync(sync, d_sync.p, 16, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
     drain_events();
-    if (sync[1]) {
-        err = "jacobi grid barrier timed out";
-        return -4;
-    }
     unsigned long long ev = cv[0];
     if (count_ops_) kops_["direct_e"] += cv[3];
     out.evals = ev;
@@ -1395,7 +1434,7 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
         CK(hipMemcpyAsync(rgb8, d_rgb8.p, 3ull * npx, hipMemcpyDeviceToHost, stream));
     }
     if (film) {
-        CK(hipMemcpy2DAsync(film, 12, d_film.p, 16, 12, npx, hipMemcpyDeviceToHost, stream));
+        if (int rc = copy3_d2h(film, d_film.p, npx, err)) return rc;
     }
     unsigned long long cv[8] = {0};
     CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
@@ -1560,9 +1599,11 @@ int Engine::download_radiosity(float* direct, float* irr, float* rad, std::strin
     }
     const uint64_t N = surfels.size();
     if (!N) return 0;
-    if (direct) CK(hipMemcpy2DAsync(direct, 12, d_direct.p, 16, 12, N, hipMemcpyDeviceToHost, stream));
-    if (irr) CK(hipMemcpy2DAsync(irr, 12, d_irr.p, 16, 12, N, hipMemcpyDeviceToHost, stream));
-    if (rad) CK(hipMemcpy2DAsync(rad, 12, d_rad.p, 16, 12, N, hipMemcpyDeviceToHost, stream));
+    if (int rc = copy3_d2h(direct, d_direct.p, N, err)) return rc;
+    CK(hipStreamSynchronize(stream));
+    if (int rc = copy3_d2h(irr, d_irr.p, N, err)) return rc;
+    CK(hipStreamSynchronize(stream));
+    if (int rc = copy3_d2h(rad, d_rad.p, N, err)) return rc;
     CK(hipStreamSynchronize(stream));
     return 0;
 }
@@ -1577,9 +1618,11 @@ int Engine::upload_radiosity(const float* direct, const float* irr, const float*
         CK(hipMemsetAsync(d_direct.p, 0, 16ull * N, stream));
         CK(hipMemsetAsync(d_irr.p, 0, 16ull * N, stream));
         CK(hipMemsetAsync(d_rad.p, 0, 16ull * N, stream));
-        if (direct) CK(hipMemcpy2DAsync(d_direct.p, 16, direct, 12, 12, N, hipMemcpyHostToDevice, stream));
-        if (irr) CK(hipMemcpy2DAsync(d_irr.p, 16, irr, 12, 12, N, hipMemcpyHostToDevice, stream));
-        if (rad) CK(hipMemcpy2DAsync(d_rad.p, 16, rad, 12, 12, N, hipMemcpyHostToDevice, stream));
+        if (int rc = copy3_h2d(d_direct.p, direct, N, err)) return rc;
+        CK(hipStreamSynchronize(stream));
+        if (int rc = copy3_h2d(d_irr.p, irr, N, err)) return rc;
+        CK(hipStreamSynchronize(stream));
+        if (int rc = copy3_h2d(d_rad.p, rad, N, err)) return rc;
     }
     CK(hipStreamSynchronize(stream));
     have_solve = true;

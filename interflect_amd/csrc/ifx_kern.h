@@ -122,28 +122,6 @@ __device__ __forceinline__ void stk_flush(CountStack& s, unsigned long long* ops
     stk_init(stk);                                                              \
     (void)L
 
-// Grid-wide barrier for a cooperatively launched (co-resident) grid.  The
-// wait is bounded: on timeout the fault flag is raised and the block goes on,
-// so every wave still reaches the kernel's end.
-__device__ __forceinline__ void grid_barrier(uint32_t* sync, uint32_t& target) {
-    __threadfence();
-    __syncthreads();
-    target += gridDim.x;
-    if (threadIdx.x == 0) {
-        atomicAdd(&sync[0], 1u);
-        uint32_t spins = 0;
-        while (__hip_atomic_load(&sync[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (++spins > (1u << 22)) {
-                atomicOr(&sync[1], 1u);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __syncthreads();
-    __threadfence();
-}
-
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
 // Items a wavefront takes per queue atomic: about an eighth of its even
