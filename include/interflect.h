@@ -186,6 +186,41 @@ ifx_status ifx_backward_surfels(ifx_ctx* ctx, const float* dL_dirr, float* dL_da
 ifx_status ifx_get_surfel_materials(ifx_ctx* ctx, uint32_t* mat);
 ifx_status ifx_get_direct_geometry(ifx_ctx* ctx, float* g);
 
+/* ---- Batches: many scenes of uneven size per launch (no counterpart in the
+ * reference, which renders one scene per process).  Every stage runs once
+ * over all scenes with per-scene offsets; the tracing kernels split their
+ * grid into per-scene block ranges.  A frame is transfer + solve (the scene's
+ * render.bounces, tolerance 1e-4) + beauty shade + RGB8 encode of every scene,
+ * with the scene's own camera and film size; per-scene results equal those of
+ * a single-scene context given the same surfels.  With graph mode on, the
+ * first frame after an eager frame is captured into a HIP graph and later
+ * frames replay it. */
+typedef struct ifx_batch ifx_batch;
+typedef struct {
+    uint64_t surfels, candidates, links;
+    uint64_t evals_transfer, evals_solve, evals_shade;
+    uint32_t clusters, iterations, width, height;
+    float residual, mean_row_sum;
+} ifx_batch_scene_stats;
+
+ifx_status ifx_batch_create(ifx_batch** out, int32_t device);
+ifx_status ifx_batch_destroy(ifx_batch* b);
+const char* ifx_batch_last_error(const ifx_batch* b);
+ifx_status ifx_batch_add_scene(ifx_batch* b, const char* rad_text, int32_t cluster_res, uint32_t* index);
+/* surfel::generate for every scene (render.surfels requested), on the host. */
+ifx_status ifx_batch_generate_surfels(ifx_batch* b);
+ifx_status ifx_batch_set_graph(ifx_batch* b, int32_t on);
+/* Launch shapes: persistent blocks per CU, queue chunk divisor, Jacobi row mapping (0 lane, 1 wavefront). */
+ifx_status ifx_batch_set_tuning(ifx_batch* b, uint32_t blocks_per_cu, uint32_t chunk_div, uint32_t wave_rows);
+/* Times one eager frame per candidate launch shape and keeps the fastest. */
+ifx_status ifx_batch_autotune(ifx_batch* b, float* best_ms);
+ifx_status ifx_batch_run(ifx_batch* b, float* ms);
+ifx_status ifx_batch_get_stats(ifx_batch* b, uint32_t scene, ifx_batch_scene_stats* out);
+ifx_status ifx_batch_get_image(ifx_batch* b, uint32_t scene, float* film_rgb, uint8_t* rgb8);
+ifx_status ifx_batch_get_radiosity(ifx_batch* b, uint32_t scene, float* direct, float* irr, float* rad);
+/* CSR of one scene with scene-local cluster indices (as ifx_get_transfer). */
+ifx_status ifx_batch_get_transfer(ifx_batch* b, uint32_t scene, uint32_t* starts, uint32_t* idx, float* w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,6 +38,10 @@ EXPORTS = [
     "ifx_reset_kernel_times", "ifx_set_counting", "ifx_kernel_ops", "ifx_set_tuning",
     "ifx_set_material_albedo", "ifx_set_light_emit", "ifx_set_solve_tolerance", "ifx_set_adjoint", "ifx_backward",
     "ifx_backward_surfels", "ifx_get_surfel_materials", "ifx_get_direct_geometry",
+    "ifx_batch_create", "ifx_batch_destroy", "ifx_batch_last_error", "ifx_batch_add_scene",
+    "ifx_batch_generate_surfels", "ifx_batch_set_graph", "ifx_batch_set_tuning", "ifx_batch_autotune",
+    "ifx_batch_run", "ifx_batch_get_stats", "ifx_batch_get_image", "ifx_batch_get_radiosity",
+    "ifx_batch_get_transfer",
 ]
 
 
@@ -73,6 +77,13 @@ class SolveStats(C.Structure):
 
 class ShadeStats(C.Structure):
     _fields_ = [("evals", C.c_uint64), ("ms", C.c_float), ("reserved", C.c_float)]
+
+
+class BatchSceneStats(C.Structure):
+    _fields_ = [("surfels", C.c_uint64), ("candidates", C.c_uint64), ("links", C.c_uint64),
+                ("evals_transfer", C.c_uint64), ("evals_solve", C.c_uint64), ("evals_shade", C.c_uint64),
+                ("clusters", C.c_uint32), ("iterations", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("residual", C.c_float), ("mean_row_sum", C.c_float)]
 
 
 class BackwardStats(C.Structure):
@@ -124,6 +135,18 @@ def lib():
         "ifx_backward_surfels": [vp, P, P, P, P, C.POINTER(BackwardStats)],
         "ifx_get_surfel_materials": [vp, P],
         "ifx_get_direct_geometry": [vp, P],
+        "ifx_batch_create": [C.POINTER(vp), i32],
+        "ifx_batch_destroy": [vp],
+        "ifx_batch_add_scene": [vp, C.c_char_p, i32, C.POINTER(u32)],
+        "ifx_batch_generate_surfels": [vp],
+        "ifx_batch_set_graph": [vp, i32],
+        "ifx_batch_set_tuning": [vp, u32, u32, u32],
+        "ifx_batch_autotune": [vp, C.POINTER(f32)],
+        "ifx_batch_run": [vp, C.POINTER(f32)],
+        "ifx_batch_get_stats": [vp, u32, C.POINTER(BatchSceneStats)],
+        "ifx_batch_get_image": [vp, u32, P, P],
+        "ifx_batch_get_radiosity": [vp, u32, P, P, P],
+        "ifx_batch_get_transfer": [vp, u32, P, P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -131,6 +154,8 @@ def lib():
         fn.restype = C.c_int32
     L.ifx_last_error.argtypes = [vp]
     L.ifx_last_error.restype = C.c_char_p
+    L.ifx_batch_last_error.argtypes = [vp]
+    L.ifx_batch_last_error.restype = C.c_char_p
     _LIB = L
     return L
 
@@ -398,3 +423,82 @@ class Context:
         g = np.zeros((nl, self.num_surfels()), np.float32)
         self._check(lib().ifx_get_direct_geometry(self._h, _ptr(g)))
         return g
+
+
+class Batch:
+    """Many scenes per launch (include/interflect.h, ifx_batch_*): one frame =
+    transfer + solve + beauty shade + RGB8 of every scene, each with its own
+    render {} settings and camera."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.ifx_batch_create(C.byref(h), device)
+        self._h = h
+        if rc != 0:
+            msg = L.ifx_batch_last_error(h).decode() if h.value else "create failed"
+            if h.value:
+                L.ifx_batch_destroy(h)
+            self._h = C.c_void_p()
+            raise IfxError(rc, msg)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and self._h.value and _LIB is not None:
+            _LIB.ifx_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise IfxError(rc, lib().ifx_batch_last_error(self._h).decode())
+
+    def add_scene(self, text, cluster_res=8):
+        i = C.c_uint32()
+        self._check(lib().ifx_batch_add_scene(self._h, text.encode(), int(cluster_res), C.byref(i)))
+        return int(i.value)
+
+    def generate_surfels(self):
+        self._check(lib().ifx_batch_generate_surfels(self._h))
+
+    def set_graph(self, on):
+        self._check(lib().ifx_batch_set_graph(self._h, 1 if on else 0))
+
+    def set_tuning(self, blocks_per_cu=8, chunk_div=8, wave_rows=0):
+        self._check(lib().ifx_batch_set_tuning(self._h, int(blocks_per_cu), int(chunk_div), int(wave_rows)))
+
+    def autotune(self):
+        ms = C.c_float()
+        self._check(lib().ifx_batch_autotune(self._h, C.byref(ms)))
+        return float(ms.value)
+
+    def run(self):
+        ms = C.c_float()
+        self._check(lib().ifx_batch_run(self._h, C.byref(ms)))
+        return float(ms.value)
+
+    def stats(self, scene):
+        st = BatchSceneStats()
+        self._check(lib().ifx_batch_get_stats(self._h, int(scene), C.byref(st)))
+        return {k: getattr(st, k) for k, _ in BatchSceneStats._fields_}
+
+    def image(self, scene, film=True, rgb8=True):
+        st = self.stats(scene)
+        w, h = st["width"], st["height"]
+        f = np.zeros((h, w, 3), np.float32) if film else None
+        b = np.zeros((h, w, 3), np.uint8) if rgb8 else None
+        self._check(lib().ifx_batch_get_image(self._h, int(scene), _ptr(f), _ptr(b)))
+        return {"film": f, "rgb8": b}
+
+    def radiosity(self, scene):
+        n = self.stats(scene)["surfels"]
+        d, i, r = (np.zeros((n, 3), np.float32) for _ in range(3))
+        self._check(lib().ifx_batch_get_radiosity(self._h, int(scene), _ptr(d), _ptr(i), _ptr(r)))
+        return d, i, r
+
+    def transfer_arrays(self, scene):
+        st = self.stats(scene)
+        n, L = st["surfels"], st["links"]
+        starts = np.zeros(n + 1, np.uint32)
+        idx = np.zeros(max(L, 1), np.uint32)
+        w = np.zeros(max(L, 1), np.float32)
+        self._check(lib().ifx_batch_get_transfer(self._h, int(scene), _ptr(starts), _ptr(idx), _ptr(w)))
+        return starts, idx[:L], w[:L]

@@ -1,0 +1,1254 @@
+// Batched frames over many scenes (see batch.h).  The per-item arithmetic is
+// the single-scene path's (the same device functions of ifx_sdf.h,
+// ifx_shade.h and ifx_kern.h); only the addressing differs: global arrays
+// with per-scene offsets, scene ids in the high bits of every sort key so
+// each scene's segment of a sorted array is its single-scene order, and a
+// per-(scene, bounce) convergence table for the Jacobi solve.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <thread>
+
+#include "batch.h"
+#include "ifx_kern.h"
+#include "ifx_sort.h"
+
+namespace ifx {
+namespace {
+
+// Per-scene device record.
+struct BSlot {
+    SceneView S;
+    LightsView L;
+    LdsPlan Pn, Pl;  // LDS plans without / with lights
+    uint32_t staged_n, staged_l;
+    uint32_t s0, N, cl0, C, nl, bounces, cand0, M, term0, px0, cell0;
+    uint32_t chunk_c, chunk_v;  // queue chunks of closure / visibility
+    float diag, exposure;
+    f3 lo, cinv, mscale;
+    int32_t res;
+    ShadeArgs A;
+};
+
+// Block partition of a grid: scene s owns blocks [blk[s], blk[s+1]).
+__device__ __forceinline__ uint32_t slot_of(const uint32_t* __restrict__ blk, uint32_t ns, uint32_t b) {
+    uint32_t lo = 0, hi = ns;  // blk[lo] <= b < blk[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (blk[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Prologue of a block-partitioned tracing kernel: find the block's scene,
+// stage its records (and lights) in LDS when they fit, place the stack.
+#define IFX_BSETUP(LIGHTS)                                                        \
+    extern __shared__ float4 ifx_lds[];                                          \
+    const uint32_t sid = slot_of(blk, ns, blockIdx.x);                           \
+    const BSlot& T = slots[sid];                                                 \
+    const uint32_t bid = blockIdx.x - blk[sid];                                  \
+    SceneView S = T.S;                                                           \
+    LightsView L = T.L;                                                          \
+    const LdsPlan P = (LIGHTS) ? T.Pl : T.Pn;                                    \
+    if ((LIGHTS) ? T.staged_l : T.staged_n) stage_in<true, LIGHTS>(ifx_lds, S, L); \
+    LdsStack stk;                                                                \
+    stk.dbase = (float*)(ifx_lds + P.scene_f4 + P.light_f4) + threadIdx.x;       \
+    stk.base = (uint16_t*)(stk.dbase - threadIdx.x + P.stack_depth * kBlock) +   \
+               threadIdx.x;                                                      \
+    stk.stride = (uint32_t)kBlock;                                               \
+    (void)L;                                                                     \
+    (void)bid
+
+__global__ void kb_fill(uint32_t* p, uint64_t n, uint32_t v) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------- clusters
+__global__ void kb_cluster_keys(const float4* __restrict__ spos, const float4* __restrict__ snrm,
+                                const uint32_t* __restrict__ ssid, uint32_t n, const BSlot* __restrict__ slots,
+                                uint64_t* keys, uint32_t* vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = ssid[i];
+    const BSlot& T = slots[s];
+    f3 c = (ld3(spos[i]) - T.lo) * T.cinv;
+    const int32_t res = T.res;
+    uint64_t kx = (uint64_t)clampi(sat_i32(c.x), 0, res - 1);
+    uint64_t ky = (uint64_t)clampi(sat_i32(c.y), 0, res - 1);
+    uint64_t kz = (uint64_t)clampi(sat_i32(c.z), 0, res - 1);
+    uint64_t r = (uint64_t)res;
+    keys[i] = ((uint64_t)s << 40) | (((kx * r + ky) * r + kz) * 6ull + normal_bucket(ld3(snrm[i])));
+    vals[i] = i;
+}
+
+__global__ void kb_cluster_scene(uint32_t C, const uint2* __restrict__ range, const uint32_t* __restrict__ members,
+                                 const uint32_t* __restrict__ ssid, uint32_t* csid, uint32_t* count) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    uint32_t s = ssid[members[range[c].x]];
+    csid[c] = s;
+    atomicAdd(&count[s], 1u);
+}
+
+__global__ void kb_morton(const float4* __restrict__ spos, const uint32_t* __restrict__ ssid, uint32_t n,
+                          const BSlot* __restrict__ slots, uint32_t* code, uint32_t* key, uint32_t* iota) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = ssid[i];
+    f3 q = (ld3(spos[i]) - slots[s].lo) * slots[s].mscale;
+    uint32_t x = (uint32_t)clampi(sat_i32(q.x), 0, 31), y = (uint32_t)clampi(sat_i32(q.y), 0, 31),
+             z = (uint32_t)clampi(sat_i32(q.z), 0, 31);
+    uint32_t m = spread5(x) | (spread5(y) << 1) | (spread5(z) << 2);
+    code[i] = m;
+    key[i] = (s << 15) | m;
+    iota[i] = i;
+}
+
+// ---------------------------------------------------------------- candidates
+// One wavefront per surfel row over its own scene's clusters (k_ff).
+template <bool EMIT>
+__global__ __launch_bounds__(kBlock) void kb_ff(const float4* __restrict__ spos, const float4* __restrict__ snrm,
+                                                const uint32_t* __restrict__ ssid, uint32_t N,
+                                                const BSlot* __restrict__ slots, const float4* __restrict__ cpos,
+                                                const float4* __restrict__ cnrm, uint32_t* count,
+                                                const uint32_t* __restrict__ start, uint32_t* cand_ci,
+                                                uint32_t* cand_row, const uint32_t* __restrict__ code,
+                                                uint64_t* key, uint32_t* val) {
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
+    for (uint32_t i = wave; i < N; i += nwaves) {
+        const uint32_t s = ssid[i];
+        const uint32_t c0 = slots[s].cl0, c1 = c0 + slots[s].C;
+        f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
+        uint32_t base = EMIT ? start[i] : 0u;
+        uint64_t mc = EMIT ? (uint64_t)code[i] : 0ull;
+        uint32_t run = 0;
+        for (uint32_t cb = c0; cb < c1; cb += 64) {
+            uint32_t c = cb + lane;
+            bool pass = false;
+            if (c < c1) {
+                float r, ff;
+                f3 dir;
+                pass = ff_geom(sp, sn, cpos[c], cnrm[c], r, dir, ff);
+            }
+            uint64_t m = __ballot(pass);
+            if (EMIT && pass) {
+                uint32_t o = base + run + lanes_below(m);
+                cand_ci[o] = c;
+                cand_row[o] = i;
+                key[o] = ((uint64_t)c << 15) | mc;
+                val[o] = o;
+            }
+            run += (uint32_t)__popcll(m);
+        }
+        if (!EMIT && lane == 0) count[i] = run;
+    }
+}
+
+// ---------------------------------------------------------------- tracing (block-partitioned)
+// Hemisphere closure (k_closure): scene queue, direction-major over the
+// scene's Morton-ordered rows.
+__global__ __launch_bounds__(kBlock) void kb_closure(const BSlot* __restrict__ slots, const uint32_t* __restrict__ blk,
+                                                     uint32_t ns, const float4* __restrict__ spos,
+                                                     const float4* __restrict__ snrm,
+                                                     const uint32_t* __restrict__ rorder,
+                                                     const float4* __restrict__ dirs, uint32_t* open,
+                                                     uint32_t* queues, unsigned long long* evals) {
+    IFX_BSETUP(false);
+    const uint32_t N = T.N, s0 = T.s0;
+    const float tmax = T.diag * 2.0f;
+    uint32_t* queue = queues + 4 * sid + 1;
+    uint32_t acc = 0;
+    bool live = false;
+    WaveQueue wq{0u, 0u, T.chunk_c, false};
+    uint32_t row = 0;
+    TraceMarch m;
+    m.begin(spl(0.0f), spl(0.0f), 0.0f, 0.0f);
+    const uint32_t total = 16u * N;
+    for (;;) {
+        uint32_t q;
+        if (refill(live, wq, queue, total, q)) {
+            uint32_t k = q / N;
+            row = rorder[s0 + (q - k * N)];
+            f3 p = ld3(spos[row]), n = ld3(snrm[row]);
+            f3 t, b;
+            frame(n, t, b);
+            float4 D = dirs[k];
+            f3 d = (t * D.x + b * D.y) + n * D.z;
+            m.begin(p + n * 1e-3f, nrm3(d), 1e-3f, tmax);
+            live = true;
+        }
+        if (__ballot(live) == 0) break;
+        if (live) {
+            DistHit h;
+            int st = m.step(S, h, stk);
+            if (st) {
+                if (st == 2) atomicAdd(&open[row], 1u);
+                acc += m.steps;
+                live = false;
+            }
+        }
+    }
+    count_steps(evals + 4 * sid, acc);
+}
+
+// Link visibility (k_visibility): scene queue over the scene's segment of
+// the cluster-major candidate order.
+__global__ __launch_bounds__(kBlock) void kb_visibility(
+    const BSlot* __restrict__ slots, const uint32_t* __restrict__ blk, uint32_t ns, const float4* __restrict__ spos,
+    const float4* __restrict__ snrm, const float4* __restrict__ cpos, const float4* __restrict__ cnrm,
+    const uint32_t* __restrict__ cand_ci, const uint32_t* __restrict__ cand_row, const uint32_t* __restrict__ order,
+    float* cand_v, uint32_t* queues, unsigned long long* evals) {
+    IFX_BSETUP(false);
+    const uint32_t M = T.M, c0 = T.cand0;
+    const float off = T.diag * 5e-4f, tmin = T.diag * 1e-3f;
+    uint32_t* queue = queues + 4 * sid;
+    uint32_t acc = 0;
+    bool live = false;
+    WaveQueue wq{0u, 0u, T.chunk_v, false};
+    uint32_t slot = 0;
+    float ff = 0.0f;
+    ShadowMarch m;
+    m.begin(spl(0.0f), spl(0.0f), 0.0f, 0.0f, 0.0f);
+    for (;;) {
+        uint32_t q;
+        if (refill(live, wq, queue, M, q)) {
+            slot = order[c0 + q];
+            uint32_t i = cand_row[slot], c = cand_ci[slot];
+            f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
+            float4 CN = cnrm[c];
+            float r = 0.0f;
+            f3 dir = spl(0.0f);
+            ff_geom(sp, sn, cpos[c], CN, r, dir, ff);
+            float kk = clmp(r / fmx(CN.w, 1e-4f), 1.5f, 32.0f);
+            m.begin(sp + sn * off, dir, tmin, r - CN.w * 0.75f, kk);
+            live = true;
+        }
+        if (__ballot(live) == 0) break;
+        if (live) {
+            float vis;
+            if (m.step(S, vis, stk)) {
+                cand_v[slot] = vis <= 1e-3f ? 0.0f : ff * vis;
+                acc += m.steps;
+                live = false;
+            }
+        }
+    }
+    count_steps(evals + 4 * sid, acc);
+}
+
+// Direct light terms (k_direct_terms), one lane per (light, row).
+__global__ __launch_bounds__(kBlock) void kb_direct_terms(const BSlot* __restrict__ slots,
+                                                          const uint32_t* __restrict__ blk, uint32_t ns,
+                                                          const float4* __restrict__ spos,
+                                                          const float4* __restrict__ snrm,
+                                                          const float4* __restrict__ salb,
+                                                          const uint32_t* __restrict__ rorder, float4* terms,
+                                                          float* gterm, unsigned long long* evals) {
+    IFX_BSETUP(true);
+    const uint32_t N = T.N, nl = T.nl;
+    const uint32_t t = bid * kBlock + threadIdx.x;
+    uint32_t steps = 0;
+    if (t < N * nl) {
+        uint32_t li = t / N, i = rorder[T.s0 + (t - li * N)];
+        f3 p = ld3(spos[i]), n = ld3(snrm[i]);
+        f3 tan, bit;
+        frame(n, tan, bit);
+        f3 term = spl(0.0f);
+        float g = 0.0f;
+        bool on = diffuse_term(S, L, li, p, n, tan, bit, ld3(salb[i]), term, g, steps, stk);
+        const size_t o = (size_t)T.term0 + (size_t)li * N + (i - T.s0);
+        terms[o] = on ? st4(term, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        gterm[o] = on ? g : 0.0f;
+    }
+    count_steps(evals + 4 * sid + 1, steps);
+}
+
+// Beauty pixels (k_shade): 16x16 blocks of the scene's film, 8x8 per wavefront.
+__global__ __launch_bounds__(kBlock) void kb_shade(const BSlot* __restrict__ slots, const uint32_t* __restrict__ blk,
+                                                   uint32_t ns, float4* film, unsigned long long* evals) {
+    IFX_BSETUP(true);
+    const ShadeArgs& A = T.A;
+    const uint32_t gx = (A.W + 15u) / 16u;
+    const uint32_t bx = bid % gx, by = bid / gx;
+    const uint32_t wv = threadIdx.x >> 6, l = threadIdx.x & 63u;
+    const uint32_t x = bx * 16u + (wv & 1u) * 8u + (l & 7u);
+    const uint32_t y = by * 16u + (wv >> 1) * 8u + (l >> 3);
+    uint32_t local = 0;
+    if (x < A.W && y < A.H) {
+        f3 c = shade_pixel(S, L, A, x, y, local, stk);
+        film[(size_t)T.px0 + (size_t)y * A.W + x] = st4(c);
+    }
+    count_steps(evals + 4 * sid + 2, local);
+}
+
+__global__ void kb_encode(const BSlot* __restrict__ slots, const uint32_t* __restrict__ blk, uint32_t ns,
+                          const float4* __restrict__ film, const float* __restrict__ thr, uint8_t* out) {
+    const uint32_t sid = slot_of(blk, ns, blockIdx.x);
+    const BSlot& T = slots[sid];
+    const uint32_t i = (blockIdx.x - blk[sid]) * blockDim.x + threadIdx.x;
+    if (i >= T.A.W * T.A.H) return;
+    const size_t p = (size_t)T.px0 + i;
+    encode_px(film[p], T.exposure, 1, thr, out + 3 * p);
+}
+
+// ---------------------------------------------------------------- solve
+__global__ void kb_direct_sum(const uint32_t* __restrict__ ssid, uint32_t N, const BSlot* __restrict__ slots,
+                              const float4* __restrict__ terms, float4* direct, float4* rad, float4* irr) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const BSlot& T = slots[ssid[i]];
+    f3 out = spl(0.0f);
+    for (uint32_t li = 0; li < T.nl; ++li) {
+        float4 t = terms[(size_t)T.term0 + (size_t)li * T.N + (i - T.s0)];
+        if (t.w != 0.0f) out = out + ld3(t);
+    }
+    direct[i] = st4(out);
+    rad[i] = st4(out);
+    irr[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+struct BJacobi {
+    uint32_t N, C, Bmax, wave_rows;
+    float tol;
+    const BSlot* slots;
+    const uint32_t* ssid;
+    const uint32_t* csid;
+    const uint2* range;
+    const uint32_t* members;
+    const float4* spos;
+    const float4* cpos;
+    const uint32_t* lstart;
+    const uint32_t* idx;
+    const float* w;
+    const float4* direct;
+    const float4* salb;
+    float4* rad;
+    float4* irr;
+    float4* bc;
+    uint32_t* resid;  // [scene][Bmax] max |delta| bits
+};
+
+// Scene s runs bounce `it` while it < its bounce count and every earlier
+// bounce's update was >= tol (src/solve.rs:98), exactly as alone.
+__device__ __forceinline__ bool bj_active(const BJacobi& A, uint32_t s, uint32_t it) {
+    if (it >= A.slots[s].bounces) return false;
+    return it == 0 || !(bitsf(A.resid[(size_t)s * A.Bmax + it - 1]) < A.tol);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_jacobi_agg(BJacobi A, uint32_t it) {
+    const uint32_t lane = lane_id();
+    const uint32_t gw = (blockIdx.x * kBlock + threadIdx.x) >> 6, nw = (gridDim.x * kBlock) >> 6;
+    for (uint32_t c = gw; c < A.C; c += nw) {
+        if (!bj_active(A, A.csid[c], it)) continue;
+        uint2 r = A.range[c];
+        f3 acc = spl(0.0f);
+        for (uint32_t b = r.x; b < r.y; b += 64u) {
+            uint32_t k = b + lane;
+            f3 v = spl(0.0f);
+            if (k < r.y) {
+                uint32_t m = A.members[k];
+                v = ld3(A.rad[m]) * A.spos[m].w;
+            }
+            uint32_t cnt = r.y - b < 64u ? r.y - b : 64u;
+            for (uint32_t j = 0; j < cnt; ++j) {
+                f3 vj = mk(bitsf(__builtin_amdgcn_readlane(fbits(v.x), j)),
+                           bitsf(__builtin_amdgcn_readlane(fbits(v.y), j)),
+                           bitsf(__builtin_amdgcn_readlane(fbits(v.z), j)));
+                acc = acc + vj;
+            }
+        }
+        if (lane == 0) A.bc[c] = st4(acc / fmx(A.cpos[c].w, 1e-12f));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_jacobi_rows(BJacobi A, uint32_t it) {
+    const uint32_t lane = lane_id();
+    const uint32_t gt = blockIdx.x * kBlock + threadIdx.x, nt = gridDim.x * kBlock;
+    if (A.wave_rows) {
+        for (uint32_t i = gt >> 6; i < A.N; i += nt >> 6) {
+            const uint32_t s = A.ssid[i];
+            if (!bj_active(A, s, it)) continue;
+            const uint32_t s0 = A.lstart[i], e = A.lstart[i + 1];
+            f3 g = spl(0.0f);
+            for (uint32_t b = s0; b < e; b += 64u) {
+                uint32_t k = b + lane;
+                f3 term = spl(0.0f);
+                if (k < e) term = ld3(A.bc[A.idx[k]]) * A.w[k];
+                uint32_t cnt = e - b < 64u ? e - b : 64u;
+                for (uint32_t j = 0; j < cnt; ++j) {
+                    f3 tj = mk(bitsf(__builtin_amdgcn_readlane(fbits(term.x), j)),
+                               bitsf(__builtin_amdgcn_readlane(fbits(term.y), j)),
+                               bitsf(__builtin_amdgcn_readlane(fbits(term.z), j)));
+                    g = g + tj;
+                }
+            }
+            if (lane == 0) {
+                f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
+                uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
+                A.irr[i] = st4(g);
+                A.rad[i] = st4(nr);
+                if (d) atomicMax(&A.resid[(size_t)s * A.Bmax + it], d);
+            }
+        }
+    } else {
+        for (uint32_t i = gt; i < A.N; i += nt) {
+            const uint32_t s = A.ssid[i];
+            if (!bj_active(A, s, it)) continue;
+            f3 g = spl(0.0f);
+            uint32_t e = A.lstart[i + 1];
+            for (uint32_t k = A.lstart[i]; k < e; ++k) g = g + ld3(A.bc[A.idx[k]]) * A.w[k];
+            f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
+            uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
+            A.irr[i] = st4(g);
+            A.rad[i] = st4(nr);
+            if (d) atomicMax(&A.resid[(size_t)s * A.Bmax + it], d);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- shade grid
+__global__ void kb_grid_keys(const float4* __restrict__ spos, const uint32_t* __restrict__ ssid, uint32_t N,
+                             const BSlot* __restrict__ slots, uint32_t* keys, uint32_t* vals, uint32_t* counts) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const BSlot& T = slots[ssid[i]];
+    const GridView& g = T.A.grid;
+    int32_t c[3];
+    grid_cell(g, ld3(spos[i]), c);
+    uint32_t k = T.cell0 + (uint32_t)(c[0] + g.dim[0] * (c[1] + g.dim[1] * c[2]));
+    keys[i] = k;
+    vals[i] = i;
+    atomicAdd(&counts[k], 1u);
+}
+
+uint32_t persist_chunk(uint64_t items, uint32_t blocks, uint32_t div) {
+    uint64_t waves = (uint64_t)blocks * (kBlock / 64);
+    uint64_t c = items / (waves * div + 1);
+    return (uint32_t)(c < 64 ? 64 : (c > 256 ? 256 : c));
+}
+
+// Splits `total` persistent blocks over the scenes in proportion to their
+// work (each working scene gets at least one block and at most one block per
+// kBlock items); returns the prefix table.
+std::vector<uint32_t> split_blocks(const std::vector<uint64_t>& work, uint32_t total) {
+    uint64_t sum = 0;
+    for (uint64_t w : work) sum += w;
+    std::vector<uint32_t> blk(work.size() + 1, 0);
+    for (size_t s = 0; s < work.size(); ++s) {
+        uint32_t b = 0;
+        if (work[s]) {
+            double share = sum ? (double)total * (double)work[s] / (double)sum : 1.0;
+            b = (uint32_t)std::max<double>(1.0, std::floor(share + 0.5));
+            b = std::min<uint32_t>(b, blocks_for(work[s], kBlock));
+        }
+        blk[s + 1] = blk[s] + b;
+    }
+    return blk;
+}
+
+std::vector<uint32_t> exact_blocks(const std::vector<uint64_t>& work) {
+    std::vector<uint32_t> blk(work.size() + 1, 0);
+    for (size_t s = 0; s < work.size(); ++s) blk[s + 1] = blk[s] + blocks_for(work[s], kBlock);
+    return blk;
+}
+
+}  // namespace
+
+#define BK(call)                                       \
+    do {                                               \
+        int rc_ = check((call), #call, err);           \
+        if (rc_) return rc_;                           \
+    } while (0)
+
+int Batch::check(hipError_t e, const char* what, std::string& err) {
+    if (e == hipSuccess) return 0;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    (void)hipGetLastError();
+    return e == hipErrorOutOfMemory ? -5 : -4;
+}
+
+Batch::~Batch() {
+    drop_graph();
+    if (stream_) (void)hipStreamSynchronize(stream_);
+}
+
+void Batch::drop_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+}
+
+int Batch::init(int device, std::string& err) {
+    if (device < 0) {
+        err = "a batch needs a HIP device";
+        return -1;
+    }
+    if (int rc = eng_.init(device, err)) return rc;
+    stream_ = eng_.stream;
+    return 0;
+}
+
+int Batch::add_scene(const std::string& text, int32_t res, uint32_t& index, std::string& err) {
+    if (res < 1 || res > 4096) {
+        err = "cluster resolution must be in 1..4096 in a batch";
+        return -1;
+    }
+    SceneIn in;
+    if (!load_rad(text, in.sc, err)) return -2;
+    if (in.sc.max_stack > (uint32_t)kStackCap) {
+        err = "BVH needs a traversal stack deeper than the kernels hold";
+        return -6;
+    }
+    if (in.sc.node_rec.size() / 2 > 65535) {
+        err = "scene has more than 65535 BVH nodes";
+        return -6;
+    }
+    if (scenes_.size() >= 4096) {
+        err = "a batch holds at most 4096 scenes";
+        return -6;
+    }
+    in.res = res;
+    index = (uint32_t)scenes_.size();
+    BatchSceneStats st0;
+    st0.width = in.sc.cfg.width;
+    st0.height = in.sc.cfg.height;
+    scenes_.push_back(std::move(in));
+    stats_.push_back(st0);
+    static_ok_ = sized_ = false;
+    drop_graph();
+    return 0;
+}
+
+int Batch::generate_surfels(std::string& err) {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    for (SceneIn& s : scenes_) s.surfels = ifx::generate_surfels(s.sc, s.sc.cfg.surfels, (int)hw);
+    static_ok_ = sized_ = false;
+    drop_graph();
+    (void)err;
+    return 0;
+}
+
+// Scene records, surfels and the per-scene offsets that do not depend on
+// the transfer.
+int Batch::upload_static(std::string& err) {
+    BK(hipSetDevice(eng_.device()));
+    const uint32_t ns = size();
+    s0_.assign(ns, 0);
+    px0_.assign(ns, 0);
+    term0_.assign(ns, 0);
+    cell0_.assign(ns, 0);
+    nl_.assign(ns, 0);
+    uint64_t N = 0, P = 0, T = 0, cells = 0, rec = 0;
+    Bmax_ = 1;
+    for (uint32_t s = 0; s < ns; ++s) {
+        const HostScene& sc = scenes_[s].sc;
+        s0_[s] = (uint32_t)N;
+        px0_[s] = (uint32_t)P;
+        term0_[s] = (uint32_t)T;
+        cell0_[s] = (uint32_t)cells;
+        nl_[s] = (uint32_t)(sc.quads.size() + sc.spheres.size() + sc.points.size());
+        const uint64_t n = scenes_[s].surfels.size();
+        N += n;
+        P += (uint64_t)sc.cfg.width * sc.cfg.height;
+        T += n * nl_[s];
+        const float diag = sc.diag();
+        const float radius = (diag / sqrtf((float)sc.cfg.surfels)) * 2.6f;
+        const float inv_cell = 1.0f / radius;
+        const f3 ext = sc.hi - sc.lo;
+        uint64_t dx = clampi(sat_i32(ceilf(ext.x * inv_cell)), 1, 512);
+        uint64_t dy = clampi(sat_i32(ceilf(ext.y * inv_cell)), 1, 512);
+        uint64_t dz = clampi(sat_i32(ceilf(ext.z * inv_cell)), 1, 512);
+        cells += dx * dy * dz;
+        rec += sc.obj_rec.size() + sc.plane_rec.size() + sc.sub_rec.size() + sc.node_rec.size() +
+               (sc.lin.size() + 3) / 4 + sc.quad_rec.size() + sc.sph_rec.size() + sc.pnt_rec.size() +
+               sc.mat_rec.size();
+        Bmax_ = std::max(Bmax_, sc.cfg.bounces);
+    }
+    if (N >= (1ull << 31) || P >= (1ull << 31) || T >= (1ull << 32) || cells >= (1ull << 31)) {
+        err = "batch exceeds 2^31 surfels, pixels or grid cells";
+        return -6;
+    }
+    Ntot_ = (uint32_t)N;
+    Ptot_ = (uint32_t)P;
+    Ttot_ = (uint32_t)T;
+    cells_ = (uint32_t)cells;
+    // records
+    std::vector<float4> hrec;
+    hrec.reserve(rec + 1);
+    BK(d_rec.ensure(16ull * (rec + 1)));
+    // surfels
+    std::vector<float4> hp(N), hn(N), ha(N);
+    std::vector<uint32_t> hs(N);
+    for (uint32_t s = 0; s < ns; ++s) {
+        const auto& sf = scenes_[s].surfels;
+        for (size_t k = 0; k < sf.size(); ++k) {
+            size_t i = s0_[s] + k;
+            hp[i] = st4(sf[k].p, sf[k].area);
+            hn[i] = st4(sf[k].n);
+            ha[i] = st4(sf[k].albedo);
+            hs[i] = s;
+        }
+    }
+    for (uint32_t s = 0; s < ns; ++s) {
+        const HostScene& sc = scenes_[s].sc;
+        auto put = [&](const std::vector<float4>& v) { hrec.insert(hrec.end(), v.begin(), v.end()); };
+        put(sc.obj_rec);
+        put(sc.plane_rec);
+        put(sc.sub_rec);
+        put(sc.node_rec);
+        std::vector<uint32_t> lin((sc.lin.size() + 3) & ~(size_t)3, 0u);
+        std::copy(sc.lin.begin(), sc.lin.end(), lin.begin());
+        const float4* lf = (const float4*)lin.data();
+        hrec.insert(hrec.end(), lf, lf + lin.size() / 4);
+        put(sc.quad_rec);
+        put(sc.sph_rec);
+        put(sc.pnt_rec);
+        put(sc.mat_rec);
+    }
+    if (!hrec.empty()) BK(hipMemcpy(d_rec.p, hrec.data(), 16ull * hrec.size(), hipMemcpyHostToDevice));
+    BK(d_spos.ensure(16ull * N));
+    BK(d_snrm.ensure(16ull * N));
+    BK(d_salb.ensure(16ull * N));
+    BK(d_ssid.ensure(4ull * N));
+    if (N) {
+        BK(hipMemcpy(d_spos.p, hp.data(), 16ull * N, hipMemcpyHostToDevice));
+        BK(hipMemcpy(d_snrm.p, hn.data(), 16ull * N, hipMemcpyHostToDevice));
+        BK(hipMemcpy(d_salb.p, ha.data(), 16ull * N, hipMemcpyHostToDevice));
+        BK(hipMemcpy(d_ssid.p, hs.data(), 4ull * N, hipMemcpyHostToDevice));
+    }
+    cl0_.assign(ns, 0);
+    C_.assign(ns, 0);
+    cand0_.assign(ns, 0);
+    M_.assign(ns, 0);
+    static_ok_ = true;
+    sized_ = false;
+    return 0;
+}
+
+// Per-scene device records (views into d_rec, offsets, LDS plans, shading
+// arguments, queue chunks) and the block partitions of the tracing kernels.
+int Batch::build_slots(std::string& err) {
+    const uint32_t ns = size();
+    std::vector<BSlot> hs(ns);
+    const float4* rec = d_rec.as<float4>();
+    uint64_t r = 0;
+    lds_n_ = lds_l_ = 16;
+    const uint32_t G = (uint32_t)eng_.n_cu() * std::max(1u, tuning.blocks_per_cu);
+    std::vector<uint64_t> wc(ns), wv(ns), wd(ns), wp(ns), we(ns);
+    for (uint32_t s = 0; s < ns; ++s) {
+        const HostScene& sc = scenes_[s].sc;
+        BSlot& B = hs[s];
+        B.S = sc.host_view();
+        B.L = sc.host_lights();
+        B.S.obj = rec + r;
+        r += sc.obj_rec.size();
+        B.S.planes = rec + r;
+        r += sc.plane_rec.size();
+        B.S.subs = rec + r;
+        r += sc.sub_rec.size();
+        B.S.nodes = rec + r;
+        r += sc.node_rec.size();
+        B.S.lin = (const uint32_t*)(rec + r);
+        r += (sc.lin.size() + 3) / 4;
+        B.L.quad = rec + r;
+        r += sc.quad_rec.size();
+        B.L.sph = rec + r;
+        r += sc.sph_rec.size();
+        B.L.pnt = rec + r;
+        r += sc.pnt_rec.size();
+        B.L.mat = rec + r;
+        r += sc.mat_rec.size();
+        B.S.use_bvh = 1;
+        // LDS plans (as Engine::plan_lds)
+        const uint32_t n_obj = (uint32_t)sc.objects.size();
+        const uint32_t scene_f4 = (uint32_t)(sc.obj_rec.size() + sc.plane_rec.size() + sc.sub_rec.size() +
+                                             sc.node_rec.size()) + (n_obj + 3u) / 4u;
+        const uint32_t light_f4 =
+            (uint32_t)(sc.quad_rec.size() + sc.sph_rec.size() + sc.pnt_rec.size() + sc.mat_rec.size());
+        const uint32_t depth = sc.max_stack ? sc.max_stack : 1u;
+        size_t stack = ((size_t)kBlock * depth * 6u + 15u) & ~(size_t)15u;
+        size_t bn = (size_t)scene_f4 * 16u + stack, bl = (size_t)(scene_f4 + light_f4) * 16u + stack;
+        B.staged_n = bn <= kLdsBudget ? 1u : 0u;
+        B.staged_l = bl <= kLdsBudget ? 1u : 0u;
+        B.Pn = B.staged_n ? LdsPlan{scene_f4, 0u, depth} : LdsPlan{0u, 0u, depth};
+        B.Pl = B.staged_l ? LdsPlan{scene_f4, light_f4, depth} : LdsPlan{0u, 0u, depth};
+        lds_n_ = std::max(lds_n_, B.staged_n ? bn : stack);
+        lds_l_ = std::max(lds_l_, B.staged_l ? bl : stack);
+        // offsets and sizes
+        const uint32_t N = (uint32_t)scenes_[s].surfels.size();
+        B.s0 = s0_[s];
+        B.N = N;
+        B.cl0 = cl0_[s];
+        B.C = C_[s];
+        B.nl = nl_[s];
+        B.bounces = sc.cfg.bounces;
+        B.cand0 = cand0_[s];
+        B.M = M_[s];
+        B.term0 = term0_[s];
+        B.px0 = px0_[s];
+        B.cell0 = cell0_[s];
+        B.diag = sc.diag();
+        B.exposure = sc.cfg.exposure;
+        const f3 ext = sc.hi - sc.lo;
+        const int32_t res = scenes_[s].res;
+        B.lo = sc.lo;
+        B.res = res;
+        B.cinv = mk((float)res / fmx(ext.x, 1e-6f), (float)res / fmx(ext.y, 1e-6f), (float)res / fmx(ext.z, 1e-6f));
+        B.mscale = mk(32.0f / fmx(ext.x, 1e-6f), 32.0f / fmx(ext.y, 1e-6f), 32.0f / fmx(ext.z, 1e-6f));
+        // shading (Engine::shade_setup, beauty with GI)
+        ShadeArgs& A = B.A;
+        const HCamera& cam = sc.cam;
+        A.cam.origin = cam.pos;
+        A.cam.fwd = nrm3(cam.look - cam.pos);
+        A.cam.right = nrm3(cross3(A.cam.fwd, cam.up));
+        A.cam.up = cross3(A.cam.right, A.cam.fwd);
+        const float rads_per_deg = kPi / 180.0f;
+        A.cam.tan_half = tanf(cam.fov * rads_per_deg * 0.5f);
+        A.cam.aspect = (float)sc.cfg.width / (float)sc.cfg.height;
+        A.W = sc.cfg.width;
+        A.H = sc.cfg.height;
+        A.mode = M_BEAUTY;
+        A.use_gi = 1;
+        A.tmax = B.diag * 4.0f;
+        A.radius = (B.diag / sqrtf((float)sc.cfg.surfels)) * 2.6f;
+        A.spos = d_spos.as<float4>();
+        A.snrm = d_snrm.as<float4>();
+        A.irr = d_irr.as<float4>();
+        const float inv_cell = 1.0f / A.radius;
+        A.grid.lo = sc.lo;
+        A.grid.inv_cell = inv_cell;
+        A.grid.dim[0] = clampi(sat_i32(ceilf(ext.x * inv_cell)), 1, 512);
+        A.grid.dim[1] = clampi(sat_i32(ceilf(ext.y * inv_cell)), 1, 512);
+        A.grid.dim[2] = clampi(sat_i32(ceilf(ext.z * inv_cell)), 1, 512);
+        A.grid.starts = d_gstart.as<uint32_t>() + cell0_[s];
+        A.grid.items = (gitems_alt_ ? d_gval2 : d_gval).as<uint32_t>();
+        wc[s] = 16ull * N;
+        wv[s] = M_[s];
+        wd[s] = (uint64_t)N * nl_[s];
+        wp[s] = (uint64_t)((A.W + 15) / 16) * ((A.H + 15) / 16) * kBlock;
+        we[s] = (uint64_t)A.W * A.H;
+    }
+    blk_closure_ = split_blocks(wc, G);
+    blk_vis_ = split_blocks(wv, G);
+    blk_direct_ = exact_blocks(wd);
+    blk_shade_ = exact_blocks(wp);
+    blk_encode_ = exact_blocks(we);
+    for (uint32_t s = 0; s < ns; ++s) {
+        hs[s].chunk_c = persist_chunk(wc[s], std::max(1u, blk_closure_[s + 1] - blk_closure_[s]), tuning.chunk_div);
+        hs[s].chunk_v = persist_chunk(wv[s], std::max(1u, blk_vis_[s + 1] - blk_vis_[s]), tuning.chunk_div);
+    }
+    BK(hipStreamSynchronize(stream_));  // kernels already queued read the previous slots
+    BK(d_slots.ensure(sizeof(BSlot) * (ns + 1)));
+    BK(d_blk.ensure(4ull * 5 * (ns + 1)));
+    std::vector<uint32_t> hb;
+    for (auto* v : {&blk_closure_, &blk_vis_, &blk_direct_, &blk_shade_, &blk_encode_})
+        hb.insert(hb.end(), v->begin(), v->end());
+    BK(hipMemcpy(d_slots.p, hs.data(), sizeof(BSlot) * ns, hipMemcpyHostToDevice));
+    BK(hipMemcpy(d_blk.p, hb.data(), 4ull * hb.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int Batch::run(double& ms, std::string& err) {
+    if (scenes_.empty()) {
+        err = "empty batch";
+        return -3;
+    }
+    BK(hipSetDevice(eng_.device()));
+    auto t0 = std::chrono::steady_clock::now();
+    if (!static_ok_)
+        if (int rc = upload_static(err)) return rc;
+    if (use_graph_ && sized_) {
+        if (!graph_exec_) {
+            // Capture one frame with the sizes and slots of the last eager frame.
+            BK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            std::string e2;
+            int rc = frame(false, e2);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(stream_, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                err = e2;
+                return rc;
+            }
+            BK(ce);
+            graph_ = g;
+            BK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        }
+        BK(hipGraphLaunch(graph_exec_, stream_));
+        BK(hipStreamSynchronize(stream_));
+    } else {
+        if (int rc = frame(true, err)) return rc;
+        BK(hipStreamSynchronize(stream_));
+        if (int rc = finish_stats(err)) return rc;
+    }
+    ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+// One frame.  Eager: sizes are read back where the single-scene path reads
+// them, buffers grow to fit and the slots are rebuilt.  Capture (eager =
+// false): every size and slot comes from the previous eager frame, nothing
+// is read back, allocated, copied from the host or memset.
+int Batch::frame(bool eager, std::string& err) {
+    const uint32_t ns = size();
+    const uint32_t N = Ntot_;
+    hipStream_t st = stream_;
+    auto fill = [&](DevBuf& b, uint64_t words, uint32_t v) {
+        if (words) kb_fill<<<blocks_for(words, 256), 256, 0, st>>>(b.as<uint32_t>(), words, v);
+    };
+    auto scratch = [&](uint64_t words) -> int {
+        if (eager) BK(d_scratch.ensure(4ull * words + 16));
+        return 0;
+    };
+    if (eager) {
+        if (int rc = build_slots(err)) return rc;  // sizes of the last frame; refreshed below
+        BK(d_queues.ensure(16ull * ns));
+        BK(d_counters.ensure(32ull * ns));
+        BK(d_open.ensure(4ull * N + 4));
+        BK(d_sky.ensure(4ull * N + 4));
+    }
+    fill(d_queues, 4ull * ns, 0u);
+    fill(d_counters, 8ull * ns, 0u);
+    fill(d_open, N, 0u);
+    const BSlot* slots = d_slots.as<BSlot>();
+    const uint32_t* blk = d_blk.as<uint32_t>();
+    const uint32_t* blk_c = blk;
+    const uint32_t* blk_v = blk + (ns + 1);
+    const uint32_t* blk_d = blk + 2 * (ns + 1);
+    const uint32_t* blk_s = blk + 3 * (ns + 1);
+    const uint32_t* blk_e = blk + 4 * (ns + 1);
+    unsigned long long* ctr = d_counters.as<unsigned long long>();
+    const float4* spos = d_spos.as<float4>();
+    const float4* snrm = d_snrm.as<float4>();
+    const uint32_t* ssid = d_ssid.as<uint32_t>();
+    bool alt = false;
+
+    // ---- clusters
+    if (eager) {
+        BK(d_ckey.ensure(8ull * N + 8));
+        BK(d_ckey2.ensure(8ull * N + 8));
+        BK(d_cval.ensure(4ull * N + 4));
+        BK(d_cval2.ensure(4ull * N + 4));
+        BK(d_cflag.ensure(4ull * (N + 1)));
+        BK(d_cseg.ensure(4ull * (N + 1)));
+        if (int rc = scratch(std::max(gsort::sort_scratch_words(N), gsort::scan_scratch_words(N + 1)))) return rc;
+    }
+    if (N) {
+        kb_cluster_keys<<<blocks_for(N, 256), 256, 0, st>>>(spos, snrm, ssid, N, slots, d_ckey.as<uint64_t>(),
+                                                            d_cval.as<uint32_t>());
+        BK(gsort::sort_pairs<uint64_t>(d_ckey.as<uint64_t>(), d_cval.as<uint32_t>(), d_ckey2.as<uint64_t>(),
+                                       d_cval2.as<uint32_t>(), N, 40u + bits_for(ns ? ns - 1 : 0), d_scratch.as<uint32_t>(),
+                                       st, alt));
+        members_alt_ = alt;
+    }
+    DevBuf& ckeys = members_alt_ ? d_ckey2 : d_ckey;
+    DevBuf& members = members_alt_ ? d_cval2 : d_cval;
+    if (N) {
+        k_seg_flags<<<blocks_for(N, 256), 256, 0, st>>>(ckeys.as<uint64_t>(), N, d_cflag.as<uint32_t>());
+        BK(gsort::exclusive_scan(d_cflag.as<uint32_t>(), d_cseg.as<uint32_t>(), N, d_scratch.as<uint32_t>(), st));
+    }
+    if (eager) {
+        uint32_t tail[2] = {0, 0};
+        if (N) {
+            BK(hipMemcpyAsync(&tail[0], d_cseg.as<uint32_t>() + N - 1, 4, hipMemcpyDeviceToHost, st));
+            BK(hipMemcpyAsync(&tail[1], d_cflag.as<uint32_t>() + N - 1, 4, hipMemcpyDeviceToHost, st));
+            BK(hipStreamSynchronize(st));
+        }
+        nseg_ = tail[0] + tail[1];
+        BK(d_cbegin.ensure(4ull * (nseg_ + 1)));
+        BK(d_craw_pos.ensure(16ull * nseg_ + 16));
+        BK(d_craw_nrm.ensure(16ull * nseg_ + 16));
+        BK(d_cvalid.ensure(4ull * (nseg_ + 1)));
+        BK(d_cidx.ensure(4ull * (nseg_ + 1)));
+        BK(d_cpos.ensure(16ull * nseg_ + 16));
+        BK(d_cnrm.ensure(16ull * nseg_ + 16));
+        BK(d_crange.ensure(8ull * nseg_ + 8));
+        BK(d_csid.ensure(4ull * nseg_ + 4));
+        BK(d_ccount.ensure(4ull * ns + 4));
+    }
+    const uint32_t nseg = nseg_;
+    if (nseg) {
+        k_seg_begin<<<blocks_for(N, 256), 256, 0, st>>>(d_cflag.as<uint32_t>(), d_cseg.as<uint32_t>(), N,
+                                                        d_cbegin.as<uint32_t>());
+        k_cluster_build<<<blocks_for(nseg, 64), 64, 0, st>>>(d_cbegin.as<uint32_t>(), nseg, members.as<uint32_t>(),
+                                                             spos, snrm, d_craw_pos.as<float4>(),
+                                                             d_craw_nrm.as<float4>(), d_cvalid.as<uint32_t>());
+        BK(gsort::exclusive_scan(d_cvalid.as<uint32_t>(), d_cidx.as<uint32_t>(), nseg, d_scratch.as<uint32_t>(), st));
+        k_cluster_compact<<<blocks_for(nseg, 256), 256, 0, st>>>(
+            nseg, d_cvalid.as<uint32_t>(), d_cidx.as<uint32_t>(), d_cbegin.as<uint32_t>(), d_craw_pos.as<float4>(),
+            d_craw_nrm.as<float4>(), d_cpos.as<float4>(), d_cnrm.as<float4>(), d_crange.as<uint2>());
+    }
+    if (eager) {
+        uint32_t tail[2] = {0, 0};
+        if (nseg) {
+            BK(hipMemcpyAsync(&tail[0], d_cidx.as<uint32_t>() + nseg - 1, 4, hipMemcpyDeviceToHost, st));
+            BK(hipMemcpyAsync(&tail[1], d_cvalid.as<uint32_t>() + nseg - 1, 4, hipMemcpyDeviceToHost, st));
+            BK(hipStreamSynchronize(st));
+        }
+        Ctot_ = tail[0] + tail[1];
+    }
+    const uint32_t C = Ctot_;
+    fill(d_ccount, ns, 0u);
+    if (C)
+        kb_cluster_scene<<<blocks_for(C, 256), 256, 0, st>>>(C, d_crange.as<uint2>(), members.as<uint32_t>(), ssid,
+                                                             d_csid.as<uint32_t>(), d_ccount.as<uint32_t>());
+    if (eager) {
+        std::vector<uint32_t> cc(ns, 0);
+        BK(hipMemcpyAsync(cc.data(), d_ccount.p, 4ull * ns, hipMemcpyDeviceToHost, st));
+        BK(hipStreamSynchronize(st));
+        uint32_t acc = 0;
+        for (uint32_t s = 0; s < ns; ++s) {
+            cl0_[s] = acc;
+            C_[s] = cc[s];
+            acc += cc[s];
+        }
+        if (int rc = build_slots(err)) return rc;
+    }
+
+    // ---- Morton work order (scene-major)
+    if (eager) {
+        BK(d_code.ensure(4ull * N + 4));
+        BK(d_mkey.ensure(4ull * N + 4));
+        BK(d_mkey2.ensure(4ull * N + 4));
+        BK(d_mval.ensure(4ull * N + 4));
+        BK(d_rorder.ensure(4ull * N + 4));
+    }
+    if (N) {
+        kb_morton<<<blocks_for(N, 256), 256, 0, st>>>(spos, ssid, N, slots, d_code.as<uint32_t>(),
+                                                      d_mkey.as<uint32_t>(), d_mval.as<uint32_t>());
+        BK(gsort::sort_pairs<uint32_t>(d_mkey.as<uint32_t>(), d_mval.as<uint32_t>(), d_mkey2.as<uint32_t>(),
+                                       d_rorder.as<uint32_t>(), N, 15u + bits_for(ns ? ns - 1 : 0),
+                                       d_scratch.as<uint32_t>(), st, alt));
+        rorder_alt_ = alt;
+    }
+    const uint32_t* rorder = (rorder_alt_ ? d_rorder : d_mval).as<uint32_t>();
+
+    // ---- closure (independent of the links)
+    const uint32_t gc = blk_closure_.back();
+    if (gc)
+        kb_closure<<<gc, kBlock, lds_n_, st>>>(slots, blk_c, ns, spos, snrm, rorder, eng_.closure_dirs(),
+                                               d_open.as<uint32_t>(), d_queues.as<uint32_t>(), ctr);
+
+    // ---- candidates
+    if (eager) {
+        BK(d_fcount.ensure(4ull * (N + 1)));
+        BK(d_cstart.ensure(4ull * (N + 1)));
+    }
+    fill(d_fcount, N + 1, 0u);
+    const uint32_t ffb = std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(N, 4), 8192));
+    if (N)
+        kb_ff<false><<<ffb, kBlock, 0, st>>>(spos, snrm, ssid, N, slots, d_cpos.as<float4>(), d_cnrm.as<float4>(),
+                                             d_fcount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                                             nullptr);
+    BK(gsort::exclusive_scan(d_fcount.as<uint32_t>(), d_cstart.as<uint32_t>(), N + 1, d_scratch.as<uint32_t>(), st));
+    if (eager) {
+        std::vector<uint32_t> cs(N + 1, 0);
+        BK(hipMemcpyAsync(cs.data(), d_cstart.p, 4ull * (N + 1), hipMemcpyDeviceToHost, st));
+        BK(hipStreamSynchronize(st));
+        Mtot_ = cs[N];
+        if (Mtot_ >= (1u << 31)) {
+            err = "candidate count exceeds 2^31";
+            return -6;
+        }
+        for (uint32_t s = 0; s < ns; ++s) {
+            cand0_[s] = cs[s0_[s]];
+            M_[s] = cs[s0_[s] + scenes_[s].surfels.size()] - cand0_[s];
+        }
+        BK(d_cand_ci.ensure(4ull * Mtot_ + 4));
+        BK(d_cand_row.ensure(4ull * Mtot_ + 4));
+        BK(d_cand_v.ensure(4ull * Mtot_ + 4));
+        BK(d_vkey.ensure(8ull * Mtot_ + 8));
+        BK(d_vkey2.ensure(8ull * Mtot_ + 8));
+        BK(d_vval.ensure(4ull * Mtot_ + 4));
+        BK(d_vorder.ensure(4ull * Mtot_ + 4));
+        if (int rc = scratch(std::max(gsort::sort_scratch_words(Mtot_), gsort::sort_scratch_words(N)))) return rc;
+        if (int rc = build_slots(err)) return rc;
+    }
+    const uint32_t M = Mtot_;
+    if (N && M)
+        kb_ff<true><<<ffb, kBlock, 0, st>>>(spos, snrm, ssid, N, slots, d_cpos.as<float4>(), d_cnrm.as<float4>(),
+                                            nullptr, d_cstart.as<uint32_t>(), d_cand_ci.as<uint32_t>(),
+                                            d_cand_row.as<uint32_t>(), d_code.as<uint32_t>(), d_vkey.as<uint64_t>(),
+                                            d_vval.as<uint32_t>());
+    if (M) {
+        BK(gsort::sort_pairs<uint64_t>(d_vkey.as<uint64_t>(), d_vval.as<uint32_t>(), d_vkey2.as<uint64_t>(),
+                                       d_vorder.as<uint32_t>(), M, bits_for(C ? C - 1 : 0) + 15u,
+                                       d_scratch.as<uint32_t>(), st, alt));
+        vorder_alt_ = alt;
+    }
+    const uint32_t* vorder = (vorder_alt_ ? d_vorder : d_vval).as<uint32_t>();
+    const uint32_t gv = blk_vis_.back();
+    if (gv)
+        kb_visibility<<<gv, kBlock, lds_n_, st>>>(slots, blk_v, ns, spos, snrm, d_cpos.as<float4>(),
+                                                  d_cnrm.as<float4>(), d_cand_ci.as<uint32_t>(),
+                                                  d_cand_row.as<uint32_t>(), vorder, d_cand_v.as<float>(),
+                                                  d_queues.as<uint32_t>(), ctr);
+
+    // ---- calibration, link scan, compaction
+    if (eager) {
+        BK(d_lcount.ensure(4ull * (N + 1)));
+        BK(d_lstart.ensure(4ull * (N + 1)));
+        BK(d_scale.ensure(4ull * N + 4));
+        BK(d_rowsum.ensure(4ull * N + 4));
+    }
+    fill(d_lcount, N + 1, 0u);
+    if (N)
+        k_row_sum<<<blocks_for(N, 256), 256, 0, st>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
+                                                      d_open.as<uint32_t>(), d_sky.as<float>(),
+                                                      d_lcount.as<uint32_t>(), d_scale.as<float>(),
+                                                      d_rowsum.as<float>());
+    BK(gsort::exclusive_scan(d_lcount.as<uint32_t>(), d_lstart.as<uint32_t>(), N + 1, d_scratch.as<uint32_t>(), st));
+    if (eager) {
+        uint32_t L = 0;
+        BK(hipMemcpyAsync(&L, d_lstart.as<uint32_t>() + N, 4, hipMemcpyDeviceToHost, st));
+        BK(hipStreamSynchronize(st));
+        Ltot_ = L;
+        BK(d_idx.ensure(4ull * L + 4));
+        BK(d_w.ensure(4ull * L + 4));
+    }
+    if (N)
+        k_compact<<<blocks_for(N, 256), 256, 0, st>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
+                                                      d_cand_ci.as<uint32_t>(), d_lstart.as<uint32_t>(),
+                                                      d_scale.as<float>(), d_idx.as<uint32_t>(), d_w.as<float>());
+
+    // ---- solve: direct light, Jacobi
+    if (eager) {
+        BK(d_terms.ensure(16ull * Ttot_ + 16));
+        BK(d_gterm.ensure(4ull * Ttot_ + 4));
+        BK(d_direct.ensure(16ull * N + 16));
+        BK(d_rad.ensure(16ull * N + 16));
+        BK(d_irr.ensure(16ull * N + 16));
+        BK(d_bc.ensure(16ull * C + 16));
+        BK(d_resid.ensure(4ull * ns * Bmax_ + 4));
+        if (int rc = build_slots(err)) return rc;  // irr pointer of the shading arguments
+    }
+    const uint32_t gd = blk_direct_.back();
+    if (gd)
+        kb_direct_terms<<<gd, kBlock, lds_l_, st>>>(slots, blk_d, ns, spos, snrm, d_salb.as<float4>(), rorder,
+                                                    d_terms.as<float4>(), d_gterm.as<float>(), ctr);
+    if (N)
+        kb_direct_sum<<<blocks_for(N, 256), 256, 0, st>>>(ssid, N, slots, d_terms.as<float4>(), d_direct.as<float4>(),
+                                                          d_rad.as<float4>(), d_irr.as<float4>());
+    fill(d_resid, (uint64_t)ns * Bmax_, 0u);
+    if (C && N) {
+        BJacobi A;
+        A.N = N;
+        A.C = C;
+        A.Bmax = Bmax_;
+        A.wave_rows = tuning.wave_rows ? 1u : 0u;
+        A.tol = 1e-4f;
+        A.slots = slots;
+        A.ssid = ssid;
+        A.csid = d_csid.as<uint32_t>();
+        A.range = d_crange.as<uint2>();
+        A.members = members.as<uint32_t>();
+        A.spos = spos;
+        A.cpos = d_cpos.as<float4>();
+        A.lstart = d_lstart.as<uint32_t>();
+        A.idx = d_idx.as<uint32_t>();
+        A.w = d_w.as<float>();
+        A.direct = d_direct.as<float4>();
+        A.salb = d_salb.as<float4>();
+        A.rad = d_rad.as<float4>();
+        A.irr = d_irr.as<float4>();
+        A.bc = d_bc.as<float4>();
+        A.resid = d_resid.as<uint32_t>();
+        const uint32_t g_agg = blocks_for(64ull * C, kBlock);
+        const uint32_t g_rows = blocks_for(A.wave_rows ? 64ull * N : (uint64_t)N, kBlock);
+        const uint32_t group = 8;
+        for (uint32_t it0 = 0; it0 < Bmax_; it0 += group) {
+            const uint32_t it1 = std::min(Bmax_, it0 + group);
+            for (uint32_t it = it0; it < it1; ++it) {
+                kb_jacobi_agg<<<g_agg, kBlock, 0, st>>>(A, it);
+                kb_jacobi_rows<<<g_rows, kBlock, 0, st>>>(A, it);
+            }
+            if (eager && it1 < Bmax_) {
+                // stop enqueuing once every scene has converged or run out of bounces
+                std::vector<uint32_t> res((size_t)ns * Bmax_);
+                BK(hipMemcpyAsync(res.data(), A.resid, 4ull * res.size(), hipMemcpyDeviceToHost, st));
+                BK(hipStreamSynchronize(st));
+                bool any = false;
+                for (uint32_t s = 0; s < ns && !any; ++s) {
+                    const uint32_t b = scenes_[s].sc.cfg.bounces;
+                    if (it1 >= b) continue;
+                    bool conv = false;
+                    for (uint32_t it = 0; it < it1; ++it)
+                        if (bitsf(res[(size_t)s * Bmax_ + it]) < 1e-4f) conv = true;
+                    if (!conv) any = true;
+                }
+                if (!any) break;
+            }
+        }
+    }
+
+    // ---- shade: per-scene surfel grids in one counting sort, beauty pixels, encode
+    if (eager) {
+        BK(d_gkey.ensure(4ull * N + 4));
+        BK(d_gkey2.ensure(4ull * N + 4));
+        BK(d_gval.ensure(4ull * N + 4));
+        BK(d_gval2.ensure(4ull * N + 4));
+        BK(d_gcount.ensure(4ull * (cells_ + 1)));
+        BK(d_gstart.ensure(4ull * (cells_ + 1)));
+        BK(d_film.ensure(16ull * Ptot_ + 16));
+        BK(d_rgb8.ensure(3ull * Ptot_ + 4));
+        if (int rc = scratch(std::max({gsort::sort_scratch_words(Mtot_), gsort::sort_scratch_words(N),
+                                       gsort::scan_scratch_words(cells_ + 1)})))
+            return rc;
+        gitems_alt_ = (bits_for(cells_ ? cells_ - 1 : 0) + 7u) / 8u % 2u == 1u;
+        if (int rc = build_slots(err)) return rc;  // grid pointers
+    }
+    fill(d_gcount, cells_ + 1, 0u);
+    if (N)
+        kb_grid_keys<<<blocks_for(N, 256), 256, 0, st>>>(spos, ssid, N, slots, d_gkey.as<uint32_t>(),
+                                                         d_gval.as<uint32_t>(), d_gcount.as<uint32_t>());
+    BK(gsort::exclusive_scan(d_gcount.as<uint32_t>(), d_gstart.as<uint32_t>(), cells_ + 1, d_scratch.as<uint32_t>(),
+                             st));
+    if (N) {
+        BK(gsort::sort_pairs<uint32_t>(d_gkey.as<uint32_t>(), d_gval.as<uint32_t>(), d_gkey2.as<uint32_t>(),
+                                       d_gval2.as<uint32_t>(), N, bits_for(cells_ ? cells_ - 1 : 0),
+                                       d_scratch.as<uint32_t>(), st, alt));
+        if (alt != gitems_alt_) {
+            err = "internal: grid sort parity";
+            return -4;
+        }
+    }
+    const uint32_t gs = blk_shade_.back();
+    if (gs) kb_shade<<<gs, kBlock, lds_l_, st>>>(slots, blk_s, ns, d_film.as<float4>(), ctr);
+    const uint32_t ge = blk_encode_.back();
+    if (ge)
+        kb_encode<<<ge, 256, 0, st>>>(slots, blk_e, ns, d_film.as<float4>(), eng_.srgb_thresholds(),
+                                      d_rgb8.as<uint8_t>());
+    BK(hipGetLastError());
+    if (eager) sized_ = true;
+    return 0;
+}
+
+int Batch::finish_stats(std::string& err) {
+    const uint32_t ns = size();
+    std::vector<unsigned long long> cv(4ull * ns, 0ull);
+    std::vector<uint32_t> res((size_t)ns * Bmax_, 0u), ls(Ntot_ + 1, 0u);
+    std::vector<float> rs(Ntot_);
+    BK(hipMemcpy(cv.data(), d_counters.p, 32ull * ns, hipMemcpyDeviceToHost));
+    BK(hipMemcpy(res.data(), d_resid.p, 4ull * res.size(), hipMemcpyDeviceToHost));
+    BK(hipMemcpy(ls.data(), d_lstart.p, 4ull * (Ntot_ + 1), hipMemcpyDeviceToHost));
+    if (Ntot_) BK(hipMemcpy(rs.data(), d_rowsum.p, 4ull * Ntot_, hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < ns; ++s) {
+        BatchSceneStats& o = stats_[s];
+        const uint32_t n = (uint32_t)scenes_[s].surfels.size(), a = s0_[s];
+        o.surfels = n;
+        o.clusters = C_[s];
+        o.candidates = M_[s];
+        o.links = ls[a + n] - ls[a];
+        o.evals_transfer = cv[4 * s];
+        o.evals_solve = cv[4 * s + 1];
+        o.evals_shade = cv[4 * s + 2];
+        const uint32_t b = scenes_[s].sc.cfg.bounces;
+        o.iterations = 0;
+        o.residual = 0;
+        if (C_[s] && n) {
+            o.iterations = b;
+            for (uint32_t it = 0; it < b; ++it)
+                if (bitsf(res[(size_t)s * Bmax_ + it]) < 1e-4f) {
+                    o.iterations = it + 1;
+                    break;
+                }
+            o.residual = bitsf(res[(size_t)s * Bmax_ + o.iterations - 1]);
+        }
+        float acc = -0.0f;
+        uint64_t interior = 0;
+        for (uint32_t i = a; i < a + n; ++i)
+            if (rs[i] >= 0.0f) acc += rs[i], ++interior;
+        o.mean_row_sum = interior ? acc / (float)interior : 0.0f;
+    }
+    return 0;
+}
+
+int Batch::autotune(double* best_ms, std::string& err) {
+    // Each candidate shape runs one eager frame; outputs are identical for
+    // every shape, so the fastest is kept.
+    BatchTuning best = tuning;
+    double best_t = 1e300;
+    // Persistent blocks per CU x queue chunk divisor first, then the Jacobi
+    // row mapping at the best of those: six eager frames in all.
+    drop_graph();
+    auto trial = [&](BatchTuning t) -> int {
+        tuning = t;
+        auto t0 = std::chrono::steady_clock::now();
+        if (int rc = frame(true, err)) return rc;
+        BK(hipStreamSynchronize(stream_));
+        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (ms < best_t) best_t = ms, best = t;
+        return 0;
+    };
+    for (uint32_t a : {8u, 16u})
+        for (uint32_t d : {4u, 16u})
+            if (int rc = trial(BatchTuning{a, d, 0u})) return rc;
+    if (int rc = trial(BatchTuning{best.blocks_per_cu, best.chunk_div, 1u})) return rc;
+    tuning = best;
+    if (int rc = frame(true, err)) return rc;
+    BK(hipStreamSynchronize(stream_));
+    if (int rc = finish_stats(err)) return rc;
+    if (best_ms) *best_ms = best_t;
+    return 0;
+}
+
+int Batch::download_image(uint32_t s, float* film, uint8_t* rgb8, std::string& err) {
+    if (s >= size() || !sized_) {
+        err = "no frame for this scene";
+        return -3;
+    }
+    const HCfg& c = scenes_[s].sc.cfg;
+    const uint64_t n = (uint64_t)c.width * c.height;
+    if (film && n) {
+        BK(d_stage.ensure(12ull * n));
+        k_pack3<<<blocks_for(n, 256), 256, 0, stream_>>>(d_film.as<float4>() + px0_[s], n, d_stage.as<float>());
+        BK(hipMemcpyAsync(film, d_stage.p, 12ull * n, hipMemcpyDeviceToHost, stream_));
+    }
+    if (rgb8 && n) BK(hipMemcpyAsync(rgb8, d_rgb8.as<uint8_t>() + 3ull * px0_[s], 3ull * n, hipMemcpyDeviceToHost, stream_));
+    BK(hipStreamSynchronize(stream_));
+    return 0;
+}
+
+int Batch::download_radiosity(uint32_t s, float* direct, float* irr, float* rad, std::string& err) {
+    if (s >= size() || !sized_) {
+        err = "no frame for this scene";
+        return -3;
+    }
+    const uint64_t n = scenes_[s].surfels.size();
+    BK(d_stage.ensure(12ull * n + 12));
+    for (auto [host, dev] : {std::pair<float*, DevBuf*>{direct, &d_direct}, {irr, &d_irr}, {rad, &d_rad}}) {
+        if (!host || !n) continue;
+        k_pack3<<<blocks_for(n, 256), 256, 0, stream_>>>(dev->as<float4>() + s0_[s], n, d_stage.as<float>());
+        BK(hipMemcpyAsync(host, d_stage.p, 12ull * n, hipMemcpyDeviceToHost, stream_));
+        BK(hipStreamSynchronize(stream_));
+    }
+    return 0;
+}
+
+int Batch::download_transfer(uint32_t s, uint32_t* starts, uint32_t* idx, float* w, std::string& err) {
+    if (s >= size() || !sized_) {
+        err = "no frame for this scene";
+        return -3;
+    }
+    const uint32_t n = (uint32_t)scenes_[s].surfels.size(), a = s0_[s];
+    std::vector<uint32_t> ls(n + 1);
+    BK(hipMemcpy(ls.data(), d_lstart.as<uint32_t>() + a, 4ull * (n + 1), hipMemcpyDeviceToHost));
+    const uint32_t l0 = ls[0], L = ls[n] - l0;
+    if (starts)
+        for (uint32_t i = 0; i <= n; ++i) starts[i] = ls[i] - l0;
+    if (idx && L) {
+        BK(hipMemcpy(idx, d_idx.as<uint32_t>() + l0, 4ull * L, hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < L; ++k) idx[k] -= cl0_[s];  // scene-local cluster index
+    }
+    if (w && L) BK(hipMemcpy(w, d_w.as<float>() + l0, 4ull * L, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // namespace ifx

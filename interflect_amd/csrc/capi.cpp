@@ -7,12 +7,18 @@
 #include <thread>
 
 #include "../../include/interflect.h"
+#include "batch.h"
 #include "engine.h"
 
 struct ifx_ctx {
     ifx::Engine eng;
     std::string err;
     bool have_scene = false;
+};
+
+struct ifx_batch {
+    ifx::Batch b;
+    std::string err;
 };
 
 namespace {
@@ -36,6 +42,24 @@ ifx_status guarded(ifx_ctx* ctx, F&& f) {
         return IFX_E_INVALID;
     } catch (...) {
         ctx->err = "unknown error";
+        return IFX_E_INVALID;
+    }
+}
+
+template <class F>
+ifx_status bguarded(ifx_batch* b, F&& f) {
+    if (!b) return IFX_E_INVALID;
+    try {
+        b->err.clear();
+        return f();
+    } catch (const std::bad_alloc&) {
+        b->err = "host allocation failed";
+        return IFX_E_NOMEM;
+    } catch (const std::exception& e) {
+        b->err = e.what();
+        return IFX_E_INVALID;
+    } catch (...) {
+        b->err = "unknown error";
         return IFX_E_INVALID;
     }
 }
@@ -447,6 +471,109 @@ ifx_status ifx_get_direct_geometry(ifx_ctx* ctx, float* g) {
         if (ifx_status s = need_scene(ctx)) return s;
         return ctx->eng.download_direct_geometry(g, ctx->err);
     });
+}
+
+ifx_status ifx_batch_create(ifx_batch** out, int32_t device) {
+    if (!out) return IFX_E_INVALID;
+    *out = nullptr;
+    ifx_batch* b = new (std::nothrow) ifx_batch();
+    if (!b) return IFX_E_NOMEM;
+    int rc = b->b.init(device, b->err);
+    *out = b;  // kept on failure so the caller can read the message
+    return rc;
+}
+
+ifx_status ifx_batch_destroy(ifx_batch* b) {
+    delete b;
+    return IFX_OK;
+}
+
+const char* ifx_batch_last_error(const ifx_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+ifx_status ifx_batch_add_scene(ifx_batch* b, const char* rad_text, int32_t cluster_res, uint32_t* index) {
+    return bguarded(b, [&]() -> ifx_status {
+        if (!rad_text) return IFX_E_INVALID;
+        uint32_t i = 0;
+        int rc = b->b.add_scene(rad_text, cluster_res, i, b->err);
+        if (rc) return rc;
+        if (index) *index = i;
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_batch_generate_surfels(ifx_batch* b) {
+    return bguarded(b, [&]() -> ifx_status { return b->b.generate_surfels(b->err); });
+}
+
+ifx_status ifx_batch_set_graph(ifx_batch* b, int32_t on) {
+    return bguarded(b, [&]() -> ifx_status {
+        b->b.set_graph(on != 0);
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_batch_set_tuning(ifx_batch* b, uint32_t blocks_per_cu, uint32_t chunk_div, uint32_t wave_rows) {
+    return bguarded(b, [&]() -> ifx_status {
+        if (blocks_per_cu < 1 || blocks_per_cu > 64 || chunk_div < 1 || chunk_div > 256 || wave_rows > 1) {
+            b->err = "tuning out of range (blocks_per_cu 1..64, chunk_div 1..256, wave_rows 0..1)";
+            return IFX_E_INVALID;
+        }
+        b->b.tuning = ifx::BatchTuning{blocks_per_cu, chunk_div, wave_rows};
+        b->b.set_graph(b->b.graph_active());  // a captured graph holds the old shapes
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_batch_autotune(ifx_batch* b, float* best_ms) {
+    return bguarded(b, [&]() -> ifx_status {
+        double ms = 0;
+        int rc = b->b.autotune(&ms, b->err);
+        if (rc) return rc;
+        if (best_ms) *best_ms = (float)ms;
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_batch_run(ifx_batch* b, float* ms) {
+    return bguarded(b, [&]() -> ifx_status {
+        double t = 0;
+        int rc = b->b.run(t, b->err);
+        if (rc) return rc;
+        if (ms) *ms = (float)t;
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_batch_get_stats(ifx_batch* b, uint32_t scene, ifx_batch_scene_stats* o) {
+    return bguarded(b, [&]() -> ifx_status {
+        if (!o || scene >= b->b.size()) return IFX_E_INVALID;
+        const ifx::BatchSceneStats& s = b->b.stats(scene);
+        o->surfels = s.surfels;
+        o->candidates = s.candidates;
+        o->links = s.links;
+        o->evals_transfer = s.evals_transfer;
+        o->evals_solve = s.evals_solve;
+        o->evals_shade = s.evals_shade;
+        o->clusters = s.clusters;
+        o->iterations = s.iterations;
+        o->width = s.width;
+        o->height = s.height;
+        o->residual = s.residual;
+        o->mean_row_sum = s.mean_row_sum;
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_batch_get_image(ifx_batch* b, uint32_t scene, float* film_rgb, uint8_t* rgb8) {
+    return bguarded(b, [&]() -> ifx_status { return b->b.download_image(scene, film_rgb, rgb8, b->err); });
+}
+
+ifx_status ifx_batch_get_radiosity(ifx_batch* b, uint32_t scene, float* direct, float* irr, float* rad) {
+    return bguarded(b, [&]() -> ifx_status { return b->b.download_radiosity(scene, direct, irr, rad, b->err); });
+}
+
+ifx_status ifx_batch_get_transfer(ifx_batch* b, uint32_t scene, uint32_t* starts, uint32_t* idx, float* w) {
+    return bguarded(b, [&]() -> ifx_status { return b->b.download_transfer(scene, starts, idx, w, b->err); });
 }
 
 ifx_status ifx_kernel_ops(ifx_ctx* ctx, const char* name, uint64_t* ops) {

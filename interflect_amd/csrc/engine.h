@@ -110,6 +110,12 @@ class Engine {
     int upload_radiosity(const float* direct, const float* irr, const float* rad, std::string& err);
 
     void set_profiling(bool on) { prof_ = on; }
+    // Scene-independent device tables and device facts (shared with the batch engine).
+    const float4* closure_dirs() const { return d_dirs.as<float4>(); }
+    const float* srgb_thresholds() const { return d_srgb.as<float>(); }
+    int n_cu() const { return n_cu_; }
+    int device() const { return device_; }
+    bool has_device() const { return has_device_; }
     // Instrumented launches of the tracing kernels (algorithmic FP32 operation tallies).
     void set_counting(bool on) { count_ops_ = on; }
     const std::map<std::string, uint64_t>& kernel_ops() const { return kops_; }

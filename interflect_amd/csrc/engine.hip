@@ -44,29 +44,6 @@ void DevBuf::release() {
     cap = 0;
 }
 
-// src/formfactor.rs:70-89
-__device__ __forceinline__ uint32_t normal_bucket(f3 n) {
-    f3 a = abs3(n);
-    if (a.x >= a.y && a.x >= a.z) return n.x >= 0.0f ? 0u : 1u;
-    if (a.y >= a.z) return n.y >= 0.0f ? 2u : 3u;
-    return n.z >= 0.0f ? 4u : 5u;
-}
-
-// Surfel -> cluster candidate geometry and culls (src/formfactor.rs:169-192).
-__device__ __forceinline__ bool ff_geom(f3 sp, f3 sn, float4 cp, float4 cn, float& r, f3& dir, float& ff) {
-    f3 d = ld3(cp) - sp;
-    float r2 = dot3(d, d);
-    if (r2 < 1e-9f) return false;
-    r = sqrtf(r2);
-    dir = d / r;
-    float cos_i = dot3(sn, dir);
-    float cos_c = dot3(ld3(cn), -dir);
-    if (cos_i <= 1e-3f || cos_c <= 1e-3f) return false;
-    float area = cp.w;
-    ff = cos_i * cos_c * area / (kPi * r2 + area);
-    return !(ff < 2e-5f);
-}
-
 // ====================================================================== probes
 template <bool LDS>
 __global__ __launch_bounds__(kBlock) void k_probe_dist(SceneView Sg, LightsView Lg, LdsPlan P, const float* pts,
@@ -218,14 +195,6 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
     }
 }
 
-// 15-bit Morton code (5 bits per axis) of each surfel's cell in the scene
-// box; it only orders work (coherent wavefronts), never enters a result.
-__device__ __forceinline__ uint32_t spread5(uint32_t v) {
-    uint32_t r = 0;
-    for (int b = 0; b < 5; ++b) r |= ((v >> b) & 1u) << (3 * b);
-    return r;
-}
-
 __global__ void k_morton(const float4* __restrict__ spos, uint32_t N, f3 lo, f3 scale, uint32_t* code,
                          uint32_t* iota) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -235,38 +204,6 @@ __global__ void k_morton(const float4* __restrict__ spos, uint32_t N, f3 lo, f3 
              z = (uint32_t)clampi(sat_i32(q.z), 0, 31);
     code[i] = spread5(x) | (spread5(y) << 1) | (spread5(z) << 2);
     iota[i] = i;
-}
-
-// Wavefront-private slice of a global work queue.  A wave takes kChunk
-// consecutive items per atomic and hands them to its idle lanes in order;
-// all fields are wave-uniform.
-struct WaveQueue {
-    uint32_t next, end, chunk;
-    bool drained;
-};
-
-__device__ __forceinline__ bool refill(bool live, WaveQueue& wq, uint32_t* queue, uint32_t total, uint32_t& item) {
-    uint64_t idle = __ballot(!live);
-    if (!idle) return false;
-    if (wq.next >= wq.end) {
-        if (wq.drained) return false;
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(queue, wq.chunk);
-        base = __shfl(base, 0, 64);
-        if (base >= total) {
-            wq.drained = true;
-            return false;
-        }
-        wq.next = base;
-        wq.end = total - base < wq.chunk ? total : base + wq.chunk;
-    }
-    uint32_t nidle = (uint32_t)__popcll(idle);
-    uint32_t avail = wq.end - wq.next;
-    uint32_t take = nidle < avail ? nidle : avail;
-    uint32_t rank = lanes_below(idle);
-    item = wq.next + rank;
-    wq.next += take;
-    return !live && rank < take;
 }
 
 // Cone-traced link visibility (src/formfactor.rs:194-214) as a persistent
@@ -582,71 +519,6 @@ __global__ void k_grid_keys(const float4* __restrict__ spos, uint32_t N, f3 lo, 
     atomicAdd(&counts[c], 1u);
 }
 
-// One pixel of render_frame (src/main.rs:247-361).
-template <class Stk>
-__device__ __forceinline__ f3 shade_pixel(const SceneView& S, const LightsView& L, const ShadeArgs& A, uint32_t x, uint32_t y,
-                          uint32_t& local, Stk& stk) {
-    const f3 bg = mk(0.02f, 0.03f, 0.05f);
-    const CamV& cam = A.cam;
-    float px = (2.0f * ((float)x + 0.5f) / (float)A.W - 1.0f) * cam.aspect * cam.tan_half;
-    float py = (1.0f - 2.0f * ((float)y + 0.5f) / (float)A.H) * cam.tan_half;
-    f3 rd = nrm3((cam.fwd + cam.right * px) + cam.up * py);
-    uint32_t steps = 0;
-    Hit hh = trace<true>(S, cam.origin, rd, 1e-3f, A.tmax, steps, stk);
-    local += steps;
-    float surf_t = hh.hit ? hh.t : kFltMax;
-    bool tonemapped = A.mode <= M_INDIRECT;
-    f3 e;
-    if (emitter_hit(L, cam.origin, rd, fmn(surf_t, A.tmax), e)) return tonemapped ? e : spl(0.9f);
-    if (!hh.hit) return bg;
-    if (A.mode == M_NORMALS) return hh.n * 0.5f + spl(0.5f);
-    if (A.mode == M_DEPTH) {
-        float d = 1.0f - clmp(hh.t / A.tmax, 0.0f, 1.0f);
-        return spl(d * d);
-    }
-    if (A.mode == M_STEPS) {
-        float f = (float)steps / (float)kMaxSteps;
-        return mk(f, 1.0f - f, 0.2f);
-    }
-    MatV m = load_mat(L, hh.mat);
-    f3 c = spl(0.0f);
-    if (A.mode != M_INDIRECT) {
-        uint32_t s = 0;
-        f3 d = direct(S, L, hh.p, hh.n, -rd, m, s, stk);
-        local += s;
-        c = c + (d + m.emissive);
-    }
-    if (A.use_gi) {
-        f3 irr = gather(A.grid, A.spos, A.snrm, A.irr, hh.p, hh.n, A.radius);
-        c = c + m.kd * irr;
-        if (m.roughness < 0.45f || m.metallic > 0.05f) {
-            f3 refl = rd - hh.n * (2.0f * dot3(hh.n, rd));
-            uint32_t s = 0;
-            Hit rh = trace<true>(S, hh.p + hh.n * 1e-3f, refl, 1e-3f, A.tmax, s, stk);
-            local += s;
-            float rt = rh.hit ? rh.t : kFltMax;
-            f3 refl_col, e2;
-            if (emitter_hit(L, hh.p, refl, fmn(rt, A.tmax), e2)) {
-                refl_col = e2;
-            } else if (rh.hit) {
-                MatV m2 = load_mat(L, rh.mat);
-                uint32_t s2 = 0;
-                f3 d2 = direct(S, L, rh.p, rh.n, -refl, m2, s2, stk);
-                local += s2;
-                f3 i2 = gather(A.grid, A.spos, A.snrm, A.irr, rh.p, rh.n, A.radius);
-                refl_col = (d2 + m2.kd * i2) + m2.emissive;
-            } else {
-                refl_col = bg;
-            }
-            f3 f = fresnel_schlick(fmx(dot3(hh.n, -rd), 0.0f), m.f0);
-            float gloss = clmp(1.0f - m.roughness / 0.45f, 0.0f, 1.0f);
-            gloss = gloss * gloss;
-            c = c + (f * refl_col) * gloss;
-        }
-    }
-    return c;
-}
-
 // 16x16 pixel blocks, one 8x8 tile per wavefront.
 template <bool COUNT, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneView Sg, LightsView Lg, LdsPlan P, ShadeArgs A, float4* film,
@@ -664,34 +536,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView Sg, LightsView Lg, L
     stk_flush(stk, ops);
 }
 
-// Film::encode: exposure, ACES fit, sRGB through host-built thresholds
-// (thr[k-1] = least f32 that the sRGB transfer maps to >= k).
-__device__ __forceinline__ uint8_t srgb_lut(float c, const float* __restrict__ thr) {
-    c = clmp(c, 0.0f, 1.0f);
-    uint32_t lo = 0, hi = 255;  // count of thresholds <= c
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (c >= thr[mid])
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return (uint8_t)lo;
-}
-
 __global__ void k_encode(const float4* __restrict__ film, uint32_t npx, float exposure, int32_t tonemap,
                          const float* __restrict__ thr, uint8_t* out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npx) return;
-    f3 x = ld3(film[i]) * exposure;
-    if (tonemap) {
-        f3 num = x * (x * 2.51f + spl(0.03f));
-        f3 den = x * (x * 2.43f + spl(0.59f)) + spl(0.14f);
-        x = min3(max3(num / den, spl(0.0f)), spl(1.0f));
-    }
-    out[3 * i] = srgb_lut(x.x, thr);
-    out[3 * i + 1] = srgb_lut(x.y, thr);
-    out[3 * i + 2] = srgb_lut(x.z, thr);
+    encode_px(film[i], exposure, tonemap, thr, out + 3ull * i);
 }
 
 // Packed xyz triples <-> float4 records, so host copies stay contiguous.

@@ -135,14 +135,16 @@ def stress(seed=0x5EED, prims=256, surfels=200000, width=1920, height=1080, boun
     return "\n".join(L) + "\n"
 
 
-def batched_scene(j):
+def batched_scene(j, surfel_div=1, side_div=1):
     """Scene j of the 64-scene batch: 8..256 primitives, 5k..100k surfels,
-    square film 256 + 64 * U{0..12}; otherwise as `stress`."""
+    square film 256 + 64 * U{0..12}; otherwise as `stress`.  The divisors
+    shrink surfels and film for quick checks (same geometry)."""
     r = Pcg32(1000 + j)
     prims = 8 + r.below(249)
     surfels = 5000 + r.below(95001)
     side = 256 + 64 * r.below(13)
-    return stress(seed=1000 + j, prims=prims, surfels=surfels, width=side, height=side)
+    return stress(seed=1000 + j, prims=prims, surfels=max(1, surfels // surfel_div), width=max(8, side // side_div),
+                  height=max(8, side // side_div))
 
 
 # --------------------------------------------------------------------------- E: differentiable cornell

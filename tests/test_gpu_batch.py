@@ -1,0 +1,133 @@
+"""GPU batches: many scenes per launch against separate single-scene runs.
+
+Every scene of a batch must reproduce, bit for bit, what a single-scene
+context computes from the same scene text and surfels: the CSR (indices and
+weight bits), cluster / candidate / link counts, evaluation counts, the
+solve's iterations and residual, radiosity, the linear film and the RGB8
+image.  Graph replays and every launch shape must reproduce the eager frame.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import interflect_amd as ia
+from conftest import scene_text
+from interflect_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def batch_texts():
+    t = [scenes.batched_scene(j, surfel_div=20, side_div=4) for j in (0, 1, 2, 5)]
+    t.append(scenes.diff_cornell(3, surfels=3000, width=72, height=56))
+    p = scene_text("product.rad")  # quad lights, CSG, chrome (mirror branch)
+    t.append(p.replace("width: 640", "width: 96").replace("height: 480", "height: 72")
+             .replace("surfels: 20000", "surfels: 4000"))
+    t.append(scenes.product_literal(width=80, height=60, surfels=4000))
+    return t
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def batch():
+    texts = batch_texts()
+    say("batch: %d scenes" % len(texts))
+    b = ia.Batch(0)
+    for t in texts:
+        b.add_scene(t, 8)
+    b.generate_surfels()
+    ms = b.run()
+    say("batch eager frame %.1f ms" % ms)
+    return b, texts
+
+
+def single(text):
+    ctx = ia.Context(0)
+    ctx.load_scene(text)
+    ctx.generate_surfels()
+    tr = ctx.transfer(8)
+    so = ctx.solve()
+    sh = ctx.shade("beauty")
+    return ctx, tr, so, sh
+
+
+def test_batch_matches_single_scene_runs(batch):
+    b, texts = batch
+    for s, text in enumerate(texts):
+        say("batch scene", s)
+        ctx, tr, so, sh = single(text)
+        st = b.stats(s)
+        assert st["surfels"] == ctx.num_surfels()
+        assert st["clusters"] == tr["clusters"]
+        assert st["candidates"] == tr["candidates"]
+        assert st["links"] == tr["links"]
+        assert st["evals_transfer"] == tr["evals"]
+        assert st["iterations"] == so["iterations"]
+        assert bits(st["residual"]) == bits(so["residual"])
+        assert bits(st["mean_row_sum"]) == bits(tr["mean_row_sum"])
+        assert st["evals_solve"] == so["evals"]
+        assert st["evals_shade"] == sh["evals"]
+        starts, idx, w = b.transfer_arrays(s)
+        s1, i1, w1, _, _ = ctx.transfer_arrays()
+        assert np.array_equal(starts, s1)
+        assert np.array_equal(idx, i1)
+        assert np.array_equal(bits(w), bits(w1))
+        for a, c in zip(b.radiosity(s), ctx.radiosity()):
+            assert np.array_equal(bits(a), bits(c))
+        img = b.image(s)
+        assert np.array_equal(bits(img["film"]), bits(sh["film"]))
+        assert np.array_equal(img["rgb8"], sh["rgb8"])
+
+
+def test_graph_replays_reproduce_the_eager_frame(batch):
+    b, texts = batch
+    ref = [b.image(s) for s in range(len(texts))]
+    b.set_graph(True)
+    for rep in range(3):
+        ms = b.run()
+        say("graph replay %d: %.1f ms" % (rep, ms))
+        junk = np.random.default_rng(rep).random((256, 256)) @ np.random.default_rng(rep + 1).random((256, 256))
+        assert junk.shape == (256, 256)
+        for s in range(len(texts)):
+            img = b.image(s)
+            assert np.array_equal(bits(img["film"]), bits(ref[s]["film"]))
+            assert np.array_equal(img["rgb8"], ref[s]["rgb8"])
+    b.set_graph(False)
+
+
+def test_launch_shapes_and_autotune_reproduce_the_frame(batch):
+    b, texts = batch
+    ref = [b.image(s)["rgb8"] for s in range(len(texts))]
+    rad = [b.radiosity(s)[2] for s in range(len(texts))]
+    for shape in ((4, 16, 1), (16, 4, 0), (1, 1, 1)):
+        b.set_tuning(*shape)
+        b.run()
+        for s in range(len(texts)):
+            assert np.array_equal(b.image(s)["rgb8"], ref[s])
+            assert np.array_equal(bits(b.radiosity(s)[2]), bits(rad[s]))
+    best = b.autotune()
+    say("autotune best %.1f ms" % best)
+    for s in range(len(texts)):
+        assert np.array_equal(b.image(s)["rgb8"], ref[s])
+
+
+def test_batch_errors():
+    b = ia.Batch(0)
+    with pytest.raises(ia.IfxError) as e:
+        b.run()
+    assert e.value.status == -3
+    with pytest.raises(ia.IfxError) as e:
+        b.add_scene("sphere { center: [0, 0, 0] radius: oops }")
+    assert e.value.status == -2
+    with pytest.raises(ia.IfxError) as e:
+        b.add_scene(scenes.cornell_unit(), 5000)
+    assert e.value.status == -1
