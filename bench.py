@@ -15,7 +15,10 @@ adds the 1920x1080 seeded stress scene as a secondary measurement, and
 `--also diff` the gradient workload: grads/s of the implicit adjoint
 (forward solve + beauty shade + backward to 24 albedo and 3 emission
 parameters) on the 4.4-unit Cornell room with 8 materials and 3 quad lights,
-50000 surfels, 512x512, geometry frozen.
+50000 surfels, 512x512, geometry frozen.  `--also batch` measures the
+64-scene batch (scenes of 8..256 primitives, 5k..100k surfels, films 256^2
+to 1024^2): frames/s of one batched frame (autotuned launch shapes, HIP
+graph replay) against the same 64 frames run one scene at a time.
 """
 import argparse
 import json
@@ -324,6 +327,65 @@ def measure_grads(steps, warmup, with_cpu):
     return out
 
 
+def measure_batch(steps, n_scenes=64, surfel_div=1, side_div=1):
+    """Batched frames of `n_scenes` seeded scenes vs sequential single-scene
+    frames (the divisors shrink surfels and films of the same scenes)."""
+    import numpy as np
+
+    import interflect_amd as ia
+    from interflect_amd import scenes
+
+    texts = [scenes.batched_scene(j, surfel_div, side_div) for j in range(n_scenes)]
+    b = ia.Batch(0)
+    for t in texts:
+        b.add_scene(t, 8)
+    note("batch", "placing surfels for %d scenes" % n_scenes)
+    b.generate_surfels()
+    note("batch", "first eager frame")
+    first = b.run()
+    note("batch", "eager frame %.0f ms; autotuning" % first)
+    best = b.autotune()
+    note("batch", "autotuned eager frame %.0f ms; graph capture" % best)
+    b.set_graph(True)
+    b.run()  # capture + first replay
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+        note("batch", "graph replay done")
+    batched_ms = (time.perf_counter() - t0) * 1e3 / steps
+    b.set_graph(False)
+    stats = [b.stats(s) for s in range(n_scenes)]
+    evals = sum(st["evals_transfer"] + st["evals_solve"] + st["evals_shade"] for st in stats)
+    # Sequential: one context, one scene at a time; surfel placement is
+    # outside the timed region, as in the batch.
+    ctx = ia.Context(0)
+    seq_ms = 0.0
+    same = 0
+    for j, t in enumerate(texts):
+        ctx.load_scene(t)
+        ctx.generate_surfels()
+        ctx.transfer(8)  # warm the buffers of this size
+        t1 = time.perf_counter()
+        ctx.transfer(8)
+        ctx.solve()
+        r = ctx.shade("beauty", film=False, rgb8=True)
+        seq_ms += (time.perf_counter() - t1) * 1e3
+        same += int(np.array_equal(r["rgb8"], b.image(j, film=False)["rgb8"]))
+        if j % 8 == 7:
+            note("batch", "sequential %d/%d scenes" % (j + 1, n_scenes))
+    return {"value": 1e3 / batched_ms, "unit": "batched frames/s", "ms_per_step": batched_ms,
+            "sequential_ms": seq_ms, "speedup_vs_sequential": seq_ms / batched_ms,
+            "first_eager_ms": first, "autotuned_eager_ms": best, "gsdf_evals_per_s": evals / batched_ms / 1e6,
+            "images_identical_to_single_scene": "%d/%d" % (same, n_scenes),
+            "config": {"workload": "%d seeded scenes (seeds 1000+j): 8..256 random primitives, 5k..100k surfels "
+                                   "/ %d, films 256^2..1024^2 / %d^2, as the stress scene otherwise; cluster res 8; "
+                                   "beauty + RGB8" % (n_scenes, surfel_div, side_div),
+                       "surfels": int(sum(st["surfels"] for st in stats)),
+                       "links": int(sum(st["links"] for st in stats)),
+                       "candidates": int(sum(st["candidates"] for st in stats)),
+                       "pixels": int(sum(st["width"] * st["height"] for st in stats))}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,6 +420,10 @@ def main():
     for extra in [x for x in args.also.split(",") if x]:
         if extra == "diff":
             line.setdefault("secondary", {})[extra] = measure_grads(args.also_steps, 1, not args.no_cpu)
+        elif extra == "batch":
+            line.setdefault("secondary", {})[extra] = measure_batch(args.also_steps)
+        elif extra == "batch_small":
+            line.setdefault("secondary", {})[extra] = measure_batch(max(3, args.also_steps), 64, 20, 4)
         else:
             line.setdefault("secondary", {})[extra] = measure(extra, args.also_steps, 1, not args.no_cpu)
     print(json.dumps(line))

@@ -210,7 +210,8 @@ ifx_status ifx_batch_add_scene(ifx_batch* b, const char* rad_text, int32_t clust
 /* surfel::generate for every scene (render.surfels requested), on the host. */
 ifx_status ifx_batch_generate_surfels(ifx_batch* b);
 ifx_status ifx_batch_set_graph(ifx_batch* b, int32_t on);
-/* Launch shapes: persistent blocks per CU, queue chunk divisor, Jacobi row mapping (0 lane, 1 wavefront). */
+/* Launch shapes: persistent blocks per CU of the tracing kernels (1..64), wavefront queue chunk
+ * divisor, Jacobi row mapping (0 lane, 1 wavefront).  Results do not depend on them. */
 ifx_status ifx_batch_set_tuning(ifx_batch* b, uint32_t blocks_per_cu, uint32_t chunk_div, uint32_t wave_rows);
 /* Times one eager frame per candidate launch shape and keeps the fastest. */
 ifx_status ifx_batch_autotune(ifx_batch* b, float* best_ms);

@@ -4,12 +4,13 @@
 // concatenated device arrays addressed through per-scene offsets.  Every
 // stage is one launch (or one launch per bounce) over all scenes: per-item
 // kernels find their scene from a per-surfel / per-cluster scene index, and
-// the tracing kernels split their grid into per-scene block ranges (sized by
-// each scene's work) so a block stages one scene in LDS and takes work from
-// that scene's queue.  After one eager frame has fixed the sizes, a frame can
-// be captured into a HIP graph and replayed; the launch shapes (persistent
-// blocks per CU, queue chunk divisor, row mapping of the Jacobi gather) are
-// picked by a small autotuner.  Per-scene results are identical to separate
+// the persistent tracing kernels walk a list of scenes: a block stages one
+// scene in LDS and its lanes refill from that scene's queue until it drains,
+// then the block moves on (one-item-per-lane kernels split their grid into
+// per-scene block ranges instead).  After one eager frame has fixed the
+// sizes, a frame can be captured into a HIP graph and replayed; the launch
+// shapes (persistent blocks per CU, queue chunk divisor, row mapping of the
+// Jacobi gather) are picked by a small autotuner.  Per-scene results are identical to separate
 // single-scene runs.
 #pragma once
 #include <string>
@@ -27,9 +28,9 @@ struct BatchSceneStats {
 };
 
 struct BatchTuning {
-    uint32_t blocks_per_cu = 8;   // persistent blocks per CU of the tracing kernels
-    uint32_t chunk_div = 8;       // queue chunk = even share per wavefront / chunk_div (64..256)
-    uint32_t wave_rows = 0;       // Jacobi gather: 0 lane per row, 1 wavefront per row
+    uint32_t blocks_per_cu = 8;  // persistent blocks per CU of the closure / visibility kernels
+    uint32_t chunk_div = 8;      // queue chunk = even share per wavefront / chunk_div, 64..256 items
+    uint32_t wave_rows = 0;      // Jacobi gather: 0 lane per row, 1 wavefront per row
 };
 
 class Batch {
@@ -68,6 +69,7 @@ class Batch {
     int finish_stats(std::string& err);
     void drop_graph();
     int check(hipError_t e, const char* what, std::string& err);
+    uint32_t persist_grid() const;
 
     Engine eng_;  // device, stream and scene-independent tables
     hipStream_t stream_ = nullptr;
@@ -80,15 +82,16 @@ class Batch {
     // host offsets
     std::vector<uint32_t> s0_, px0_, term0_, cell0_, cl0_, C_, cand0_, M_, nl_;
     uint32_t Ntot_ = 0, Ptot_ = 0, Ttot_ = 0, cells_ = 0, nseg_ = 0, Ctot_ = 0, Mtot_ = 0, Ltot_ = 0, Bmax_ = 0;
-    std::vector<uint32_t> blk_closure_, blk_vis_, blk_direct_, blk_shade_, blk_encode_;
-    size_t lds_n_ = 0, lds_l_ = 0;
+    // block partitions per LDS footprint class (see batch.hip) and their LDS bytes
+    std::vector<uint32_t> blk_closure_[4], blk_vis_[4], blk_direct_[4], blk_shade_[4], blk_encode_;
+    size_t lds_n_[4] = {16, 16, 16, 16}, lds_l_[4] = {16, 16, 16, 16};
 
     // device
     DevBuf d_rec, d_slots, d_blk, d_ssid, d_spos, d_snrm, d_salb;
     DevBuf d_ckey, d_ckey2, d_cval, d_cval2, d_cflag, d_cseg, d_cbegin, d_craw_pos, d_craw_nrm, d_cvalid, d_cidx;
     DevBuf d_cpos, d_cnrm, d_crange, d_csid, d_ccount;
     DevBuf d_code, d_mkey, d_mkey2, d_mval, d_rorder;
-    DevBuf d_open, d_sky, d_queues, d_counters;
+    DevBuf d_open, d_sky, d_counters, d_queues;
     DevBuf d_fcount, d_cstart, d_cand_ci, d_cand_row, d_cand_v, d_vkey, d_vkey2, d_vval, d_vorder;
     DevBuf d_lcount, d_lstart, d_scale, d_rowsum, d_idx, d_w;
     DevBuf d_terms, d_gterm, d_direct, d_rad, d_irr, d_bc, d_resid;

@@ -151,95 +151,138 @@ __global__ __launch_bounds__(kBlock) void kb_ff(const float4* __restrict__ spos,
 }
 
 // ---------------------------------------------------------------- tracing (block-partitioned)
-// Hemisphere closure (k_closure): scene queue, direction-major over the
-// scene's Morton-ordered rows.
-__global__ __launch_bounds__(kBlock) void kb_closure(const BSlot* __restrict__ slots, const uint32_t* __restrict__ blk,
-                                                     uint32_t ns, const float4* __restrict__ spos,
+// Per-scene setup inside a persistent block: views, LDS staging when the
+// scene fits, stack placement.  Every thread of the block calls it.
+__device__ __forceinline__ void bscene(const BSlot& T, float4* lds, SceneView& S, LightsView& L, LdsStack& stk) {
+    S = T.S;
+    L = T.L;
+    const LdsPlan P = T.Pn;
+    if (T.staged_n) stage_in<true, false>(lds, S, L);
+    stk.dbase = (float*)(lds + P.scene_f4 + P.light_f4) + threadIdx.x;
+    stk.base = (uint16_t*)(stk.dbase - threadIdx.x + P.stack_depth * kBlock) + threadIdx.x;
+    stk.stride = (uint32_t)kBlock;
+}
+
+// Whether scene `sid`'s queue still has items (read by thread 0, the same
+// answer for the whole block).
+__device__ __forceinline__ bool scene_open(const uint32_t* queue, uint32_t total, uint32_t* flag) {
+    __syncthreads();  // the previous scene's LDS contents are no longer read
+    if (threadIdx.x == 0) *flag = __hip_atomic_load(queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < total;
+    __syncthreads();
+    return *flag != 0;
+}
+
+// Hemisphere closure (k_closure) as a persistent kernel over a list of
+// scenes: every block takes the scenes in list order, stages one, and its
+// lanes refill from that scene's queue (direction-major over the scene's
+// Morton-ordered rows) until it is drained; blocks that finish a scene move
+// on while others finish its last rays.
+__global__ __launch_bounds__(kBlock) void kb_closure(const BSlot* __restrict__ slots, const uint32_t* __restrict__ list,
+                                                     uint32_t nlist, const float4* __restrict__ spos,
                                                      const float4* __restrict__ snrm,
                                                      const uint32_t* __restrict__ rorder,
                                                      const float4* __restrict__ dirs, uint32_t* open,
                                                      uint32_t* queues, unsigned long long* evals) {
-    IFX_BSETUP(false);
-    const uint32_t N = T.N, s0 = T.s0;
-    const float tmax = T.diag * 2.0f;
-    uint32_t* queue = queues + 4 * sid + 1;
-    uint32_t acc = 0;
-    bool live = false;
-    WaveQueue wq{0u, 0u, T.chunk_c, false};
-    uint32_t row = 0;
-    TraceMarch m;
-    m.begin(spl(0.0f), spl(0.0f), 0.0f, 0.0f);
-    const uint32_t total = 16u * N;
-    for (;;) {
-        uint32_t q;
-        if (refill(live, wq, queue, total, q)) {
-            uint32_t k = q / N;
-            row = rorder[s0 + (q - k * N)];
-            f3 p = ld3(spos[row]), n = ld3(snrm[row]);
-            f3 t, b;
-            frame(n, t, b);
-            float4 D = dirs[k];
-            f3 d = (t * D.x + b * D.y) + n * D.z;
-            m.begin(p + n * 1e-3f, nrm3(d), 1e-3f, tmax);
-            live = true;
-        }
-        if (__ballot(live) == 0) break;
-        if (live) {
-            DistHit h;
-            int st = m.step(S, h, stk);
-            if (st) {
-                if (st == 2) atomicAdd(&open[row], 1u);
-                acc += m.steps;
-                live = false;
+    extern __shared__ float4 ifx_lds[];
+    __shared__ uint32_t flag;
+    for (uint32_t li = 0; li < nlist; ++li) {
+        const uint32_t sid = list[li];
+        const BSlot& T = slots[sid];
+        const uint32_t N = T.N, s0 = T.s0, total = 16u * N;
+        uint32_t* queue = queues + 4 * sid + 1;
+        if (!scene_open(queue, total, &flag)) continue;
+        SceneView S;
+        LightsView L;
+        LdsStack stk;
+        bscene(T, ifx_lds, S, L, stk);
+        const float tmax = T.diag * 2.0f;
+        uint32_t acc = 0;
+        bool live = false;
+        WaveQueue wq{0u, 0u, T.chunk_c, false};
+        uint32_t row = 0;
+        TraceMarch m;
+        m.begin(spl(0.0f), spl(0.0f), 0.0f, 0.0f);
+        for (;;) {
+            uint32_t q;
+            if (refill(live, wq, queue, total, q)) {
+                uint32_t k = q / N;
+                row = rorder[s0 + (q - k * N)];
+                f3 p = ld3(spos[row]), n = ld3(snrm[row]);
+                f3 t, b;
+                frame(n, t, b);
+                float4 D = dirs[k];
+                f3 d = (t * D.x + b * D.y) + n * D.z;
+                m.begin(p + n * 1e-3f, nrm3(d), 1e-3f, tmax);
+                live = true;
+            }
+            if (__ballot(live) == 0) break;
+            if (live) {
+                DistHit h;
+                int st = m.step(S, h, stk);
+                if (st) {
+                    if (st == 2) atomicAdd(&open[row], 1u);
+                    acc += m.steps;
+                    live = false;
+                }
             }
         }
+        count_steps(evals + 4 * sid, acc);
     }
-    count_steps(evals + 4 * sid, acc);
 }
 
-// Link visibility (k_visibility): scene queue over the scene's segment of
-// the cluster-major candidate order.
+// Link visibility (k_visibility) over each scene's segment of the
+// cluster-major candidate order, persistent over the scene list as kb_closure.
 __global__ __launch_bounds__(kBlock) void kb_visibility(
-    const BSlot* __restrict__ slots, const uint32_t* __restrict__ blk, uint32_t ns, const float4* __restrict__ spos,
-    const float4* __restrict__ snrm, const float4* __restrict__ cpos, const float4* __restrict__ cnrm,
-    const uint32_t* __restrict__ cand_ci, const uint32_t* __restrict__ cand_row, const uint32_t* __restrict__ order,
-    float* cand_v, uint32_t* queues, unsigned long long* evals) {
-    IFX_BSETUP(false);
-    const uint32_t M = T.M, c0 = T.cand0;
-    const float off = T.diag * 5e-4f, tmin = T.diag * 1e-3f;
-    uint32_t* queue = queues + 4 * sid;
-    uint32_t acc = 0;
-    bool live = false;
-    WaveQueue wq{0u, 0u, T.chunk_v, false};
-    uint32_t slot = 0;
-    float ff = 0.0f;
-    ShadowMarch m;
-    m.begin(spl(0.0f), spl(0.0f), 0.0f, 0.0f, 0.0f);
-    for (;;) {
-        uint32_t q;
-        if (refill(live, wq, queue, M, q)) {
-            slot = order[c0 + q];
-            uint32_t i = cand_row[slot], c = cand_ci[slot];
-            f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
-            float4 CN = cnrm[c];
-            float r = 0.0f;
-            f3 dir = spl(0.0f);
-            ff_geom(sp, sn, cpos[c], CN, r, dir, ff);
-            float kk = clmp(r / fmx(CN.w, 1e-4f), 1.5f, 32.0f);
-            m.begin(sp + sn * off, dir, tmin, r - CN.w * 0.75f, kk);
-            live = true;
-        }
-        if (__ballot(live) == 0) break;
-        if (live) {
-            float vis;
-            if (m.step(S, vis, stk)) {
-                cand_v[slot] = vis <= 1e-3f ? 0.0f : ff * vis;
-                acc += m.steps;
-                live = false;
+    const BSlot* __restrict__ slots, const uint32_t* __restrict__ list, uint32_t nlist,
+    const float4* __restrict__ spos, const float4* __restrict__ snrm, const float4* __restrict__ cpos,
+    const float4* __restrict__ cnrm, const uint32_t* __restrict__ cand_ci, const uint32_t* __restrict__ cand_row,
+    const uint32_t* __restrict__ order, float* cand_v, uint32_t* queues, unsigned long long* evals) {
+    extern __shared__ float4 ifx_lds[];
+    __shared__ uint32_t flag;
+    for (uint32_t li = 0; li < nlist; ++li) {
+        const uint32_t sid = list[li];
+        const BSlot& T = slots[sid];
+        const uint32_t M = T.M, c0 = T.cand0;
+        uint32_t* queue = queues + 4 * sid;
+        if (!scene_open(queue, M, &flag)) continue;
+        SceneView S;
+        LightsView L;
+        LdsStack stk;
+        bscene(T, ifx_lds, S, L, stk);
+        const float off = T.diag * 5e-4f, tmin = T.diag * 1e-3f;
+        uint32_t acc = 0;
+        bool live = false;
+        WaveQueue wq{0u, 0u, T.chunk_v, false};
+        uint32_t slot = 0;
+        float ff = 0.0f;
+        ShadowMarch m;
+        m.begin(spl(0.0f), spl(0.0f), 0.0f, 0.0f, 0.0f);
+        for (;;) {
+            uint32_t q;
+            if (refill(live, wq, queue, M, q)) {
+                slot = order[c0 + q];
+                uint32_t i = cand_row[slot], c = cand_ci[slot];
+                f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
+                float4 CN = cnrm[c];
+                float r = 0.0f;
+                f3 dir = spl(0.0f);
+                ff_geom(sp, sn, cpos[c], CN, r, dir, ff);
+                float kk = clmp(r / fmx(CN.w, 1e-4f), 1.5f, 32.0f);
+                m.begin(sp + sn * off, dir, tmin, r - CN.w * 0.75f, kk);
+                live = true;
+            }
+            if (__ballot(live) == 0) break;
+            if (live) {
+                float vis;
+                if (m.step(S, vis, stk)) {
+                    cand_v[slot] = vis <= 1e-3f ? 0.0f : ff * vis;
+                    acc += m.steps;
+                    live = false;
+                }
             }
         }
+        count_steps(evals + 4 * sid, acc);
     }
-    count_steps(evals + 4 * sid, acc);
 }
 
 // Direct light terms (k_direct_terms), one lane per (light, row).
@@ -427,34 +470,13 @@ __global__ void kb_grid_keys(const float4* __restrict__ spos, const uint32_t* __
     atomicAdd(&counts[k], 1u);
 }
 
-uint32_t persist_chunk(uint64_t items, uint32_t blocks, uint32_t div) {
-    uint64_t waves = (uint64_t)blocks * (kBlock / 64);
-    uint64_t c = items / (waves * div + 1);
-    return (uint32_t)(c < 64 ? 64 : (c > 256 ? 256 : c));
-}
+// LDS footprint class of a block (16 KiB steps, the last open-ended).
+constexpr uint32_t kLdsClasses = 4;
+uint32_t lds_class(size_t bytes) { return (uint32_t)std::min<size_t>(kLdsClasses - 1, (bytes - 1) / 16384); }
 
-// Splits `total` persistent blocks over the scenes in proportion to their
-// work (each working scene gets at least one block and at most one block per
-// kBlock items); returns the prefix table.
-std::vector<uint32_t> split_blocks(const std::vector<uint64_t>& work, uint32_t total) {
-    uint64_t sum = 0;
-    for (uint64_t w : work) sum += w;
+std::vector<uint32_t> exact_blocks(const std::vector<uint64_t>& work, uint32_t per = kBlock) {
     std::vector<uint32_t> blk(work.size() + 1, 0);
-    for (size_t s = 0; s < work.size(); ++s) {
-        uint32_t b = 0;
-        if (work[s]) {
-            double share = sum ? (double)total * (double)work[s] / (double)sum : 1.0;
-            b = (uint32_t)std::max<double>(1.0, std::floor(share + 0.5));
-            b = std::min<uint32_t>(b, blocks_for(work[s], kBlock));
-        }
-        blk[s + 1] = blk[s] + b;
-    }
-    return blk;
-}
-
-std::vector<uint32_t> exact_blocks(const std::vector<uint64_t>& work) {
-    std::vector<uint32_t> blk(work.size() + 1, 0);
-    for (size_t s = 0; s < work.size(); ++s) blk[s + 1] = blk[s] + blocks_for(work[s], kBlock);
+    for (size_t s = 0; s < work.size(); ++s) blk[s + 1] = blk[s] + blocks_for(work[s], per);
     return blk;
 }
 
@@ -465,6 +487,8 @@ std::vector<uint32_t> exact_blocks(const std::vector<uint64_t>& work) {
         int rc_ = check((call), #call, err);           \
         if (rc_) return rc_;                           \
     } while (0)
+
+uint32_t Batch::persist_grid() const { return (uint32_t)eng_.n_cu() * std::max(1u, tuning.blocks_per_cu); }
 
 int Batch::check(hipError_t e, const char* what, std::string& err) {
     if (e == hipSuccess) return 0;
@@ -639,8 +663,7 @@ int Batch::build_slots(std::string& err) {
     std::vector<BSlot> hs(ns);
     const float4* rec = d_rec.as<float4>();
     uint64_t r = 0;
-    lds_n_ = lds_l_ = 16;
-    const uint32_t G = (uint32_t)eng_.n_cu() * std::max(1u, tuning.blocks_per_cu);
+    std::vector<size_t> bytes_n(ns), bytes_l(ns);
     std::vector<uint64_t> wc(ns), wv(ns), wd(ns), wp(ns), we(ns);
     for (uint32_t s = 0; s < ns; ++s) {
         const HostScene& sc = scenes_[s].sc;
@@ -679,8 +702,8 @@ int Batch::build_slots(std::string& err) {
         B.staged_l = bl <= kLdsBudget ? 1u : 0u;
         B.Pn = B.staged_n ? LdsPlan{scene_f4, 0u, depth} : LdsPlan{0u, 0u, depth};
         B.Pl = B.staged_l ? LdsPlan{scene_f4, light_f4, depth} : LdsPlan{0u, 0u, depth};
-        lds_n_ = std::max(lds_n_, B.staged_n ? bn : stack);
-        lds_l_ = std::max(lds_l_, B.staged_l ? bl : stack);
+        bytes_n[s] = B.staged_n ? bn : stack;
+        bytes_l[s] = B.staged_l ? bl : stack;
         // offsets and sizes
         const uint32_t N = (uint32_t)scenes_[s].surfels.size();
         B.s0 = s0_[s];
@@ -735,21 +758,63 @@ int Batch::build_slots(std::string& err) {
         wp[s] = (uint64_t)((A.W + 15) / 16) * ((A.H + 15) / 16) * kBlock;
         we[s] = (uint64_t)A.W * A.H;
     }
-    blk_closure_ = split_blocks(wc, G);
-    blk_vis_ = split_blocks(wv, G);
-    blk_direct_ = exact_blocks(wd);
-    blk_shade_ = exact_blocks(wp);
-    blk_encode_ = exact_blocks(we);
+    // LDS size classes: a launch allocates its largest block's LDS, so scenes
+    // are launched in classes of similar footprint and small scenes keep the
+    // occupancy they have alone.
+    for (uint32_t k = 0; k < kLdsClasses; ++k) lds_n_[k] = lds_l_[k] = 16;
     for (uint32_t s = 0; s < ns; ++s) {
-        hs[s].chunk_c = persist_chunk(wc[s], std::max(1u, blk_closure_[s + 1] - blk_closure_[s]), tuning.chunk_div);
-        hs[s].chunk_v = persist_chunk(wv[s], std::max(1u, blk_vis_[s + 1] - blk_vis_[s]), tuning.chunk_div);
+        size_t& a = lds_n_[lds_class(bytes_n[s])];
+        a = std::max(a, bytes_n[s]);
+        size_t& b = lds_l_[lds_class(bytes_l[s])];
+        b = std::max(b, bytes_l[s]);
+    }
+    auto classes = [&](const std::vector<size_t>& bytes, const std::vector<uint64_t>& work, uint32_t per,
+                       std::vector<uint32_t>* blk) {
+        for (uint32_t k = 0; k < kLdsClasses; ++k) {
+            std::vector<uint64_t> wk(ns, 0);
+            for (uint32_t s = 0; s < ns; ++s)
+                if (lds_class(bytes[s]) == k) wk[s] = work[s];
+            blk[k] = exact_blocks(wk, per);
+        }
+    };
+    // Scene lists per class of the persistent kernels (scenes with work, in order).
+    for (uint32_t k = 0; k < kLdsClasses; ++k) {
+        blk_closure_[k].clear();
+        blk_vis_[k].clear();
+        for (uint32_t s = 0; s < ns; ++s)
+            if (lds_class(bytes_n[s]) == k) {
+                if (wc[s]) blk_closure_[k].push_back(s);
+                if (wv[s]) blk_vis_[k].push_back(s);
+            }
+    }
+    classes(bytes_l, wd, kBlock, blk_direct_);
+    classes(bytes_l, wp, kBlock, blk_shade_);
+    blk_encode_ = exact_blocks(we);
+    // Queue chunk per wavefront: about 1/chunk_div of a wavefront's even share
+    // of the scene's items over the persistent grid, 64..256 items.
+    const uint32_t G = persist_grid();
+    auto chunk = [&](uint64_t items) {
+        uint64_t waves = (uint64_t)G * (kBlock / 64);
+        uint64_t c = items / (waves * std::max(1u, tuning.chunk_div) + 1);
+        return (uint32_t)(c < 64 ? 64 : (c > 256 ? 256 : c));
+    };
+    for (uint32_t s = 0; s < ns; ++s) {
+        hs[s].chunk_c = chunk(wc[s]);
+        hs[s].chunk_v = chunk(wv[s]);
     }
     BK(hipStreamSynchronize(stream_));  // kernels already queued read the previous slots
     BK(d_slots.ensure(sizeof(BSlot) * (ns + 1)));
-    BK(d_blk.ensure(4ull * 5 * (ns + 1)));
+    // d_blk: [closure, visibility, direct, shade][LDS class][ns + 1], then encode;
+    // closure / visibility hold scene lists (padded), the others block prefixes.
+    BK(d_blk.ensure(4ull * (4 * kLdsClasses + 1) * (ns + 1)));
     std::vector<uint32_t> hb;
-    for (auto* v : {&blk_closure_, &blk_vis_, &blk_direct_, &blk_shade_, &blk_encode_})
-        hb.insert(hb.end(), v->begin(), v->end());
+    for (auto* v : {blk_closure_, blk_vis_, blk_direct_, blk_shade_})
+        for (uint32_t k = 0; k < kLdsClasses; ++k) {
+            std::vector<uint32_t> row = v[k];
+            row.resize(ns + 1, 0u);
+            hb.insert(hb.end(), row.begin(), row.end());
+        }
+    hb.insert(hb.end(), blk_encode_.begin(), blk_encode_.end());
     BK(hipMemcpy(d_slots.p, hs.data(), sizeof(BSlot) * ns, hipMemcpyHostToDevice));
     BK(hipMemcpy(d_blk.p, hb.data(), 4ull * hb.size(), hipMemcpyHostToDevice));
     return 0;
@@ -809,21 +874,19 @@ int Batch::frame(bool eager, std::string& err) {
     };
     if (eager) {
         if (int rc = build_slots(err)) return rc;  // sizes of the last frame; refreshed below
-        BK(d_queues.ensure(16ull * ns));
         BK(d_counters.ensure(32ull * ns));
+        BK(d_queues.ensure(16ull * ns));
         BK(d_open.ensure(4ull * N + 4));
         BK(d_sky.ensure(4ull * N + 4));
     }
-    fill(d_queues, 4ull * ns, 0u);
     fill(d_counters, 8ull * ns, 0u);
+    fill(d_queues, 4ull * ns, 0u);
     fill(d_open, N, 0u);
     const BSlot* slots = d_slots.as<BSlot>();
     const uint32_t* blk = d_blk.as<uint32_t>();
-    const uint32_t* blk_c = blk;
-    const uint32_t* blk_v = blk + (ns + 1);
-    const uint32_t* blk_d = blk + 2 * (ns + 1);
-    const uint32_t* blk_s = blk + 3 * (ns + 1);
-    const uint32_t* blk_e = blk + 4 * (ns + 1);
+    // d_blk: [closure, visibility, direct, shade][LDS class][ns + 1], then encode
+    auto table = [&](uint32_t kernel, uint32_t k) { return blk + (size_t)(kernel * kLdsClasses + k) * (ns + 1); };
+    const uint32_t* blk_e = blk + (size_t)4 * kLdsClasses * (ns + 1);
     unsigned long long* ctr = d_counters.as<unsigned long long>();
     const float4* spos = d_spos.as<float4>();
     const float4* snrm = d_snrm.as<float4>();
@@ -931,10 +994,12 @@ int Batch::frame(bool eager, std::string& err) {
     const uint32_t* rorder = (rorder_alt_ ? d_rorder : d_mval).as<uint32_t>();
 
     // ---- closure (independent of the links)
-    const uint32_t gc = blk_closure_.back();
-    if (gc)
-        kb_closure<<<gc, kBlock, lds_n_, st>>>(slots, blk_c, ns, spos, snrm, rorder, eng_.closure_dirs(),
-                                               d_open.as<uint32_t>(), d_queues.as<uint32_t>(), ctr);
+    const uint32_t G = persist_grid();
+    for (uint32_t k = 0; k < kLdsClasses; ++k)
+        if (!blk_closure_[k].empty())
+            kb_closure<<<G, kBlock, lds_n_[k], st>>>(slots, table(0, k), (uint32_t)blk_closure_[k].size(), spos,
+                                                     snrm, rorder, eng_.closure_dirs(), d_open.as<uint32_t>(),
+                                                     d_queues.as<uint32_t>(), ctr);
 
     // ---- candidates
     if (eager) {
@@ -984,12 +1049,12 @@ int Batch::frame(bool eager, std::string& err) {
         vorder_alt_ = alt;
     }
     const uint32_t* vorder = (vorder_alt_ ? d_vorder : d_vval).as<uint32_t>();
-    const uint32_t gv = blk_vis_.back();
-    if (gv)
-        kb_visibility<<<gv, kBlock, lds_n_, st>>>(slots, blk_v, ns, spos, snrm, d_cpos.as<float4>(),
-                                                  d_cnrm.as<float4>(), d_cand_ci.as<uint32_t>(),
-                                                  d_cand_row.as<uint32_t>(), vorder, d_cand_v.as<float>(),
-                                                  d_queues.as<uint32_t>(), ctr);
+    for (uint32_t k = 0; k < kLdsClasses; ++k)
+        if (!blk_vis_[k].empty())
+            kb_visibility<<<G, kBlock, lds_n_[k], st>>>(slots, table(1, k), (uint32_t)blk_vis_[k].size(), spos, snrm,
+                                                        d_cpos.as<float4>(), d_cnrm.as<float4>(),
+                                                        d_cand_ci.as<uint32_t>(), d_cand_row.as<uint32_t>(), vorder,
+                                                        d_cand_v.as<float>(), d_queues.as<uint32_t>(), ctr);
 
     // ---- calibration, link scan, compaction
     if (eager) {
@@ -1029,9 +1094,9 @@ int Batch::frame(bool eager, std::string& err) {
         BK(d_resid.ensure(4ull * ns * Bmax_ + 4));
         if (int rc = build_slots(err)) return rc;  // irr pointer of the shading arguments
     }
-    const uint32_t gd = blk_direct_.back();
-    if (gd)
-        kb_direct_terms<<<gd, kBlock, lds_l_, st>>>(slots, blk_d, ns, spos, snrm, d_salb.as<float4>(), rorder,
+    for (uint32_t k = 0; k < kLdsClasses; ++k)
+        if (const uint32_t g = blk_direct_[k].back())
+        kb_direct_terms<<<g, kBlock, lds_l_[k], st>>>(slots, table(2, k), ns, spos, snrm, d_salb.as<float4>(), rorder,
                                                     d_terms.as<float4>(), d_gterm.as<float>(), ctr);
     if (N)
         kb_direct_sum<<<blocks_for(N, 256), 256, 0, st>>>(ssid, N, slots, d_terms.as<float4>(), d_direct.as<float4>(),
@@ -1119,8 +1184,9 @@ int Batch::frame(bool eager, std::string& err) {
             return -4;
         }
     }
-    const uint32_t gs = blk_shade_.back();
-    if (gs) kb_shade<<<gs, kBlock, lds_l_, st>>>(slots, blk_s, ns, d_film.as<float4>(), ctr);
+    for (uint32_t k = 0; k < kLdsClasses; ++k)
+        if (const uint32_t g = blk_shade_[k].back())
+            kb_shade<<<g, kBlock, lds_l_[k], st>>>(slots, table(3, k), ns, d_film.as<float4>(), ctr);
     const uint32_t ge = blk_encode_.back();
     if (ge)
         kb_encode<<<ge, 256, 0, st>>>(slots, blk_e, ns, d_film.as<float4>(), eng_.srgb_thresholds(),
@@ -1176,7 +1242,7 @@ int Batch::autotune(double* best_ms, std::string& err) {
     BatchTuning best = tuning;
     double best_t = 1e300;
     // Persistent blocks per CU x queue chunk divisor first, then the Jacobi
-    // row mapping at the best of those: six eager frames in all.
+    // row mapping at the best of those: seven eager frames in all.
     drop_graph();
     auto trial = [&](BatchTuning t) -> int {
         tuning = t;
@@ -1187,7 +1253,7 @@ int Batch::autotune(double* best_ms, std::string& err) {
         if (ms < best_t) best_t = ms, best = t;
         return 0;
     };
-    for (uint32_t a : {8u, 16u})
+    for (uint32_t a : {4u, 8u, 16u})
         for (uint32_t d : {4u, 16u})
             if (int rc = trial(BatchTuning{a, d, 0u})) return rc;
     if (int rc = trial(BatchTuning{best.blocks_per_cu, best.chunk_div, 1u})) return rc;
