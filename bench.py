@@ -391,16 +391,38 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="product", choices=["product", "product_rad", "cornell_unit", "stress"])
+    ap.add_argument("--config", default="product",
+                    choices=["product", "product_rad", "cornell_unit", "stress", "diff", "batch", "batch_small"])
     ap.add_argument("--also", default="", help="comma-separated secondary configs (e.g. stress)")
     ap.add_argument("--also-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run only a plain device copy of MIB MiB (HBM-counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this path has no multi-GPU mode (see DESIGN.md, sharding)",
               file=sys.stderr)
         sys.exit(2)
 
+    if args.config in ("diff", "batch", "batch_small"):
+        # one secondary workload on its own (profiling runs)
+        if args.config == "diff":
+            m = measure_grads(args.steps, args.warmup, not args.no_cpu)
+        else:
+            m = measure_batch(args.steps, 64, *((20, 4) if args.config == "batch_small" else (1, 1)))
+        print(json.dumps(dict(m, metric=METRIC, n_gpus=1, steps=args.steps, warmup=args.warmup)))
+        return
     m = measure(args.config, args.steps, args.warmup, not args.no_cpu)
     line = {
         "metric": METRIC,
