@@ -139,7 +139,20 @@ def measure(name, steps, warmup, with_cpu):
     note(name, "placing surfels")
     ctx = ia.Context(0)
     info = ctx.load_scene(cfg["text"])
+    # Surfel placement (outside the frame, as in the reference's stage
+    # timers): on the device, then the host path for comparison.
+    ctx.generate_surfels()  # warm-up (first launches, allocations)
+    t0 = time.perf_counter()
     placed = ctx.generate_surfels()
+    place_gpu = (time.perf_counter() - t0) * 1e3
+    ctx.set_tuning("surfels_gpu", 0)
+    t0 = time.perf_counter()
+    placed_host = ctx.generate_surfels()
+    place_host = (time.perf_counter() - t0) * 1e3
+    ctx.set_tuning("surfels_gpu", 1)
+    ctx.generate_surfels()
+    if placed_host != placed:
+        raise SystemExit("device and host surfel placement disagree")
     bounces = info["bounces"]
     note(name, "%d surfels placed; warm-up" % placed)
     for _ in range(warmup):
@@ -221,6 +234,8 @@ def measure(name, steps, warmup, with_cpu):
         "roofline": roof,
         "roofline_jacobi": jac_roof,
         "roofline_tracing": trace_roofs,
+        "surfel_placement": {"surfels": placed, "gpu_ms": place_gpu, "host_ms": place_host,
+                             "host_threads": max(1, min(16, os.cpu_count() or 1))},
     }
     if with_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))

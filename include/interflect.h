@@ -89,7 +89,8 @@ ifx_status ifx_set_render(ifx_ctx* ctx, uint32_t width, uint32_t height, uint64_
 /* Scene::use_bvh, the --no-bvh switch (src/scene.rs:172-174). */
 ifx_status ifx_set_use_bvh(ifx_ctx* ctx, int32_t on);
 
-/* surfel::generate (src/surfel.rs:147-255), on the host. */
+/* surfel::generate (src/surfel.rs:147-255): on the device when the context has one
+ * (tuning key "surfels_gpu", default 1), else on the host; both give the same surfels. */
 ifx_status ifx_generate_surfels(ifx_ctx* ctx, uint64_t target, uint64_t* placed);
 /* Supply surfels directly (the pipeline input "given identical surfels"). */
 ifx_status ifx_set_surfels(ifx_ctx* ctx, const float* p, const float* n, const float* albedo, const float* area,
@@ -144,7 +145,8 @@ ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
  *   "vis_blocks"    grid of the link-visibility kernel (0 = one lane per candidate)
  *   "ff_blocks"     grid of the candidate-cull kernels (0 = automatic)
  *   "jacobi_blocks" grid of the Jacobi and adjoint sweep kernels (0 = automatic)
- *   "jacobi_rows"   0 = automatic (lane per row), 1 = one lane per row, 2 = one wavefront per row */
+ *   "jacobi_rows"   0 = automatic (lane per row), 1 = one lane per row, 2 = one wavefront per row
+ *   "surfels_gpu"   1 = place surfels on the device (default), 0 = on the host */
 ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value);
 ifx_status ifx_kernel_ops(ifx_ctx* ctx, const char* name, uint64_t* ops);
 

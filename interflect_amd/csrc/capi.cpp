@@ -170,7 +170,13 @@ ifx_status ifx_generate_surfels(ifx_ctx* ctx, uint64_t target, uint64_t* placed)
             ctx->err = "surfel target must be in 1..2000000";
             return IFX_E_INVALID;
         }
-        std::vector<ifx::HSurfel> s = ifx::generate_surfels(ctx->eng.scene, target, host_threads());
+        std::vector<ifx::HSurfel> s;
+        if (ctx->eng.has_device() && ctx->eng.place_on_gpu) {
+            int rc = ctx->eng.generate_surfels_gpu(target, s, ctx->err);
+            if (rc) return rc;
+        } else {
+            s = ifx::generate_surfels(ctx->eng.scene, target, host_threads());
+        }
         if (placed) *placed = s.size();
         return ctx->eng.set_surfels(s, ctx->err);
     });
@@ -375,6 +381,7 @@ ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value) {
         else if (k == "ff_blocks") ctx->eng.ff_blocks = (uint32_t)value;
         else if (k == "jacobi_blocks") ctx->eng.jacobi_blocks = (uint32_t)value;
         else if (k == "jacobi_rows") ctx->eng.jacobi_rows = (uint32_t)value;
+        else if (k == "surfels_gpu") ctx->eng.place_on_gpu = value != 0;
 
         else {
             ctx->err = "unknown tuning key `" + k + "`";

@@ -83,6 +83,9 @@ class Engine {
 
     int load_scene(const std::string& src, std::string& err);
     int set_surfels(const std::vector<HSurfel>& s, std::string& err);
+    // surfel::generate on the device, equal to generate_surfels() on the host.
+    int generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std::string& err);
+    uint32_t place_rounds() const { return last_place_rounds_; }
     int transfer(int32_t res, TransferOut& out, std::string& err);
     int solve(uint32_t bounces, SolveOut& out, std::string& err);
     int shade(const HCamera& cam, uint32_t w, uint32_t h, int32_t mode, float* film, uint8_t* rgb8, uint64_t& evals,
@@ -137,6 +140,7 @@ class Engine {
     uint32_t vis_blocks = 0, ff_blocks = 0, jacobi_blocks = 0;  // 0 = automatic
     uint32_t jacobi_rows = 0;  // 0 automatic, 1 lane per row, 2 wavefront per row
     float solve_tol = 1e-4f;   // Jacobi stops once max |delta| < solve_tol (src/solve.rs:98)
+    bool place_on_gpu = true;  // ifx_generate_surfels on the device when there is one
     uint32_t adj_max_iters = 1000;  // adjoint sweeps at most
     float adj_tol = 1e-6f;          // adjoint stops once max |delta mu| <= adj_tol * max |mu|
     uint32_t adj_seg = 1024;        // transposed-link entries per partial sum (fixes the summation order)
@@ -205,6 +209,7 @@ class Engine {
     uint64_t transfer_gen_ = 0, adj_gen_ = ~0ull;
     uint32_t adj_nseg_ = 0;
     bool have_beauty_ = false, have_gterm_ = false;
+    uint32_t last_place_rounds_ = 0;
     HCamera beauty_cam_;
     uint32_t beauty_w_ = 0, beauty_h_ = 0;
     uint32_t gdim_[3] = {1, 1, 1};

@@ -9,21 +9,11 @@
 #include <vector>
 
 #include "host_surfels.h"
+#include "ifx_surfel.h"
 
 namespace ifx {
 
 namespace {
-float halton(uint32_t i, uint32_t base) {
-    float f = 1.0f, r = 0.0f;
-    float inv = 1.0f / (float)base;
-    while (i > 0) {
-        f *= inv;
-        r += f * (float)(i % base);
-        i /= base;
-    }
-    return r;
-}
-
 struct Seed {
     f3 p, n;
     uint32_t mat;
@@ -36,41 +26,16 @@ std::vector<HSurfel> generate_surfels(const HostScene& sc, uint64_t target, int 
     if (target == 0) return out;
     const SceneView S = sc.host_view();
     const LightsView L = sc.host_lights();
-    const f3 lo = sc.lo, hi = sc.hi;
-    const float diag = len3(hi - lo);
-    const float d_min = fmx(diag / sqrtf((float)target), 1e-4f);
-    const float inv_cell = 1.0f / d_min;
-    const int32_t dim[3] = {clampi(sat_i32(ceilf((hi.x - lo.x) * inv_cell)), 1, 512),
-                            clampi(sat_i32(ceilf((hi.y - lo.y) * inv_cell)), 1, 512),
-                            clampi(sat_i32(ceilf((hi.z - lo.z) * inv_cell)), 1, 512)};
-    const float eps = d_min * 0.02f;
-    const float d_min_sq = d_min * d_min;
-    const float area = d_min * d_min / 0.65f;
-    const uint64_t s24 = (uint64_t)(uint32_t)target * 24ull;
-    const uint32_t seeds = (uint32_t)std::min<uint64_t>(s24, 6000000ull);
-
-    auto cell_of = [&](f3 p, int32_t c[3]) {
-        c[0] = clampi(sat_i32((p.x - lo.x) * inv_cell), 0, dim[0] - 1);
-        c[1] = clampi(sat_i32((p.y - lo.y) * inv_cell), 0, dim[1] - 1);
-        c[2] = clampi(sat_i32((p.z - lo.z) * inv_cell), 0, dim[2] - 1);
-    };
+    const SurfelParams P = surfel_params(sc.lo, sc.hi, target);
+    const int32_t* dim = P.dim;
+    const float d_min_sq = P.d_min_sq;
+    const float area = P.area;
+    const uint32_t seeds = P.seeds;
 
     auto project = [&](uint32_t i, Seed& sd) {
         ArrStack stk;
-        f3 s = mk(halton(i + 1, 2), halton(i + 1, 3), halton(i + 1, 5));
-        f3 p = lo + (hi - lo) * s;
-        bool ok = false;
-        for (int it = 0; it < 10; ++it) {
-            float d = dist(S, p, stk);
-            if (fabsf(d) < eps) {
-                ok = true;
-                break;
-            }
-            if (!std::isfinite(d) || fabsf(d) > diag) break;
-            p = p - scene_normal(S, p, stk) * d;
-        }
-        sd.ok = ok && fabsf(dist(S, p, stk)) < eps && !(p.x < lo.x || p.y < lo.y || p.z < lo.z || p.x > hi.x ||
-                                                        p.y > hi.y || p.z > hi.z);
+        f3 p;
+        sd.ok = project_seed(S, P, i, p, stk);
         sd.p = p;
         if (sd.ok) {
             sd.mat = dist_mat(S, p, stk).mat;
@@ -97,7 +62,7 @@ std::vector<HSurfel> generate_surfels(const HostScene& sc, uint64_t target, int 
             const Seed& sd = buf[k];
             if (!sd.ok) continue;
             int32_t c[3];
-            cell_of(sd.p, c);
+            surfel_cell(P, sd.p, c);
             bool close = false;
             for (int32_t z = std::max(c[2] - 1, 0); z <= std::min(c[2] + 1, dim[2] - 1) && !close; ++z)
                 for (int32_t y = std::max(c[1] - 1, 0); y <= std::min(c[1] + 1, dim[1] - 1) && !close; ++y)
