@@ -342,6 +342,46 @@ def measure_grads(steps, warmup, with_cpu):
     return out
 
 
+def measure_turntable(steps, with_cpu, frames=12):
+    """scenes/product.rad turntable (src/main.rs:492-520): one solve, then
+    `frames` orbit views in one launch; ms per view."""
+    import interflect_amd as ia
+
+    text = open(os.path.join(ROOT, "tests", "golden", "scenes", "product.rad")).read()
+    ctx = ia.Context(0)
+    info = ctx.load_scene(text)
+    ctx.generate_surfels()
+    ctx.transfer(8)
+    ctx.solve()
+    cams = ctx.turntable_cameras(frames)
+    ctx.shade_views(cams, "beauty", film=False)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = ctx.shade_views(cams, "beauty", film=False)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    out = {"value": frames * 1e3 / ms, "unit": "views/s", "ms_per_step": ms, "ms_per_view": ms / frames,
+           "evals_per_view": r["evals"] / frames,
+           "config": {"workload": "scenes/product.rad, %d turntable views at %dx%d from one solve, beauty + RGB8"
+                                  % (frames, info["width"], info["height"])},
+           "reference_published_ms_per_view": 80.0}
+    if with_cpu:
+        from oracle import pyoracle as po
+
+        threads = max(1, min(16, os.cpu_count() or 1))
+        sc = po.Scene(text, threads=threads)
+        sc.set_surfels(*ctx.surfels())
+        sc.transfer(8)
+        sc.solve()
+        t1 = time.perf_counter()
+        sc.render("beauty", camera=cams[1])
+        cpu = (time.perf_counter() - t1) * 1e3
+        out["cpu_baseline"] = {"value": 1e3 / cpu, "unit": "views/s", "cores": threads, "kind": "port",
+                               "sample": "one oracle beauty view of the same turntable on %d threads of %s"
+                                         % (threads, cpu_model()),
+                               "ms_per_view": cpu, "speedup": cpu / (ms / frames)}
+    return out
+
+
 def measure_batch(steps, n_scenes=64, surfel_div=1, side_div=1):
     """Batched frames of `n_scenes` seeded scenes vs sequential single-scene
     frames (the divisors shrink surfels and films of the same scenes)."""
@@ -459,6 +499,8 @@ def main():
             line.setdefault("secondary", {})[extra] = measure_grads(args.also_steps, 1, not args.no_cpu)
         elif extra == "batch":
             line.setdefault("secondary", {})[extra] = measure_batch(args.also_steps)
+        elif extra == "turntable":
+            line.setdefault("secondary", {})[extra] = measure_turntable(max(3, args.also_steps), not args.no_cpu)
         elif extra == "batch_small":
             line.setdefault("secondary", {})[extra] = measure_batch(max(3, args.also_steps), 64, 20, 4)
         else:

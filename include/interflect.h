@@ -118,6 +118,15 @@ ifx_status ifx_set_radiosity(ifx_ctx* ctx, const float* direct, const float* irr
 ifx_status ifx_shade(ifx_ctx* ctx, const ifx_camera* cam, uint32_t width, uint32_t height, int32_t mode,
                      float* film_rgb, uint8_t* rgb8, ifx_shade_stats* out);
 
+/* Many cameras over one solve in one launch (the turntable of src/main.rs:492-520 and any
+ * other view set); film_rgb / rgb8 hold n x height x width x 3 values, view-major; each view
+ * equals ifx_shade with that camera. */
+ifx_status ifx_shade_views(ifx_ctx* ctx, const ifx_camera* cams, uint32_t n, uint32_t width, uint32_t height,
+                           int32_t mode, float* film_rgb, uint8_t* rgb8, ifx_shade_stats* out);
+/* The reference's n turntable cameras: the scene camera orbited about its look point
+ * (src/main.rs:494-503). */
+ifx_status ifx_turntable_cameras(ifx_ctx* ctx, uint32_t n, ifx_camera* out);
+
 /* Kernel-family probes, one launch each.
  * Scene::dist_mat / Scene::normal (src/scene.rs:191, :218); nrm may be NULL. */
 ifx_status ifx_dist_mat(ifx_ctx* ctx, const float* pts, uint64_t n, float* d, uint32_t* mat, float* nrm);

@@ -41,7 +41,7 @@ EXPORTS = [
     "ifx_batch_create", "ifx_batch_destroy", "ifx_batch_last_error", "ifx_batch_add_scene",
     "ifx_batch_generate_surfels", "ifx_batch_set_graph", "ifx_batch_set_tuning", "ifx_batch_autotune",
     "ifx_batch_run", "ifx_batch_get_stats", "ifx_batch_get_image", "ifx_batch_get_radiosity",
-    "ifx_batch_get_transfer",
+    "ifx_batch_get_transfer", "ifx_shade_views", "ifx_turntable_cameras",
 ]
 
 
@@ -147,6 +147,8 @@ def lib():
         "ifx_batch_get_image": [vp, u32, P, P],
         "ifx_batch_get_radiosity": [vp, u32, P, P, P],
         "ifx_batch_get_transfer": [vp, u32, P, P, P],
+        "ifx_shade_views": [vp, P, u32, u32, u32, i32, P, P, C.POINTER(ShadeStats)],
+        "ifx_turntable_cameras": [vp, u32, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -313,6 +315,26 @@ class Context:
         self._check(lib().ifx_shade(self._h, C.byref(cam) if cam is not None else None, w, h, m, _ptr(f), _ptr(b),
                                     C.byref(st)))
         return {"film": f, "rgb8": b, "evals": int(st.evals), "ms": float(st.ms)}
+
+    def shade_views(self, cameras, mode="beauty", width=None, height=None, film=True, rgb8=True):
+        """One launch over many cameras (n x 10 floats: pos, look, up, fov)."""
+        info = self.info()
+        w = int(width or info["width"])
+        h = int(height or info["height"])
+        cams = np.ascontiguousarray(cameras, np.float32).reshape(-1, 10)
+        n = cams.shape[0]
+        f = np.zeros((n, h, w, 3), np.float32) if film else None
+        b = np.zeros((n, h, w, 3), np.uint8) if rgb8 else None
+        st = ShadeStats()
+        m = MODES[mode] if isinstance(mode, str) else int(mode)
+        self._check(lib().ifx_shade_views(self._h, _ptr(cams), n, w, h, m, _ptr(f), _ptr(b), C.byref(st)))
+        return {"film": f, "rgb8": b, "evals": int(st.evals), "ms": float(st.ms)}
+
+    def turntable_cameras(self, n):
+        """The reference's n orbit cameras (n x 10 floats)."""
+        out = np.zeros((n, 10), np.float32)
+        self._check(lib().ifx_turntable_cameras(self._h, int(n), _ptr(out)))
+        return out
 
     # ---- kernel-family probes
     def dist_mat(self, pts, normals=False):

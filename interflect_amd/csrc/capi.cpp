@@ -310,6 +310,46 @@ ifx_status ifx_shade(ifx_ctx* ctx, const ifx_camera* cam, uint32_t width, uint32
     });
 }
 
+ifx_status ifx_shade_views(ifx_ctx* ctx, const ifx_camera* cams, uint32_t n, uint32_t width, uint32_t height,
+                           int32_t mode, float* film_rgb, uint8_t* rgb8, ifx_shade_stats* o) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        if (n && !cams) return IFX_E_INVALID;
+        std::vector<ifx::HCamera> v(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            v[i].pos = ifx::mk(cams[i].pos[0], cams[i].pos[1], cams[i].pos[2]);
+            v[i].look = ifx::mk(cams[i].look[0], cams[i].look[1], cams[i].look[2]);
+            v[i].up = ifx::mk(cams[i].up[0], cams[i].up[1], cams[i].up[2]);
+            v[i].fov = cams[i].fov;
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        uint64_t ev = 0;
+        int rc = ctx->eng.shade_views(v, width, height, mode, film_rgb, rgb8, ev, ctx->err);
+        if (rc) return rc;
+        if (o) {
+            std::memset(o, 0, sizeof(*o));
+            o->evals = ev;
+            o->ms = (float)ms_since(t0);
+        }
+        return IFX_OK;
+    });
+}
+
+ifx_status ifx_turntable_cameras(ifx_ctx* ctx, uint32_t n, ifx_camera* out) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        if (n && !out) return IFX_E_INVALID;
+        std::vector<ifx::HCamera> v = ctx->eng.turntable(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            out[i].pos[0] = v[i].pos.x, out[i].pos[1] = v[i].pos.y, out[i].pos[2] = v[i].pos.z;
+            out[i].look[0] = v[i].look.x, out[i].look[1] = v[i].look.y, out[i].look[2] = v[i].look.z;
+            out[i].up[0] = v[i].up.x, out[i].up[1] = v[i].up.y, out[i].up[2] = v[i].up.z;
+            out[i].fov = v[i].fov;
+        }
+        return IFX_OK;
+    });
+}
+
 ifx_status ifx_dist_mat(ifx_ctx* ctx, const float* pts, uint64_t n, float* d, uint32_t* mat, float* nrm) {
     return guarded(ctx, [&]() -> ifx_status {
         if (ifx_status s = need_scene(ctx)) return s;

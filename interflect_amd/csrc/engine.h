@@ -88,6 +88,10 @@ class Engine {
     uint32_t place_rounds() const { return last_place_rounds_; }
     int transfer(int32_t res, TransferOut& out, std::string& err);
     int solve(uint32_t bounces, SolveOut& out, std::string& err);
+    // Many cameras over one solve, one launch (film / rgb8: views x h x w x 3).
+    int shade_views(const std::vector<HCamera>& cams, uint32_t w, uint32_t h, int32_t mode, float* film,
+                    uint8_t* rgb8, uint64_t& evals, std::string& err);
+    std::vector<HCamera> turntable(uint32_t n) const;
     int shade(const HCamera& cam, uint32_t w, uint32_t h, int32_t mode, float* film, uint8_t* rgb8, uint64_t& evals,
               std::string& err);
 
@@ -153,6 +157,7 @@ class Engine {
     };
     Lds plan_lds(bool lights) const;
     int shade_setup(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, ShadeArgs& A, std::string& err);
+    CamV cam_basis(const HCamera& cam, uint32_t W, uint32_t H) const;
     int adjoint_prepare(std::string& err);
     int adjoint_solve(BackwardOut& out, std::string& err);
     int adjoint_params(bool pixels, uint64_t npx, float* d_albedo, float* d_emit, std::string& err);

@@ -536,6 +536,27 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView Sg, LightsView Lg, L
     stk_flush(stk, ops);
 }
 
+// Many cameras over one solve (the turntable, src/main.rs:492-520): grid z
+// is the view; each view is k_shade with its own camera.
+template <bool COUNT, bool LDS>
+__global__ __launch_bounds__(kBlock) void k_shade_views(SceneView Sg, LightsView Lg, LdsPlan P, ShadeArgs A0,
+                                                        const CamV* __restrict__ cams, float4* film,
+                                                        unsigned long long* evals, unsigned long long* ops) {
+    IFX_SETUP(LDS, true, COUNT);
+    ShadeArgs A = A0;
+    A.cam = cams[blockIdx.z];
+    uint32_t wv = threadIdx.x >> 6, l = threadIdx.x & 63u;
+    uint32_t x = blockIdx.x * 16u + (wv & 1u) * 8u + (l & 7u);
+    uint32_t y = blockIdx.y * 16u + (wv >> 1) * 8u + (l >> 3);
+    uint32_t local = 0;
+    if (x < A.W && y < A.H) {
+        f3 c = shade_pixel(S, L, A, x, y, local, stk);
+        film[(size_t)blockIdx.z * A.W * A.H + (size_t)y * A.W + x] = st4(c);
+    }
+    count_steps(evals, local);
+    stk_flush(stk, ops);
+}
+
 __global__ void k_encode(const float4* __restrict__ film, uint32_t npx, float exposure, int32_t tonemap,
                          const float* __restrict__ thr, uint8_t* out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1182,14 +1203,7 @@ int Engine::shade_setup(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode
     }
     const uint32_t N = (uint32_t)surfels.size();
     const float diag = scene.diag();
-    // Camera basis (src/main.rs:199-211), evaluated on the host like the reference.
-    A.cam.origin = cam.pos;
-    A.cam.fwd = nrm3(cam.look - cam.pos);
-    A.cam.right = nrm3(cross3(A.cam.fwd, cam.up));
-    A.cam.up = cross3(A.cam.right, A.cam.fwd);
-    const float rads_per_deg = kPi / 180.0f;
-    A.cam.tan_half = tanf(cam.fov * rads_per_deg * 0.5f);
-    A.cam.aspect = (float)W / (float)H;
+    A.cam = cam_basis(cam, W, H);
     A.W = W;
     A.H = H;
     A.mode = mode;
@@ -1298,6 +1312,82 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
         beauty_h_ = H;
     }
     return 0;
+}
+
+CamV Engine::cam_basis(const HCamera& cam, uint32_t W, uint32_t H) const {
+    // src/main.rs:199-211, evaluated on the host like the reference.
+    CamV c;
+    c.origin = cam.pos;
+    c.fwd = nrm3(cam.look - cam.pos);
+    c.right = nrm3(cross3(c.fwd, cam.up));
+    c.up = cross3(c.right, c.fwd);
+    const float rads_per_deg = kPi / 180.0f;
+    c.tan_half = tanf(cam.fov * rads_per_deg * 0.5f);
+    c.aspect = (float)W / (float)H;
+    return c;
+}
+
+int Engine::shade_views(const std::vector<HCamera>& cams, uint32_t W, uint32_t H, int32_t mode, float* film,
+                        uint8_t* rgb8, uint64_t& evals, std::string& err) {
+    evals = 0;
+    if (cams.empty()) return 0;
+    if (cams.size() > 65535) {
+        err = "at most 65535 views per call";
+        return -1;
+    }
+    ShadeArgs A;
+    if (int rc = shade_setup(cams[0], W, H, mode, A, err)) return rc;
+    const uint32_t nv = (uint32_t)cams.size();
+    std::vector<CamV> hc(nv);
+    for (uint32_t v = 0; v < nv; ++v) hc[v] = cam_basis(cams[v], W, H);
+    unsigned long long* ctr = d_counter.as<unsigned long long>();
+    CK(hipMemsetAsync(ctr, 0, 64, stream));
+    DevBuf& d_cams = d_probe_in;
+    CK(d_cams.ensure(sizeof(CamV) * nv));
+    CK(hipMemcpyAsync(d_cams.p, hc.data(), sizeof(CamV) * nv, hipMemcpyHostToDevice, stream));
+    const uint64_t npx = (uint64_t)W * H * nv;
+    CK(d_film.ensure(16ull * npx));
+    CK(d_rgb8.ensure(3ull * npx));
+    const SceneView S = dev_view();
+    const LightsView Lv = dev_lights();
+    dim3 grid((W + 15) / 16, (H + 15) / 16, nv);
+    const Lds ld = plan_lds(true);
+    launch("shade_views", [&] {
+        LAUNCH_CL(k_shade_views, count_ops_, ld.staged, grid, ld.bytes, S, Lv, ld.plan, A, d_cams.as<CamV>(),
+                  d_film.as<float4>(), ctr, count_ops_ ? ctr + 4 : nullptr);
+    });
+    if (rgb8) {
+        launch("encode", [&] {
+            k_encode<<<blocks_for(npx, 256), 256, 0, stream>>>(d_film.as<float4>(), (uint32_t)npx,
+                                                               scene.cfg.exposure, mode <= M_INDIRECT ? 1 : 0,
+                                                               d_srgb.as<float>(), d_rgb8.as<uint8_t>());
+        });
+        CK(hipMemcpyAsync(rgb8, d_rgb8.p, 3ull * npx, hipMemcpyDeviceToHost, stream));
+    }
+    if (film)
+        if (int rc = copy3_d2h(film, d_film.p, npx, err)) return rc;
+    unsigned long long cv[8] = {0};
+    CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    drain_events();
+    if (count_ops_) kops_["shade"] += cv[4];
+    evals = cv[0];
+    return 0;
+}
+
+std::vector<HCamera> Engine::turntable(uint32_t n) const {
+    // src/main.rs:494-503: orbit the camera about `look` in n equal steps.
+    std::vector<HCamera> out(n);
+    const HCamera& c = scene.cam;
+    const f3 base = c.pos - c.look;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float a = kTau * (float)i / (float)n;
+        const float s = sinf(a), co = cosf(a);
+        HCamera v = c;
+        v.pos = c.look + mk(base.x * co - base.z * s, base.y, base.x * s + base.z * co);
+        out[i] = v;
+    }
+    return out;
 }
 
 // ---------------------------------------------------------------------- probes / downloads

@@ -343,3 +343,36 @@ def test_extension_scene_pipeline(which):
     film, ev = sc.render("beauty")
     frac, worst = image_close(r["rgb8"], po.encode(film, sc.exposure, True))
     assert frac >= 0.999, (frac, worst)
+
+
+# --------------------------------------------------------------------------- many views over one solve
+def test_turntable_cameras_follow_the_reference(product_pair):
+    """src/main.rs:494-503: orbit about `look`, angle TAU * i / n in f32."""
+    ctx, _ = product_pair
+    n = 12
+    cams = ctx.turntable_cameras(n)
+    info = ctx.info()
+    pos, look = info["cam_pos"].astype(np.float32), info["cam_look"].astype(np.float32)
+    base = pos - look
+    for i in range(n):
+        a = np.float32(np.float32(2.0 * np.pi) * np.float32(i)) / np.float32(n)
+        s, c = np.sin(np.float64(a)), np.cos(np.float64(a))
+        want = look + np.array([base[0] * c - base[2] * s, base[1], base[0] * s + base[2] * c])
+        assert np.allclose(cams[i, 0:3], want, rtol=0, atol=4e-6 * (1 + np.abs(want).max()))
+        assert np.array_equal(cams[i, 3:6], look)
+        assert cams[i, 9] == info["cam_fov"]
+
+
+def test_views_equal_single_shades_and_oracle(product_pair):
+    say("views")
+    ctx, sc = product_pair
+    cams = ctx.turntable_cameras(6)
+    w, h = 160, 120
+    v = ctx.shade_views(cams, "beauty", w, h)
+    for i in range(len(cams)):
+        one = ctx.shade("beauty", w, h, camera=cams[i])
+        assert np.array_equal(bits(v["film"][i]), bits(one["film"]))
+        assert np.array_equal(v["rgb8"][i], one["rgb8"])
+    film, _ = sc.render("beauty", w, h, camera=cams[2])
+    frac, worst = image_close(v["rgb8"][2], po.encode(film, sc.exposure, True))
+    assert frac >= 0.999, (frac, worst)
