@@ -31,13 +31,15 @@ typedef int32_t ifx_status;
 #define IFX_E_NOMEM (-5)   /* device allocation failed */
 #define IFX_E_LIMIT (-6)   /* scene exceeds a kernel limit */
 
-/* Shading modes of src/main.rs:25-51 (the path tracer is not part of this path). */
+/* Shading modes of src/main.rs:25-51; REFERENCE is the brute-force path tracer
+ * (src/reference.rs, "spp" paths per pixel, quad lights). */
 #define IFX_MODE_BEAUTY 0
 #define IFX_MODE_DIRECT 1
 #define IFX_MODE_INDIRECT 2
 #define IFX_MODE_NORMALS 3
 #define IFX_MODE_DEPTH 4
 #define IFX_MODE_STEPS 5
+#define IFX_MODE_REFERENCE 6
 
 typedef struct {
     uint32_t n_objects, n_planes, n_quad_lights, n_materials; /* materials exclude the implicit default */
@@ -154,8 +156,11 @@ ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
  *   "vis_blocks"    grid of the link-visibility kernel (0 = one lane per candidate)
  *   "ff_blocks"     grid of the candidate-cull kernels (0 = automatic)
  *   "jacobi_blocks" grid of the Jacobi and adjoint sweep kernels (0 = automatic)
- *   "jacobi_rows"   0 = automatic (lane per row), 1 = one lane per row, 2 = one wavefront per row
- *   "surfels_gpu"   1 = place surfels on the device (default), 0 = on the host */
+ *   "jacobi_rows"   0 = automatic (sliced rows when rows average >= 64 links, else 1),
+ *                   1 = one lane per row on the CSR, 2 = one wavefront per row,
+ *                   3 = sliced rows (one lane per row, links stored per 64-row slice column by column)
+ *   "surfels_gpu"   1 = place surfels on the device (default), 0 = on the host
+ *   "spp"           paths per pixel of IFX_MODE_REFERENCE (--spp, default 512) */
 ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value);
 ifx_status ifx_kernel_ops(ifx_ctx* ctx, const char* name, uint64_t* ops);
 

@@ -24,7 +24,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libinterflect_hip.so")
 _LIB = None
 
-MODES = {"beauty": 0, "direct": 1, "indirect": 2, "normals": 3, "depth": 4, "steps": 5}
+MODES = {"beauty": 0, "direct": 1, "indirect": 2, "normals": 3, "depth": 4, "steps": 5, "reference": 6}
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "parse error", -3: "bad state", -4: "device error",
           -5: "out of memory", -6: "limit exceeded"}

@@ -46,7 +46,9 @@ __host__ __device__ inline double fix_scale(float umax, uint64_t npx, float kd_m
 __device__ __forceinline__ float comp(const float4& v, uint32_t c) { return c == 0 ? v.x : (c == 1 ? v.y : v.z); }
 
 // ---------------------------------------------------------------- film side
-__global__ void k_absmax(const float* __restrict__ u, uint64_t n, uint32_t* out) {
+// max |u| as non-negative float bits: grid-stride, then one atomic per block.
+__global__ __launch_bounds__(256) void k_absmax(const float* __restrict__ u, uint64_t n, uint32_t* out) {
+    __shared__ uint32_t wm[4];
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t b = 0;
     for (; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
@@ -54,7 +56,13 @@ __global__ void k_absmax(const float* __restrict__ u, uint64_t n, uint32_t* out)
         b = v > b ? v : b;
     }
     b = wave_max(b);
-    if (lane_id() == 0 && b) atomicMax(out, b);
+    if (lane_id() == 0) wm[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = wm[0];
+        for (int w = 1; w < 4; ++w) m = wm[w] > m ? wm[w] : m;
+        if (m) atomicMax(out, m);
+    }
 }
 
 struct BwdArgs {
@@ -674,7 +682,7 @@ int Engine::backward(const float* dL_dfilm, float* d_albedo, float* d_emit, Back
     B.pde = d_pde.as<float4>();
     B.uacc = d_uacc.as<unsigned long long>();
     launch("shade_bwd", [&] {
-        k_absmax<<<std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(3 * npx, 256), 4096u)), 256, 0, stream>>>(
+        k_absmax<<<std::max<uint32_t>(1u, std::min<uint32_t>(blocks_for(3 * npx, 256), 512u)), 256, 0, stream>>>(
             B.u, 3 * npx, d_umax.as<uint32_t>());
         const SceneView S = dev_view();
         const Lds ld = plan_lds(true);

@@ -741,6 +741,7 @@ int Batch::build_slots(std::string& err) {
         A.use_gi = 1;
         A.tmax = B.diag * 4.0f;
         A.radius = (B.diag / sqrtf((float)sc.cfg.surfels)) * 2.6f;
+        A.spp = 1;
         A.spos = d_spos.as<float4>();
         A.snrm = d_snrm.as<float4>();
         A.irr = d_irr.as<float4>();

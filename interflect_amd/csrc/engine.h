@@ -62,13 +62,17 @@ struct ShadeArgs {
     int32_t mode;
     int32_t use_gi;
     float tmax, radius;
+    uint32_t spp;  // paths per pixel of the reference mode
     GridView grid;
     const float4* spos;
     const float4* snrm;
     const float4* irr;
 };
 
-enum { M_BEAUTY = 0, M_DIRECT = 1, M_INDIRECT = 2, M_NORMALS = 3, M_DEPTH = 4, M_STEPS = 5 };
+enum { M_BEAUTY = 0, M_DIRECT = 1, M_INDIRECT = 2, M_NORMALS = 3, M_DEPTH = 4, M_STEPS = 5, M_REFERENCE = 6 };
+
+// Modes encoded with exposure + ACES (src/main.rs:46-50).
+inline bool mode_tonemapped(int32_t m) { return m <= M_INDIRECT || m == M_REFERENCE; }
 
 struct KernelStat {
     uint64_t launches = 0;
@@ -142,9 +146,10 @@ class Engine {
 
     bool use_lds = true;  // stage scene records in LDS when they fit
     uint32_t vis_blocks = 0, ff_blocks = 0, jacobi_blocks = 0;  // 0 = automatic
-    uint32_t jacobi_rows = 0;  // 0 automatic, 1 lane per row, 2 wavefront per row
+    uint32_t jacobi_rows = 0;  // 0 automatic, 1 lane per row on the CSR, 2 wavefront per row, 3 sliced rows
     float solve_tol = 1e-4f;   // Jacobi stops once max |delta| < solve_tol (src/solve.rs:98)
     bool place_on_gpu = true;  // ifx_generate_surfels on the device when there is one
+    uint32_t spp = 512;        // paths per pixel of the reference mode (--spp, src/main.rs:81)
     uint32_t adj_max_iters = 1000;  // adjoint sweeps at most
     float adj_tol = 1e-6f;          // adjoint stops once max |delta mu| <= adj_tol * max |mu|
     uint32_t adj_seg = 1024;        // transposed-link entries per partial sum (fixes the summation order)
@@ -159,6 +164,7 @@ class Engine {
     int shade_setup(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, ShadeArgs& A, std::string& err);
     CamV cam_basis(const HCamera& cam, uint32_t W, uint32_t H) const;
     int adjoint_prepare(std::string& err);
+    int build_sliced(std::string& err);
     int adjoint_solve(BackwardOut& out, std::string& err);
     int adjoint_params(bool pixels, uint64_t npx, float* d_albedo, float* d_emit, std::string& err);
     int check_backward(std::string& err) const;
@@ -210,7 +216,8 @@ class Engine {
     DevBuf d_smat, d_scl, d_sscale, d_lrow, d_perm, d_perm2, d_ckeyt, d_ckeyt2, d_crow, d_cw, d_colcnt, d_colstart,
         d_segcnt, d_segfirst, d_seg, d_segsum, d_ubar, d_uacc, d_umax, d_mu, d_v, d_dmax, d_pmat, d_pdr, d_pde, d_sdr,
         d_sde, d_partial;
-    DevBuf d_ufilm, d_stage;
+    DevBuf d_ufilm, d_stage, d_sbase, d_sidx, d_sw;
+    uint64_t sliced_gen_ = ~0ull, sliced_cells_ = 0;
     uint64_t transfer_gen_ = 0, adj_gen_ = ~0ull;
     uint32_t adj_nseg_ = 0;
     bool have_beauty_ = false, have_gterm_ = false;

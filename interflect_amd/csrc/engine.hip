@@ -424,6 +424,9 @@ struct JacobiArgs {
     float4* irr;
     float4* bc;
     uint32_t* resid;  // [bounces] max |delta| bits
+    const uint32_t* sbase;  // sliced layout: per 64-row slice, its first column / 64 (null = CSR)
+    const uint32_t* sidx;
+    const float* sw;
 };
 
 __device__ __forceinline__ bool jacobi_done(const JacobiArgs& A, uint32_t it) {
@@ -488,6 +491,20 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_rows(JacobiArgs A, uint32_t i
                 A.rad[i] = st4(nr);
             }
         }
+    } else if (A.sbase) {
+        // Sliced rows: slice i/64 stores its rows' k-th links side by side,
+        // so lane i%64 reads column k coalesced with its neighbours.
+        for (uint32_t i = gt; i < A.N; i += nt) {
+            f3 g = spl(0.0f);
+            const uint32_t len = A.lstart[i + 1] - A.lstart[i];
+            const uint64_t b = 64ull * A.sbase[i >> 6] + (i & 63u);
+            for (uint32_t k = 0; k < len; ++k) g = g + ld3(A.bc[A.sidx[b + 64ull * k]]) * A.sw[b + 64ull * k];
+            f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
+            uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
+            delta = d > delta ? d : delta;
+            A.irr[i] = st4(g);
+            A.rad[i] = st4(nr);
+        }
     } else {
         for (uint32_t i = gt; i < A.N; i += nt) {
             f3 g = spl(0.0f);
@@ -502,6 +519,28 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_rows(JacobiArgs A, uint32_t i
     }
     uint32_t m = wave_max(delta);
     if (lane == 0 && m) atomicMax(&A.resid[it], m);
+}
+
+// Sliced copy of the CSR (one slice per 64 consecutive rows, columns of
+// the slice's longest row): widths, then a scan, then the fill.
+__global__ void k_sell_widths(uint32_t N, const uint32_t* __restrict__ lstart, uint32_t* width) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t len = i < N ? lstart[i + 1] - lstart[i] : 0u;
+    len = wave_max(len);
+    if ((i & 63u) == 0 && i < N) width[i >> 6] = len;
+}
+
+__global__ void k_sell_fill(uint32_t N, const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ idx,
+                            const float* __restrict__ w, const uint32_t* __restrict__ sbase, uint32_t* sidx,
+                            float* sw) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const uint32_t s0 = lstart[i], len = lstart[i + 1] - s0;
+    const uint64_t b = 64ull * sbase[i >> 6] + (i & 63u);
+    for (uint32_t k = 0; k < len; ++k) {
+        sidx[b + 64ull * k] = idx[s0 + k];
+        sw[b + 64ull * k] = w[s0 + k];
+    }
 }
 
 // ====================================================================== shade
@@ -1122,7 +1161,18 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
         A.N = N;
         A.C = C;
         A.tol = solve_tol;
-        A.wave_rows = jacobi_rows == 2 ? 1u : 0u;  // lane per row measured faster at 51 and 237 links/row
+        A.wave_rows = jacobi_rows == 2 ? 1u : 0u;
+        A.sbase = nullptr;
+        A.sidx = nullptr;
+        A.sw = nullptr;
+        // automatic: sliced rows when rows are long enough (>= 64 links on
+        // average) for coalescing to repay building the copy once per transfer
+        if (jacobi_rows == 3 || (jacobi_rows == 0 && n_links >= 64ull * N)) {
+            if (int rc = build_sliced(err)) return rc;
+            A.sbase = d_sbase.as<uint32_t>();
+            A.sidx = d_sidx.as<uint32_t>();
+            A.sw = d_sw.as<float>();
+        }
         A.range = d_crange.as<uint2>();
         A.members = d_cval2.as<uint32_t>();
         A.spos = d_spos.as<float4>();
@@ -1183,11 +1233,43 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
     return 0;
 }
 
+// Sliced copy of the links for the Jacobi row gather, once per transfer.
+int Engine::build_sliced(std::string& err) {
+    if (sliced_gen_ == transfer_gen_) return 0;
+    const uint32_t N = (uint32_t)surfels.size();
+    const uint32_t ns = (N + 63u) / 64u;
+    CK(d_sbase.ensure(4ull * (ns + 1)));
+    DevBuf& width = d_cand_row;  // transfer scratch, dead after the compaction
+    CK(width.ensure(4ull * (ns + 1)));
+    CK(hipMemsetAsync(width.p, 0, 4ull * (ns + 1), stream));
+    if (N)
+        k_sell_widths<<<blocks_for(64ull * ns, 256), 256, 0, stream>>>(N, d_lstart.as<uint32_t>(),
+                                                                       width.as<uint32_t>());
+    size_t tmp = 0;
+    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, width.as<uint32_t>(), d_sbase.as<uint32_t>(), (int)(ns + 1),
+                                        stream));
+    CK(d_tmp.ensure(tmp));
+    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, width.as<uint32_t>(), d_sbase.as<uint32_t>(), (int)(ns + 1),
+                                        stream));
+    uint32_t cols = 0;
+    CK(hipMemcpyAsync(&cols, d_sbase.as<uint32_t>() + ns, 4, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    CK(d_sidx.ensure(256ull * cols + 4));
+    CK(d_sw.ensure(256ull * cols + 4));
+    if (N)
+        k_sell_fill<<<blocks_for(N, 256), 256, 0, stream>>>(N, d_lstart.as<uint32_t>(), d_idx.as<uint32_t>(),
+                                                            d_w.as<float>(), d_sbase.as<uint32_t>(),
+                                                            d_sidx.as<uint32_t>(), d_sw.as<float>());
+    sliced_cells_ = 64ull * cols;
+    sliced_gen_ = transfer_gen_;
+    return 0;
+}
+
 // Camera, gather radius and (for the GI modes) the surfel grid of one
 // render_frame call; shared by shade() and backward().
 int Engine::shade_setup(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, ShadeArgs& A, std::string& err) {
     if (int rc0 = need_device(err)) return rc0;
-    if (mode < 0 || mode > 5) {
+    if (mode < 0 || mode > 6) {
         err = "unknown shading mode";
         return -1;
     }
@@ -1210,6 +1292,7 @@ int Engine::shade_setup(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode
     A.use_gi = needs_gi ? 1 : 0;
     A.tmax = diag * 4.0f;
     A.radius = (diag / sqrtf((float)scene.cfg.surfels)) * 2.6f;
+    A.spp = spp ? spp : 1u;
     A.spos = d_spos.as<float4>();
     A.snrm = d_snrm.as<float4>();
     A.irr = d_irr.as<float4>();
@@ -1287,7 +1370,7 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
         LAUNCH_CL(k_shade, count_ops_, ld.staged, grid, ld.bytes, S, Lv, ld.plan, A, d_film.as<float4>(), ctr,
                   count_ops_ ? ctr + 4 : nullptr);
     });
-    const bool tonemapped = mode <= M_INDIRECT;
+    const bool tonemapped = mode_tonemapped(mode);
     if (rgb8) {
         launch("encode", [&] {
             k_encode<<<blocks_for(npx, 256), 256, 0, stream>>>(d_film.as<float4>(), (uint32_t)npx,
@@ -1359,7 +1442,7 @@ int Engine::shade_views(const std::vector<HCamera>& cams, uint32_t W, uint32_t H
     if (rgb8) {
         launch("encode", [&] {
             k_encode<<<blocks_for(npx, 256), 256, 0, stream>>>(d_film.as<float4>(), (uint32_t)npx,
-                                                               scene.cfg.exposure, mode <= M_INDIRECT ? 1 : 0,
+                                                               scene.cfg.exposure, mode_tonemapped(mode) ? 1 : 0,
                                                                d_srgb.as<float>(), d_rgb8.as<uint8_t>());
         });
         CK(hipMemcpyAsync(rgb8, d_rgb8.p, 3ull * npx, hipMemcpyDeviceToHost, stream));

@@ -194,6 +194,12 @@ __device__ __forceinline__ f3 shade_pixel(const SceneView& S, const LightsView& 
     float px = (2.0f * ((float)x + 0.5f) / (float)A.W - 1.0f) * cam.aspect * cam.tan_half;
     float py = (1.0f - 2.0f * ((float)y + 0.5f) / (float)A.H) * cam.tan_half;
     f3 rd = nrm3((cam.fwd + cam.right * px) + cam.up * py);
+    if (A.mode == M_REFERENCE) {  // src/main.rs:249-258: spp paths, seeded per pixel
+        PathRng rng(((uint64_t)y << 32) | (uint64_t)x);
+        f3 acc = spl(0.0f);
+        for (uint32_t k = 0; k < A.spp; ++k) acc = acc + ref_path(S, L, cam.origin, rd, A.tmax, 12u, rng, stk);
+        return acc / (float)A.spp;
+    }
     uint32_t steps = 0;
     Hit hh = trace<true>(S, cam.origin, rd, 1e-3f, A.tmax, steps, stk);
     local += steps;

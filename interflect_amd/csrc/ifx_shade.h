@@ -632,6 +632,99 @@ IFX_HD bool emitter_hit(const LightsView& Lv, f3 ro, f3 rd, float tmax, f3& emit
     return found;
 }
 
+// ---------------------------------------------------------------- path tracer
+// src/reference.rs:24-46: PCG32, output of the advanced state (XSH-RR), 24-bit float.
+struct PathRng {
+    uint64_t s;
+    IFX_HD explicit PathRng(uint64_t seed) : s(seed * 6364136223846793005ull + 1ull) {}
+    IFX_HD float next_f32() {
+        s = s * 6364136223846793005ull + 1442695040888963407ull;
+        uint64_t x = ((s >> 18) ^ s) >> 27;
+        uint32_t rot = (uint32_t)(s >> 59);
+        uint32_t xv = (uint32_t)x;
+        uint32_t v = (xv >> rot) | (xv << ((32u - rot) & 31u));
+        return (float)(v >> 8) * (1.0f / 16777216.0f);
+    }
+};
+
+// src/reference.rs:48-58 (sin / cos from the device math library).
+IFX_HD f3 cosine_hemisphere(f3 n, PathRng& rng) {
+    f3 t, b;
+    frame(n, t, b);
+    float u1 = rng.next_f32();
+    float u2 = rng.next_f32();
+    float r = sqrtf(u1);
+    float phi = kTau * u2;
+    return nrm3((t * (r * cosf(phi)) + b * (r * sinf(phi))) + n * sqrtf(fmx(1.0f - u1, 0.0f)));
+}
+
+// src/reference.rs:61-137: one path over the quad lights (the reference's
+// light list): next-event estimation with hard shadows, cosine-weighted
+// bounces, Russian roulette after depth 3.
+template <class Stk>
+IFX_HD f3 ref_path(const SceneView& S, const LightsView& Lv, f3 ro, f3 rd, float tmax, uint32_t max_depth,
+                   PathRng& rng, Stk& stk) {
+    f3 radiance = spl(0.0f), throughput = spl(1.0f);
+    f3 origin = ro, dir = rd;
+    for (uint32_t depth = 0; depth < max_depth; ++depth) {
+        uint32_t steps = 0;
+        Hit hit = trace<true>(S, origin, dir, 1e-3f, tmax, steps, stk);
+        float surf_t = hit.hit ? hit.t : kFltMax;
+        bool seen = false;
+        f3 e = spl(0.0f);
+        float bt = 0.0f;
+        for (uint32_t li = 0; li < Lv.n_quad; ++li) {  // shade::emitter_hit over the quad list
+            const float4* L = Lv.quad + 7 * li;
+            float t;
+            if (intersect_quad(L, origin, dir, t) && t < fmn(surf_t, tmax) && dot3(ld3(L[6]), dir) < 0.0f &&
+                (!seen || t < bt)) {
+                seen = true;
+                bt = t;
+                e = ld3(L[4]);
+            }
+        }
+        if (seen) {
+            if (depth == 0) radiance = radiance + throughput * e;
+            break;
+        }
+        if (!hit.hit) break;
+        const MatV m = load_mat(Lv, hit.mat);
+        const f3 albedo = m.kd;
+        radiance = radiance + throughput * m.emissive;
+        for (uint32_t li = 0; li < Lv.n_quad; ++li) {
+            const float4* L = Lv.quad + 7 * li;
+            const f3 v0 = ld3(L[0]);
+            const f3 e1 = ld3(L[1]) - v0, e2 = ld3(L[3]) - v0;
+            const float r1 = rng.next_f32();
+            const f3 a = v0 + e1 * r1;
+            const float r2 = rng.next_f32();
+            const f3 lp = a + e2 * r2;
+            const f3 to = lp - hit.p;
+            const float dist2 = dot3(to, to);
+            if (dist2 < 1e-9f) continue;
+            const float dist = sqrtf(dist2);
+            const f3 l = to / dist;
+            const float cos_s = dot3(hit.n, l);
+            const float cos_l = dot3(ld3(L[6]), -l);
+            if (cos_s <= 0.0f || cos_l <= 0.0f) continue;
+            uint32_t s2 = 0;
+            Hit occ = trace<false>(S, hit.p + hit.n * 1e-3f, l, 1e-3f, dist - 1e-2f, s2, stk);
+            if (occ.hit) continue;
+            const float pdf = dist2 / (cos_l * L[4].w);
+            radiance = radiance + ((throughput * albedo) * ld3(L[4])) * (cos_s / (kPi * pdf));
+        }
+        throughput = throughput * albedo;
+        dir = cosine_hemisphere(hit.n, rng);
+        origin = hit.p + hit.n * 1e-3f;
+        if (depth >= 3) {
+            const float q = clmp(maxe(throughput), 0.05f, 0.95f);
+            if (rng.next_f32() > q) break;
+            throughput = throughput / q;
+        }
+    }
+    return radiance;
+}
+
 // Uniform grid over surfel positions, CSR by cell, items ascending per cell
 // (src/surfel.rs:61-114).
 struct GridView {

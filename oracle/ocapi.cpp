@@ -15,6 +15,7 @@ struct Ctx {
     Transfer tr;
     bool have_tr = false;
     uint64_t ff_evals = 0;
+    uint32_t spp = 512;
 };
 V3 ld(const float* p) { return v3(p[0], p[1], p[2]); }
 void st(float* p, V3 v) {
@@ -286,9 +287,18 @@ uint64_t orc_render(void* h, int mode, uint32_t w, uint32_t hh, const float* cam
         gip = &gi;
     }
     std::vector<V3> film;
-    uint64_t ev = render_frame(sc, cm, (Mode)mode, gip, tmax, w, hh, film, threads, y0, y1);
+    uint64_t ev = render_frame(sc, cm, (Mode)mode, gip, tmax, w, hh, film, threads, y0, y1, c->spp);
     for (size_t i = 0; i < film.size(); ++i) st(film_out + 3 * i, film[i]);
     return ev;
+}
+
+// Paths per pixel of the REFERENCE mode (--spp, src/main.rs:81).
+void orc_set_spp(void* h, uint32_t spp) { ((Ctx*)h)->spp = spp; }
+
+// One path tracer sample sequence for tests: n floats of the PCG32 seeded with `seed`.
+void orc_rng(uint64_t seed, uint32_t n, float* out) {
+    Rng r(seed);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.next_f32();
 }
 
 void orc_gather(void* h, const float* p, const float* n, int64_t np, float radius, float* out) {

@@ -368,18 +368,101 @@ V3 Cam::ray(uint32_t x, uint32_t y, uint32_t w, uint32_t h) const {
 static const V3 BG = {0.02f, 0.03f, 0.05f};
 
 // ---------------------------------------------------------------- main.rs:229-367
+// src/reference.rs:48-58
+static V3 cosine_hemisphere(V3 n, Rng& rng) {
+    V3 t, b;
+    frame(n, t, b);
+    float u1 = rng.next_f32();
+    float u2 = rng.next_f32();
+    float r = std::sqrt(u1);
+    float phi = TAU_F * u2;
+    return normalize((t * (r * std::cos(phi)) + b * (r * std::sin(phi))) + n * std::sqrt(fmax_(1.0f - u1, 0.0f)));
+}
+
+// Quad emitter seen along a ray (src/shade.rs:166-181 over sc.lights only,
+// as the path tracer calls it).
+static bool quad_emitter_hit(const Scene& sc, V3 ro, V3 rd, float tmax, V3& emit) {
+    bool found = false;
+    float bt = 0.0f;
+    for (const QuadLight& l : sc.lights) {
+        float t;
+        if (intersect_quad(l.verts, ro, rd, t) && t < tmax && dot(l.normal(), rd) < 0.0f && (!found || t < bt)) {
+            found = true;
+            bt = t;
+            emit = l.emit;
+        }
+    }
+    return found;
+}
+
+V3 ref_path(const Scene& sc, V3 ro, V3 rd, float tmax, uint32_t max_depth, Rng& rng) {
+    V3 radiance = splat(0.0f), throughput = splat(1.0f);
+    V3 origin = ro, dir = rd;
+    for (uint32_t depth = 0; depth < max_depth; ++depth) {
+        uint32_t steps = 0;
+        Hit hit = trace(sc, origin, dir, 1e-3f, tmax, steps);
+        float surf_t = hit.hit ? hit.t : FLT_MAX;
+        V3 e;
+        if (quad_emitter_hit(sc, origin, dir, fmin_(surf_t, tmax), e)) {
+            if (depth == 0) radiance += throughput * e;
+            break;
+        }
+        if (!hit.hit) break;
+        Material m = sc.material(hit.mat);
+        V3 albedo = diffuse_albedo(m);
+        radiance += throughput * m.emissive;
+        for (const QuadLight& light : sc.lights) {
+            V3 e1 = light.verts[1] - light.verts[0];
+            V3 e2 = light.verts[3] - light.verts[0];
+            float r1 = rng.next_f32();
+            V3 a = light.verts[0] + e1 * r1;
+            float r2 = rng.next_f32();
+            V3 lp = a + e2 * r2;
+            V3 to = lp - hit.p;
+            float dist2 = length_sq(to);
+            if (dist2 < 1e-9f) continue;
+            float dist = std::sqrt(dist2);
+            V3 l = to / dist;
+            float cos_s = dot(hit.n, l);
+            float cos_l = dot(light.normal(), -l);
+            if (cos_s <= 0.0f || cos_l <= 0.0f) continue;
+            uint32_t s2 = 0;
+            Hit occ = trace(sc, hit.p + hit.n * 1e-3f, l, 1e-3f, dist - 1e-2f, s2);
+            if (occ.hit) continue;
+            float pdf = dist2 / (cos_l * light.area());
+            radiance += ((throughput * albedo) * light.emit) * (cos_s / (PI_F * pdf));
+        }
+        throughput = throughput * albedo;
+        dir = cosine_hemisphere(hit.n, rng);
+        origin = hit.p + hit.n * 1e-3f;
+        if (depth >= 3) {
+            float q = clampf(max_elem(throughput), 0.05f, 0.95f);
+            if (rng.next_f32() > q) break;
+            throughput = throughput / q;
+        }
+    }
+    return radiance;
+}
+
 uint64_t render_frame(const Scene& sc, const Cam& cam, Mode mode, const Gi* gi, float tmax, uint32_t w,
-                      uint32_t h, std::vector<V3>& film, int threads, uint32_t y0, uint32_t y1) {
+                      uint32_t h, std::vector<V3>& film, int threads, uint32_t y0, uint32_t y1, uint32_t spp) {
     film.assign((size_t)w * h, splat(0.0f));
     y1 = std::min(y1, h);
     std::vector<uint64_t> row_evals(h, 0);
-    bool tonemapped = mode == BEAUTY || mode == DIRECT || mode == INDIRECT;
+    bool tonemapped = mode == BEAUTY || mode == DIRECT || mode == INDIRECT || mode == REFERENCE;
     par_for(y1 > y0 ? y1 - y0 : 0, threads, [&](size_t yy) {
         uint32_t y = y0 + (uint32_t)yy;
         uint64_t local = 0;
         for (uint32_t x = 0; x < w; ++x) {
             V3& px = film[(size_t)y * w + x];
             V3 rd = cam.ray(x, y, w, h);
+            if (mode == REFERENCE) {  // src/main.rs:249-258
+                Rng rng(((uint64_t)y << 32) | (uint64_t)x);
+                V3 acc = splat(0.0f);
+                for (uint32_t k = 0; k < spp; ++k) acc += ref_path(sc, cam.origin, rd, tmax, 12, rng);
+                px = acc / (float)spp;
+                continue;
+            }
             uint32_t steps = 0;
             Hit hh = trace(sc, cam.origin, rd, 1e-3f, tmax, steps);
             local += steps;

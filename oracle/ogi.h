@@ -72,7 +72,26 @@ struct SolveStats {
 SolveStats solve(std::vector<Surfel>& s, const Transfer& t, uint32_t max_bounces, int threads);
 
 // src/main.rs:25-51
-enum Mode { BEAUTY = 0, DIRECT = 1, INDIRECT = 2, NORMALS = 3, DEPTH = 4, STEPS = 5 };
+enum Mode { BEAUTY = 0, DIRECT = 1, INDIRECT = 2, NORMALS = 3, DEPTH = 4, STEPS = 5, REFERENCE = 6 };
+
+// src/reference.rs:24-46: the path tracer's PCG32 (output of the advanced
+// state, XSH-RR, 24-bit float).
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed * 6364136223846793005ull + 1ull) {}
+    float next_f32() {
+        s = s * 6364136223846793005ull + 1442695040888963407ull;
+        uint64_t x = ((s >> 18) ^ s) >> 27;
+        uint32_t rot = (uint32_t)(s >> 59);
+        uint32_t xv = (uint32_t)x;
+        uint32_t v = (xv >> rot) | (xv << ((32u - rot) & 31u));
+        return (float)(v >> 8) * (1.0f / 16777216.0f);
+    }
+};
+
+// src/reference.rs:61-137: one path (quad lights; next-event estimation,
+// cosine-weighted bounces, Russian roulette after depth 3).
+V3 ref_path(const Scene& sc, V3 ro, V3 rd, float tmax, uint32_t max_depth, Rng& rng);
 
 // src/main.rs:188-218
 struct Cam {
@@ -88,10 +107,12 @@ struct Gi {
     float radius;
 };
 
-// src/main.rs:229-367 (modes beauty..steps; the path tracer is out of scope).
-// Rows [y0, y1) of the film only when a sample is asked for.
+// src/main.rs:229-367, all modes (REFERENCE: `spp` paths per pixel, seeded
+// per pixel, src/main.rs:249-258).  Rows [y0, y1) of the film only when a
+// sample is asked for.
 uint64_t render_frame(const Scene& sc, const Cam& cam, Mode mode, const Gi* gi, float tmax, uint32_t w,
-                      uint32_t h, std::vector<V3>& film, int threads, uint32_t y0 = 0, uint32_t y1 = UINT32_MAX);
+                      uint32_t h, std::vector<V3>& film, int threads, uint32_t y0 = 0, uint32_t y1 = UINT32_MAX,
+                      uint32_t spp = 512);
 
 // src/film.rs:8-61
 uint8_t srgb8(float c);

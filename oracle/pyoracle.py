@@ -21,7 +21,7 @@ u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 
-MODES = {"beauty": 0, "direct": 1, "indirect": 2, "normals": 3, "depth": 4, "steps": 5}
+MODES = {"beauty": 0, "direct": 1, "indirect": 2, "normals": 3, "depth": 4, "steps": 5, "reference": 6}
 
 
 def lib_path():
@@ -92,6 +92,8 @@ def lib():
     L.orc_radical_inverse.restype = C.c_float
     L.orc_radical_inverse.argtypes = [C.c_uint32, C.c_uint32]
     L.orc_frame.argtypes = [f32p, f32p, f32p]
+    L.orc_set_spp.argtypes = [vp, C.c_uint32]
+    L.orc_rng.argtypes = [C.c_uint64, C.c_uint32, f32p]
     _LIB = L
     return L
 
@@ -271,6 +273,10 @@ class Scene:
     def set_radiosity(self, direct, irr, rad):
         lib().orc_set_radiosity(self.h, _f(direct, (-1, 3)), _f(irr, (-1, 3)), _f(rad, (-1, 3)))
 
+    def set_spp(self, spp):
+        """Paths per pixel of the "reference" mode (src/main.rs:81)."""
+        lib().orc_set_spp(self.h, int(spp))
+
     def render(self, mode="beauty", width=None, height=None, camera=None, use_gi=True, rows=None):
         w = int(width or self.width)
         h = int(height or self.height)
@@ -339,3 +345,10 @@ def frame(n):
     b = np.zeros(3, np.float32)
     lib().orc_frame(_f(n), t, b)
     return t, b
+
+
+def rng(seed, n):
+    """n floats of the path tracer's PCG32 (src/reference.rs:24-46) seeded with `seed`."""
+    out = np.zeros(n, np.float32)
+    lib().orc_rng(int(seed), int(n), out)
+    return out

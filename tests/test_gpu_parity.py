@@ -254,7 +254,7 @@ def test_launch_config_invariance(name):
     ctx.generate_surfels(4000)
     base = _frame_bytes(ctx)
     for key, val in [("lds", 0), ("vis_blocks", 7), ("ff_blocks", 3), ("jacobi_blocks", 5), ("jacobi_rows", 1),
-                     ("jacobi_rows", 2)]:
+                     ("jacobi_rows", 2), ("jacobi_rows", 3)]:
         ctx.set_tuning(key, val)
         got = _frame_bytes(ctx)
         assert all(np.array_equal(a, b) for a, b in zip(base, got)), key
