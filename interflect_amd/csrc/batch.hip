@@ -55,10 +55,7 @@ __device__ __forceinline__ uint32_t slot_of(const uint32_t* __restrict__ blk, ui
     const LdsPlan P = (LIGHTS) ? T.Pl : T.Pn;                                    \
     if ((LIGHTS) ? T.staged_l : T.staged_n) stage_in<true, LIGHTS>(ifx_lds, S, L); \
     LdsStack stk;                                                                \
-    stk.dbase = (float*)(ifx_lds + P.scene_f4 + P.light_f4) + threadIdx.x;       \
-    stk.base = (uint16_t*)(stk.dbase - threadIdx.x + P.stack_depth * kBlock) +   \
-               threadIdx.x;                                                      \
-    stk.stride = (uint32_t)kBlock;                                               \
+    place_stack(stk, ifx_lds, P.scene_f4 + P.light_f4, P.stack_depth);           \
     (void)L;                                                                     \
     (void)bid
 
@@ -158,9 +155,7 @@ __device__ __forceinline__ void bscene(const BSlot& T, float4* lds, SceneView& S
     L = T.L;
     const LdsPlan P = T.Pn;
     if (T.staged_n) stage_in<true, false>(lds, S, L);
-    stk.dbase = (float*)(lds + P.scene_f4 + P.light_f4) + threadIdx.x;
-    stk.base = (uint16_t*)(stk.dbase - threadIdx.x + P.stack_depth * kBlock) + threadIdx.x;
-    stk.stride = (uint32_t)kBlock;
+    place_stack(stk, lds, P.scene_f4 + P.light_f4, P.stack_depth);
 }
 
 // Whether scene `sid`'s queue still has items (read by thread 0, the same
@@ -216,14 +211,11 @@ __global__ __launch_bounds__(kBlock) void kb_closure(const BSlot* __restrict__ s
                 live = true;
             }
             if (__ballot(live) == 0) break;
-            if (live) {
-                DistHit h;
-                int st = m.step(S, h, stk);
-                if (st) {
-                    if (st == 2) atomicAdd(&open[row], 1u);
-                    acc += m.steps;
-                    live = false;
-                }
+            const int st = trace_step(S, m, live, stk);
+            if (st) {
+                if (st == 2) atomicAdd(&open[row], 1u);
+                acc += m.steps;
+                live = false;
             }
         }
         count_steps(evals + 4 * sid, acc);
@@ -272,13 +264,11 @@ __global__ __launch_bounds__(kBlock) void kb_visibility(
                 live = true;
             }
             if (__ballot(live) == 0) break;
-            if (live) {
-                float vis;
-                if (m.step(S, vis, stk)) {
-                    cand_v[slot] = vis <= 1e-3f ? 0.0f : ff * vis;
-                    acc += m.steps;
-                    live = false;
-                }
+            float vis = 0.0f;
+            if (shadow_step(S, m, live, vis, stk)) {
+                cand_v[slot] = vis <= 1e-3f ? 0.0f : ff * vis;
+                acc += m.steps;
+                live = false;
             }
         }
         count_steps(evals + 4 * sid, acc);
@@ -692,11 +682,11 @@ int Batch::build_slots(std::string& err) {
         // LDS plans (as Engine::plan_lds)
         const uint32_t n_obj = (uint32_t)sc.objects.size();
         const uint32_t scene_f4 = (uint32_t)(sc.obj_rec.size() + sc.plane_rec.size() + sc.sub_rec.size() +
-                                             sc.node_rec.size()) + (n_obj + 3u) / 4u;
+                                             sc.node_rec.size()) + (2u * n_obj + 3u) / 4u;
         const uint32_t light_f4 =
             (uint32_t)(sc.quad_rec.size() + sc.sph_rec.size() + sc.pnt_rec.size() + sc.mat_rec.size());
         const uint32_t depth = sc.max_stack ? sc.max_stack : 1u;
-        size_t stack = ((size_t)kBlock * depth * 6u + 15u) & ~(size_t)15u;
+        const size_t stack = stack_bytes(depth);
         size_t bn = (size_t)scene_f4 * 16u + stack, bl = (size_t)(scene_f4 + light_f4) * 16u + stack;
         B.staged_n = bn <= kLdsBudget ? 1u : 0u;
         B.staged_l = bl <= kLdsBudget ? 1u : 0u;

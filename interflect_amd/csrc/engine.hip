@@ -248,13 +248,11 @@ __global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView 
             live = true;
         }
         if (__ballot(live) == 0) break;
-        if (live) {
-            float vis;
-            if (m.step(S, vis, stk)) {
-                cand_v[slot] = vis <= 1e-3f ? 0.0f : ff * vis;
-                acc += m.steps;
-                live = false;
-            }
+        float vis = 0.0f;
+        if (shadow_step(S, m, live, vis, stk)) {
+            cand_v[slot] = vis <= 1e-3f ? 0.0f : ff * vis;
+            acc += m.steps;
+            live = false;
         }
     }
     count_steps(evals, acc);
@@ -296,14 +294,11 @@ __global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg,
             live = true;
         }
         if (__ballot(live) == 0) break;
-        if (live) {
-            DistHit h;
-            int st = m.step(S, h, stk);
-            if (st) {
-                if (st == 2) atomicAdd(&open[row], 1u);
-                acc += m.steps;
-                live = false;
-            }
+        const int st = trace_step(S, m, live, stk);
+        if (st) {
+            if (st == 2) atomicAdd(&open[row], 1u);
+            acc += m.steps;
+            live = false;
         }
     }
     count_steps(evals, acc);
@@ -633,12 +628,11 @@ Engine::Lds Engine::plan_lds(bool lights) const {
     const HostScene& s = scene;
     const uint32_t n_obj = (uint32_t)s.objects.size();
     uint32_t scene_f4 = (uint32_t)(s.obj_rec.size() + s.plane_rec.size() + s.sub_rec.size() + s.node_rec.size()) +
-                        (n_obj + 3u) / 4u;
+                        (2u * n_obj + 3u) / 4u;
     uint32_t light_f4 =
         lights ? (uint32_t)(s.quad_rec.size() + s.sph_rec.size() + s.pnt_rec.size() + s.mat_rec.size()) : 0u;
     const uint32_t depth = s.max_stack ? s.max_stack : 1u;
-    size_t stack = (size_t)kBlock * depth * 6u;  // f32 distances, then u16 node indices
-    stack = (stack + 15u) & ~(size_t)15u;
+    const size_t stack = stack_bytes(depth);
     Lds r;
     size_t total = (size_t)(scene_f4 + light_f4) * 16u + stack;
     if (use_lds && total <= kLdsBudget) {

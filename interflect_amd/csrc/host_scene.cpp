@@ -413,11 +413,16 @@ void finalize_scene(HostScene& s) {
 
     // Objects in leaf order so a leaf's range indexes the records directly.
     s.obj_rec.assign(4 * s.objects.size(), make_float4(0, 0, 0, 0));
-    s.lin.assign(s.objects.size(), 0);
+    // lin[i] = slot of object i (linear scan order); lin[n + k] = leaf node of slot k.
+    const size_t n_obj = s.objects.size();
+    s.lin.assign(2 * n_obj, 0);
     for (size_t slot = 0; slot < s.order.size(); ++slot) {
         object_rec(s.objects[s.order[slot]], &s.obj_rec[4 * slot]);
         s.lin[s.order[slot]] = (uint32_t)slot;
     }
+    for (size_t ni = 0; ni < nodes.size(); ++ni)
+        for (uint32_t k = nodes[ni].start; nodes[ni].count > 0 && k < nodes[ni].start + nodes[ni].count; ++k)
+            s.lin[n_obj + k] = (uint32_t)ni;
     s.plane_rec.assign(4 * s.planes.size(), make_float4(0, 0, 0, 0));
     for (size_t i = 0; i < s.planes.size(); ++i) object_rec(s.planes[i], &s.plane_rec[4 * i]);
     s.blend_k = 0.0f;

@@ -75,7 +75,7 @@ struct HostScene {
     std::vector<float4> sub_rec;       // 4 per subtraction primitive
     std::vector<float4> node_rec;      // 2 per BVH node
     std::vector<uint32_t> order;       // BVH leaf slot -> object index
-    std::vector<uint32_t> lin;         // object index -> leaf slot
+    std::vector<uint32_t> lin;         // object index -> leaf slot, then slot -> BVH leaf node
     std::vector<float4> quad_rec, sph_rec, pnt_rec, mat_rec;
 
     float diag() const { return len3(hi - lo); }

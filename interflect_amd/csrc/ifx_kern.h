@@ -66,7 +66,7 @@ __device__ __forceinline__ void stage_in(float4* lds, SceneView& S, LightsView& 
         copy_f4(q, S.nodes, n);
         S.nodes = q;
         q += n;
-        n = (S.n_obj + 3u) / 4u;
+        n = (2u * S.n_obj + 3u) / 4u;  // slot table + leaf table
         copy_f4(q, (const float4*)S.lin, n);
         S.lin = (const uint32_t*)q;
         q += n;
@@ -108,6 +108,181 @@ __device__ __forceinline__ void stk_flush(CountStack& s, unsigned long long* ops
     if (s.ops) atomicAdd(ops, s.ops);
 }
 
+// LDS bytes of a block's lane columns: per lane `depth` f32 distances, then
+// `depth` u16 node indices.
+inline size_t stack_bytes(uint32_t depth) { return ((size_t)kBlock * depth * 6u + 15u) & ~(size_t)15u; }
+
+// This lane's columns, after `f4` staged float4s.
+template <class Stk>
+__device__ __forceinline__ void place_stack(Stk& stk, float4* lds, uint32_t f4, uint32_t depth) {
+    stk.dbase = (float*)(lds + f4) + threadIdx.x;
+    stk.base = (uint16_t*)((float*)(lds + f4) + depth * kBlock) + threadIdx.x;
+    stk.stride = (uint32_t)kBlock;
+}
+
+// ---------------------------------------------------------------- wavefront distance query
+// The distance of src/scene.rs:191-208 (the value dist() returns) at one
+// point per lane, answered by one BVH traversal shared by the wavefront:
+// the node sequence is uniform, a node is opened when any lane that needs a
+// value could find a relevant object under it, and every lane evaluates every
+// object of an opened leaf, so the loop never diverges and each object record
+// is one kind for the whole wavefront.  Lanes with `need` false only follow
+// along.  All 64 lanes must call it together (uniform control flow).
+//
+// Equality with the per-lane traversal:
+//  * SDF values are exact distances or larger (src/sdf.rs primitives;
+//    subtractions only raise them), so an object is no nearer than its leaf
+//    box, up to f32 rounding; delta = 1e-4 (1 + max|p|) bounds that rounding.
+//    A node is opened while its box distance < the running bound B + 2 delta,
+//    with B = max(hard, 0) without blending and min(hard, smin) + 1.25 K with
+//    it; so every object left unevaluated is at least B_final + delta away,
+//    where it can change neither the nearest hard object nor (farther than
+//    min(hard, smin) + 1.25 K) the blend pair's result.  Each lane thus holds
+//    the minima over all objects that matter.
+//  * The per-lane traversal returns the minimum over the objects it visits,
+//    which can miss an object whose value rounds below its leaf box distance
+//    when its leaf is pruned.  It visits a leaf whose box distance is below
+//    its final prune threshold (running thresholds only fall, ancestors' box
+//    distances are no larger in f32).  Each lane checks that the leaves of
+//    the objects deciding its result pass that test (without blending: box
+//    distance <= result, or below every other object's value and the bound
+//    of the unevaluated ones); a lane whose check fails takes the per-lane
+//    traversal, so every lane returns dist() bit for bit.
+template <class Stk>
+__device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, Stk& stk) {
+    if (!S.use_bvh || S.n_nodes == 0) return need ? dist(S, p, stk) : 0.0f;
+    if (__ballot(need) == 0ull) return 0.0f;
+    constexpr uint32_t kNone = 0xffffffffu;
+    const bool blend = S.blend_k > 0.0f;
+    const float K = S.blend_k;
+    const float delta = 1e-4f * (1.0f + maxe(abs3(p)));
+    float hard = kFltMax, b1 = kFltMax, b2 = kFltMax, h2 = kFltMax;
+    uint32_t a = kNone, la = kNone, s1 = kNone, s2 = kNone;
+    for (uint32_t i = 0; i < S.n_planes; ++i) {
+        uint32_t mat;
+        bool bl;
+        float d = obj_dist(S.planes + 4 * i, S.subs, p, mat, bl, stk);
+        if (d < hard) hard = d;
+    }
+    // The stack holds squared box distances: the opening test only compares
+    // them with the bound, and the margins above absorb the rounding.
+    auto box2 = [&](uint32_t n) {
+        const f3 e = max3(max3(ld3(S.nodes[2 * n]) - p, p - ld3(S.nodes[2 * n + 1])), spl(0.0f));
+        return dot3(e, e);
+    };
+    int sp = 0;
+    stk.put(sp++, 0u, box2(0u));
+    while (sp > 0) {
+        float dn2;
+        const uint32_t ni = __builtin_amdgcn_readfirstlane(stk.get(--sp, dn2));
+        const float bound = blend ? (fmn(hard, smin_poly(b1, b2, K)) + 1.25f * K) + 2.0f * delta
+                                  : fmx(hard, 0.0f) + 2.0f * delta;
+        if (__ballot(need && bound > 0.0f && dn2 < bound * bound) == 0ull) continue;
+        const float4 lo = S.nodes[2 * ni], hi = S.nodes[2 * ni + 1];
+        const uint32_t w0 = __builtin_amdgcn_readfirstlane(fbits(lo.w));
+        const uint32_t cnt = __builtin_amdgcn_readfirstlane(fbits(hi.w));
+        if (cnt > 0) {
+            for (uint32_t k = w0; k < w0 + cnt; ++k) {
+                uint32_t mat;
+                bool bl;
+                const float d = obj_dist(S.obj + 4 * k, S.subs, p, mat, bl, stk);
+                if (blend && bl) {
+                    if (d < b1) {
+                        b2 = b1, s2 = s1;
+                        b1 = d, s1 = k;
+                    } else if (d < b2) {
+                        b2 = d, s2 = k;
+                    }
+                } else if (d < hard) {
+                    if (ni != la) h2 = hard;  // everything evaluated so far lies outside this leaf
+                    hard = d, a = k, la = ni;
+                } else if (ni != la) {
+                    h2 = fmn(h2, d);
+                }
+            }
+        } else {
+            const uint32_t l = ni + 1, r = w0;
+            const float dl = box2(l), dr = box2(r);
+            // open first the child nearer for most lanes
+            const uint32_t nl = (uint32_t)__popcll(__ballot(need && dl < dr));
+            const uint32_t nn = (uint32_t)__popcll(__ballot(need));
+            if (2u * nl >= nn) {
+                stk.put(sp++, r, dr);
+                stk.put(sp++, l, dl);
+            } else {
+                stk.put(sp++, l, dl);
+                stk.put(sp++, r, dr);
+            }
+        }
+    }
+    if (!need) return 0.0f;
+    auto leaf_box = [&](uint32_t slot) {
+        const uint32_t n = S.lin[S.n_obj + slot];
+        return aabb_dist(p, S.nodes[2 * n], S.nodes[2 * n + 1]);
+    };
+    bool ok = true;
+    float res;
+    if (!blend) {
+        res = hard;
+        if (a != kNone) {
+            const float lb = leaf_box(a);
+            ok = lb <= hard || (lb < h2 && lb < fmx(hard, 0.0f) + delta);
+        }
+    } else {
+        const float sm = smin_poly(b1, b2, K);
+        res = sm < hard ? sm : hard;
+        // the per-lane traversal's final prune threshold, lowered by a few ulps of its terms
+        float thr = fmn(hard, sm) + K;
+        thr = thr - 1e-6f * (fabsf(thr) + K + fabsf(res));
+        if (!(sm < hard) && a != kNone && !(leaf_box(a) < thr)) ok = false;
+        if (sm < hard + 1e-5f * (fabsf(hard) + K)) {  // the blend pair decides (or nearly)
+            if (s1 != kNone && !(leaf_box(s1) < thr)) ok = false;
+            if (b2 - b1 < K && s2 != kNone && !(leaf_box(s2) < thr)) ok = false;
+        }
+    }
+    return ok ? res : dist(S, p, stk);
+}
+
+// One sample of every live lane's shadow march.  True for a live lane whose
+// march has ended, with `vis` set.  The instrumented variant (CountStack)
+// keeps the per-lane traversal, so operation counts stay those of the
+// reference algorithm.
+__device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, bool live, float& vis,
+                                            LdsStack& stk) {
+    f3 p = spl(0.0f);
+    bool ended = false, need = false;
+    if (live) {
+        ended = m.pre(vis, p);
+        need = !ended;
+    }
+    const float h = wave_dist(S, p, need, stk);
+    if (need) ended = m.post(h, vis);
+    return live && ended;
+}
+__device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, bool live, float& vis,
+                                            CountStack& stk) {
+    return live && m.step(S, vis, stk);
+}
+
+// One sample of every live lane's sphere trace when only hit / miss is
+// needed: 0 marching, 1 hit, 2 miss (for a lane that is not live: 0).
+__device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, bool live, LdsStack& stk) {
+    f3 p = spl(0.0f);
+    int st = 0;
+    bool need = false;
+    if (live) {
+        if (m.pre(p)) st = 2;
+        else need = true;
+    }
+    const float h = wave_dist(S, p, need, stk);
+    if (need) st = m.post(h);
+    return st;
+}
+__device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, bool live, CountStack& stk) {
+    DistHit h;
+    return live ? m.step(S, h, stk) : 0;
+}
+
 // Kernel prologue: stage, then place this lane's stack column after the staged records.
 #define IFX_SETUP(LDS, LIGHTS, COUNT)                                            \
     extern __shared__ float4 ifx_lds[];                                         \
@@ -115,10 +290,7 @@ __device__ __forceinline__ void stk_flush(CountStack& s, unsigned long long* ops
     LightsView L = Lg;                                                          \
     stage_in<LDS, LIGHTS>(ifx_lds, S, L);                                       \
     typename StackOf<COUNT>::T stk;                                             \
-    stk.dbase = (float*)(ifx_lds + P.scene_f4 + P.light_f4) + threadIdx.x;      \
-    stk.base = (uint16_t*)(stk.dbase - threadIdx.x + P.stack_depth * kBlock) +  \
-               threadIdx.x;                                                     \
-    stk.stride = (uint32_t)kBlock;                                              \
+    place_stack(stk, ifx_lds, P.scene_f4 + P.light_f4, P.stack_depth);          \
     stk_init(stk);                                                              \
     (void)L
 

@@ -53,7 +53,7 @@ struct SceneView {
     const float4* planes;  // unbounded objects, always evaluated first
     const float4* subs;    // CSG subtraction primitives
     const float4* nodes;
-    const uint32_t* lin;   // linear-scan order: lin[i] = slot of object i
+    const uint32_t* lin;   // lin[i] = slot of object i (linear scan); lin[n_obj + k] = leaf node of slot k
     uint32_t n_obj, n_planes, n_nodes, use_bvh;
     uint32_t n_subs;
     float blend_k;         // EXT blend-group radius, 0 = reference behaviour

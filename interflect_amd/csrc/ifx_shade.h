@@ -53,16 +53,27 @@ struct TraceMarch {
     // 2 on a miss.  On a hit `steps` already includes the normal's 4 taps.
     template <class Stk>
     IFX_HD int step(const SceneView& S, DistHit& h, Stk& stk) {
-        if (steps >= kMaxSteps) return 2;
-        f3 p = ro + rd * t;
+        f3 p;
+        if (pre(p)) return 2;
         h = dist_mat(S, p, stk);
+        return post(h.d);
+    }
+    // The step in two halves around the distance query (for callers that
+    // need only the distance): pre is true when the step budget is spent
+    // (a miss), else gives the sample point; post takes the distance there.
+    IFX_HD bool pre(f3& p) const {
+        if (steps >= kMaxSteps) return true;
+        p = ro + rd * t;
+        return false;
+    }
+    IFX_HD int post(float d) {
         steps += 1;
         float eps = 1e-4f * fmx(t, 1.0f);
-        if (h.d < eps) {
+        if (d < eps) {
             steps += 4;
             return 1;
         }
-        t += h.d;
+        t += d;
         return t > tmax ? 2 : 0;
     }
 };
@@ -110,14 +121,19 @@ struct ShadowMarch {
         res = 1.0f;
         steps = 0;
     }
-    // One sample.  Returns true when the march has ended, with `vis` set.
-    template <class Stk>
-    IFX_HD bool step(const SceneView& S, float& vis, Stk& stk) {
+    // A step in two halves around the distance query, so a caller can
+    // answer the query for a whole wavefront at once.  pre: true when the
+    // march has ended (vis set), else the sample point.
+    IFX_HD bool pre(float& vis, f3& p) const {
         if (!(steps < kShadowSteps && t < tmax)) {
             vis = clmp(res, 0.0f, 1.0f);
             return true;
         }
-        float h = dist(S, ro + rd * t, stk);
+        p = ro + rd * t;
+        return false;
+    }
+    // post: consumes the distance h at the sample; true when the march has ended.
+    IFX_HD bool post(float h, float& vis) {
         steps += 1;
         if (h < 1e-4f) {
             vis = 0.0f;
@@ -139,6 +155,13 @@ struct ShadowMarch {
         ph = h;
         t += clmp(h, 0.006f, 0.12f);
         return false;
+    }
+    // One sample.  Returns true when the march has ended, with `vis` set.
+    template <class Stk>
+    IFX_HD bool step(const SceneView& S, float& vis, Stk& stk) {
+        f3 p;
+        if (pre(vis, p)) return true;
+        return post(dist(S, p, stk), vis);
     }
 };
 
