@@ -148,13 +148,13 @@ __global__ __launch_bounds__(kBlock) void kb_ff(const float4* __restrict__ spos,
 }
 
 // ---------------------------------------------------------------- tracing (block-partitioned)
-// Per-scene setup inside a persistent block: views, LDS staging when the
-// scene fits, stack placement.  Every thread of the block calls it.
+// Per-scene setup inside a persistent block: views (records in global
+// memory, as wave_dist requires), stack placement.  Every thread of the block
+// calls it.
 __device__ __forceinline__ void bscene(const BSlot& T, float4* lds, SceneView& S, LightsView& L, LdsStack& stk) {
     S = T.S;
     L = T.L;
     const LdsPlan P = T.Pn;
-    if (T.staged_n) stage_in<true, false>(lds, S, L);
     place_stack(stk, lds, P.scene_f4 + P.light_f4, P.stack_depth);
 }
 
@@ -687,12 +687,15 @@ int Batch::build_slots(std::string& err) {
             (uint32_t)(sc.quad_rec.size() + sc.sph_rec.size() + sc.pnt_rec.size() + sc.mat_rec.size());
         const uint32_t depth = sc.max_stack ? sc.max_stack : 1u;
         const size_t stack = stack_bytes(depth);
-        size_t bn = (size_t)scene_f4 * 16u + stack, bl = (size_t)(scene_f4 + light_f4) * 16u + stack;
-        B.staged_n = bn <= kLdsBudget ? 1u : 0u;
+        const size_t bl = (size_t)(scene_f4 + light_f4) * 16u + stack;
+        // The closure / visibility kernels (plan n) share one traversal per
+        // wavefront and read records through the scalar cache from global
+        // memory: their scenes are never staged, LDS holds only the stacks.
+        B.staged_n = 0u;
         B.staged_l = bl <= kLdsBudget ? 1u : 0u;
-        B.Pn = B.staged_n ? LdsPlan{scene_f4, 0u, depth} : LdsPlan{0u, 0u, depth};
+        B.Pn = LdsPlan{0u, 0u, depth};
         B.Pl = B.staged_l ? LdsPlan{scene_f4, light_f4, depth} : LdsPlan{0u, 0u, depth};
-        bytes_n[s] = B.staged_n ? bn : stack;
+        bytes_n[s] = stack;
         bytes_l[s] = B.staged_l ? bl : stack;
         // offsets and sizes
         const uint32_t N = (uint32_t)scenes_[s].surfels.size();

@@ -160,7 +160,9 @@ class Engine {
         size_t bytes;
         bool staged;
     };
-    Lds plan_lds(bool lights) const;
+    // stage = false: records stay in global memory (kernels whose traversal is
+    // wavefront-shared read them through the scalar cache); LDS holds the stacks.
+    Lds plan_lds(bool lights, bool stage = true) const;
     int shade_setup(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, ShadeArgs& A, std::string& err);
     CamV cam_basis(const HCamera& cam, uint32_t W, uint32_t H) const;
     int adjoint_prepare(std::string& err);

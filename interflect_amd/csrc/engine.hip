@@ -624,7 +624,7 @@ float host_srgb(float c) {
 }
 }  // namespace
 
-Engine::Lds Engine::plan_lds(bool lights) const {
+Engine::Lds Engine::plan_lds(bool lights, bool stage) const {
     const HostScene& s = scene;
     const uint32_t n_obj = (uint32_t)s.objects.size();
     uint32_t scene_f4 = (uint32_t)(s.obj_rec.size() + s.plane_rec.size() + s.sub_rec.size() + s.node_rec.size()) +
@@ -635,7 +635,7 @@ Engine::Lds Engine::plan_lds(bool lights) const {
     const size_t stack = stack_bytes(depth);
     Lds r;
     size_t total = (size_t)(scene_f4 + light_f4) * 16u + stack;
-    if (use_lds && total <= kLdsBudget) {
+    if (use_lds && stage && total <= kLdsBudget) {
         r.plan = LdsPlan{scene_f4, light_f4, depth};
         r.bytes = total;
         r.staged = true;
@@ -982,7 +982,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     CK(d_open.ensure(4ull * N));
     CK(hipMemsetAsync(d_open.p, 0, 4ull * N, stream));
     {
-        const Lds ld = plan_lds(false);
+        const Lds ld = plan_lds(false, false);
         const LightsView Lg{};
         const uint32_t grid = std::max(1u, std::min<uint32_t>(persist, blocks_for(16ull * N, kBlock)));
         launch("closure", [&] {
@@ -1054,7 +1054,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     if (M) {
         const uint32_t visb =
             vis_blocks ? vis_blocks : std::max(1u, std::min<uint32_t>(persist, blocks_for(M, kBlock)));
-        const Lds ld = plan_lds(false);
+        const Lds ld = plan_lds(false, false);
         const LightsView Lg{};
         launch("visibility", [&] {
             LAUNCH_CL(k_visibility, count_ops_, ld.staged, visb, ld.bytes, S, Lg, ld.plan, spos, snrm,

@@ -148,9 +148,30 @@ __device__ __forceinline__ void place_stack(Stk& stk, float4* lds, uint32_t f4, 
 //    distance <= result, or below every other object's value and the bound
 //    of the unevaluated ones); a lane whose check fails takes the per-lane
 //    traversal, so every lane returns dist() bit for bit.
+//  Records are read through constant-address-space pointers: with the node
+//  and object indices uniform, those reads are scalar loads.  S must then
+//  point to global memory (the scene not staged in LDS).
+typedef float __attribute__((ext_vector_type(4))) NativeF4;
+struct ConstF4 {
+    const __attribute__((address_space(4))) NativeF4* q;
+    __device__ __forceinline__ float4 operator[](uint32_t i) const {
+        const NativeF4 v = q[i];
+        return make_float4(v.x, v.y, v.z, v.w);
+    }
+    __device__ __forceinline__ ConstF4 operator+(uint32_t i) const { return ConstF4{q + i}; }
+};
+__device__ __forceinline__ ConstF4 as_const(const float4* p) {
+    return ConstF4{(const __attribute__((address_space(4))) NativeF4*)p};
+}
+
 template <class Stk>
 __device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, Stk& stk) {
-    if (!S.use_bvh || S.n_nodes == 0) return need ? dist(S, p, stk) : 0.0f;
+    // (records staged in LDS cannot be read through the constant space: the
+    // per-lane traversal then answers)
+    if (!S.use_bvh || S.n_nodes == 0 || __builtin_amdgcn_is_shared((const __attribute__((address_space(0))) void*)S.nodes))
+        return need ? dist(S, p, stk) : 0.0f;
+    const ConstF4 nodes = as_const(S.nodes), objs = as_const(S.obj), subs = as_const(S.subs);
+    const ConstF4 planes = as_const(S.planes);
     if (__ballot(need) == 0ull) return 0.0f;
     constexpr uint32_t kNone = 0xffffffffu;
     const bool blend = S.blend_k > 0.0f;
@@ -161,13 +182,13 @@ __device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, 
     for (uint32_t i = 0; i < S.n_planes; ++i) {
         uint32_t mat;
         bool bl;
-        float d = obj_dist(S.planes + 4 * i, S.subs, p, mat, bl, stk);
+        float d = obj_dist_at(planes + 4 * i, subs, p, mat, bl, stk);
         if (d < hard) hard = d;
     }
     // The stack holds squared box distances: the opening test only compares
     // them with the bound, and the margins above absorb the rounding.
     auto box2 = [&](uint32_t n) {
-        const f3 e = max3(max3(ld3(S.nodes[2 * n]) - p, p - ld3(S.nodes[2 * n + 1])), spl(0.0f));
+        const f3 e = max3(max3(ld3(nodes[2 * n]) - p, p - ld3(nodes[2 * n + 1])), spl(0.0f));
         return dot3(e, e);
     };
     int sp = 0;
@@ -178,14 +199,12 @@ __device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, 
         const float bound = blend ? (fmn(hard, smin_poly(b1, b2, K)) + 1.25f * K) + 2.0f * delta
                                   : fmx(hard, 0.0f) + 2.0f * delta;
         if (__ballot(need && bound > 0.0f && dn2 < bound * bound) == 0ull) continue;
-        const float4 lo = S.nodes[2 * ni], hi = S.nodes[2 * ni + 1];
-        const uint32_t w0 = __builtin_amdgcn_readfirstlane(fbits(lo.w));
-        const uint32_t cnt = __builtin_amdgcn_readfirstlane(fbits(hi.w));
+        const uint32_t w0 = fbits(nodes[2 * ni].w), cnt = fbits(nodes[2 * ni + 1].w);
         if (cnt > 0) {
             for (uint32_t k = w0; k < w0 + cnt; ++k) {
                 uint32_t mat;
                 bool bl;
-                const float d = obj_dist(S.obj + 4 * k, S.subs, p, mat, bl, stk);
+                const float d = obj_dist_at(objs + 4 * k, subs, p, mat, bl, stk);
                 if (blend && bl) {
                     if (d < b1) {
                         b2 = b1, s2 = s1;

@@ -123,10 +123,11 @@ IFX_HD float prim_dist(float4 g0, float4 g1, float4 g2, f3 p) {
     }
 }
 
-// src/sdf.rs:164-173: base primitive, then each subtraction in order.
-template <class Stk>
-IFX_HD float obj_dist(const float4* __restrict__ rec, const float4* __restrict__ subs, f3 p, uint32_t& mat,
-                      bool& blended, Stk& stk) {
+// src/sdf.rs:164-173: base primitive, then each subtraction in order.  P is
+// the record pointer type (a constant-address-space pointer lets uniform
+// reads go through the scalar cache).
+template <class P, class Stk>
+IFX_HD float obj_dist_at(P rec, P subs, f3 p, uint32_t& mat, bool& blended, Stk& stk) {
     float4 g0 = rec[0], g1 = rec[1], g2 = rec[2], g3 = rec[3];
     mat = fbits(g3.x);
     uint32_t sb = fbits(g3.y), sc = fbits(g3.z);
@@ -134,11 +135,17 @@ IFX_HD float obj_dist(const float4* __restrict__ rec, const float4* __restrict__
     float d = prim_dist(g0, g1, g2, p);
     stk.count_prim(fbits(g2.z), 0u);
     for (uint32_t s = sb; s < sb + sc; ++s) {
-        const float4* q = subs + 4 * s;
+        P q = subs + 4 * s;
         d = fmx(d, -prim_dist(q[0], q[1], q[2], p));
         stk.count_prim(fbits(q[2].z), 2u);
     }
     return d;
+}
+
+template <class Stk>
+IFX_HD float obj_dist(const float4* __restrict__ rec, const float4* __restrict__ subs, f3 p, uint32_t& mat,
+                      bool& blended, Stk& stk) {
+    return obj_dist_at(rec, subs, p, mat, blended, stk);
 }
 
 // src/bvh.rs:45-50
