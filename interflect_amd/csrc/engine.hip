@@ -294,7 +294,9 @@ __global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg,
             live = true;
         }
         if (__ballot(live) == 0) break;
-        const int st = trace_step(S, m, live, stk);
+        uint32_t mat = 0;
+        bool have_mat = false;
+        const int st = trace_step(S, m, live, mat, have_mat, stk);
         if (st) {
             if (st == 2) atomicAdd(&open[row], 1u);
             acc += m.steps;
@@ -554,10 +556,10 @@ __global__ void k_grid_keys(const float4* __restrict__ spos, uint32_t N, f3 lo, 
 }
 
 // 16x16 pixel blocks, one 8x8 tile per wavefront.
-template <bool COUNT, bool LDS>
+template <bool COUNT, bool LDS, bool WAVE>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneView Sg, LightsView Lg, LdsPlan P, ShadeArgs A, float4* film,
                                                   unsigned long long* evals, unsigned long long* ops) {
-    IFX_SETUP(LDS, true, COUNT);
+    IFX_SETUP_W(LDS, true, COUNT, WAVE);
     uint32_t wv = threadIdx.x >> 6, l = threadIdx.x & 63u;
     uint32_t x = blockIdx.x * 16u + (wv & 1u) * 8u + (l & 7u);
     uint32_t y = blockIdx.y * 16u + (wv >> 1) * 8u + (l >> 3);
@@ -572,11 +574,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView Sg, LightsView Lg, L
 
 // Many cameras over one solve (the turntable, src/main.rs:492-520): grid z
 // is the view; each view is k_shade with its own camera.
-template <bool COUNT, bool LDS>
+template <bool COUNT, bool LDS, bool WAVE>
 __global__ __launch_bounds__(kBlock) void k_shade_views(SceneView Sg, LightsView Lg, LdsPlan P, ShadeArgs A0,
                                                         const CamV* __restrict__ cams, float4* film,
                                                         unsigned long long* evals, unsigned long long* ops) {
-    IFX_SETUP(LDS, true, COUNT);
+    IFX_SETUP_W(LDS, true, COUNT, WAVE);
     ShadeArgs A = A0;
     A.cam = cams[blockIdx.z];
     uint32_t wv = threadIdx.x >> 6, l = threadIdx.x & 63u;
@@ -746,6 +748,7 @@ SceneView Engine::dev_view() const {
     v.nodes = d_nodes.as<float4>();
     v.lin = d_lin.as<uint32_t>();
     v.use_bvh = use_bvh ? 1u : 0u;
+    v.keep_global();
     return v;
 }
 
@@ -1361,8 +1364,8 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
     dim3 grid((W + 15) / 16, (H + 15) / 16);
     const Lds ld = plan_lds(true);
     launch("shade", [&] {
-        LAUNCH_CL(k_shade, count_ops_, ld.staged, grid, ld.bytes, S, Lv, ld.plan, A, d_film.as<float4>(), ctr,
-                  count_ops_ ? ctr + 4 : nullptr);
+        LAUNCH_CLW(k_shade, count_ops_, ld.staged, S.n_obj >= kWaveShadeMinObjs, grid, ld.bytes, S, Lv, ld.plan, A,
+                   d_film.as<float4>(), ctr, count_ops_ ? ctr + 4 : nullptr);
     });
     const bool tonemapped = mode_tonemapped(mode);
     if (rgb8) {
@@ -1430,8 +1433,8 @@ int Engine::shade_views(const std::vector<HCamera>& cams, uint32_t W, uint32_t H
     dim3 grid((W + 15) / 16, (H + 15) / 16, nv);
     const Lds ld = plan_lds(true);
     launch("shade_views", [&] {
-        LAUNCH_CL(k_shade_views, count_ops_, ld.staged, grid, ld.bytes, S, Lv, ld.plan, A, d_cams.as<CamV>(),
-                  d_film.as<float4>(), ctr, count_ops_ ? ctr + 4 : nullptr);
+        LAUNCH_CLW(k_shade_views, count_ops_, ld.staged, S.n_obj >= kWaveShadeMinObjs, grid, ld.bytes, S, Lv,
+                   ld.plan, A, d_cams.as<CamV>(), d_film.as<float4>(), ctr, count_ops_ ? ctr + 4 : nullptr);
     });
     if (rgb8) {
         launch("encode", [&] {

@@ -492,6 +492,7 @@ SceneView HostScene::host_view() const {
     v.use_bvh = 1;
     v.n_subs = (uint32_t)(sub_rec.size() / 4);
     v.blend_k = blend_k;
+    v.keep_global();
     return v;
 }
 

@@ -101,6 +101,26 @@ template <>
 struct StackOf<true> {
     using T = CountStack;
 };
+
+// Stacks of the camera-ray shading kernels: LdsStack (wavefront-shared
+// queries) or LaneStack, an LdsStack whose march loops keep the per-lane
+// traversal.
+struct LaneStack : LdsStack {
+    static constexpr bool kShared = false;
+};
+
+template <bool COUNT, bool WAVE>
+struct ShadeStackOf {
+    using T = LdsStack;
+};
+template <>
+struct ShadeStackOf<false, false> {
+    using T = LaneStack;
+};
+template <bool WAVE>
+struct ShadeStackOf<true, WAVE> {
+    using T = CountStack;
+};
 __device__ __forceinline__ void stk_init(LdsStack&) {}
 __device__ __forceinline__ void stk_init(CountStack& s) { s.ops = 0ull; }
 __device__ __forceinline__ void stk_flush(LdsStack&, unsigned long long*) {}
@@ -148,9 +168,10 @@ __device__ __forceinline__ void place_stack(Stk& stk, float4* lds, uint32_t f4, 
 //    distance <= result, or below every other object's value and the bound
 //    of the unevaluated ones); a lane whose check fails takes the per-lane
 //    traversal, so every lane returns dist() bit for bit.
-//  Records are read through constant-address-space pointers: with the node
-//  and object indices uniform, those reads are scalar loads.  S must then
-//  point to global memory (the scene not staged in LDS).
+//  Records are read from their global copies (S.g_*) through
+//  constant-address-space pointers: with the node and object indices uniform,
+//  those reads are scalar loads.  The per-lane fallback reads S as given
+//  (LDS when the scene is staged).
 typedef float __attribute__((ext_vector_type(4))) NativeF4;
 struct ConstF4 {
     const __attribute__((address_space(4))) NativeF4* q;
@@ -166,12 +187,9 @@ __device__ __forceinline__ ConstF4 as_const(const float4* p) {
 
 template <class Stk>
 __device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, Stk& stk) {
-    // (records staged in LDS cannot be read through the constant space: the
-    // per-lane traversal then answers)
-    if (!S.use_bvh || S.n_nodes == 0 || __builtin_amdgcn_is_shared((const __attribute__((address_space(0))) void*)S.nodes))
-        return need ? dist(S, p, stk) : 0.0f;
-    const ConstF4 nodes = as_const(S.nodes), objs = as_const(S.obj), subs = as_const(S.subs);
-    const ConstF4 planes = as_const(S.planes);
+    if (!S.use_bvh || S.n_nodes == 0) return need ? dist(S, p, stk) : 0.0f;
+    const ConstF4 nodes = as_const(S.g_nodes), objs = as_const(S.g_obj), subs = as_const(S.g_subs);
+    const ConstF4 planes = as_const(S.g_planes);
     if (__ballot(need) == 0ull) return 0.0f;
     constexpr uint32_t kNone = 0xffffffffu;
     const bool blend = S.blend_k > 0.0f;
@@ -262,10 +280,11 @@ __device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, 
     return ok ? res : dist(S, p, stk);
 }
 
-// One sample of every live lane's shadow march.  True for a live lane whose
-// march has ended, with `vis` set.  The instrumented variant (CountStack)
-// keeps the per-lane traversal, so operation counts stay those of the
-// reference algorithm.
+// Wavefront-shared overloads of the march hooks (the per-lane versions and
+// their contracts sit beside the loops): all active lanes call together.
+__device__ __forceinline__ float query_dist(const SceneView& S, f3 p, LdsStack& stk) {
+    return wave_dist(S, p, true, stk);
+}
 __device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, bool live, float& vis,
                                             LdsStack& stk) {
     f3 p = spl(0.0f);
@@ -278,14 +297,9 @@ __device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, 
     if (need) ended = m.post(h, vis);
     return live && ended;
 }
-__device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, bool live, float& vis,
-                                            CountStack& stk) {
-    return live && m.step(S, vis, stk);
-}
-
-// One sample of every live lane's sphere trace when only hit / miss is
-// needed: 0 marching, 1 hit, 2 miss (for a lane that is not live: 0).
-__device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, bool live, LdsStack& stk) {
+// The hit's material is left to the caller (have_mat false).
+__device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, bool live, uint32_t& mat,
+                                          bool& have_mat, LdsStack& stk) {
     f3 p = spl(0.0f);
     int st = 0;
     bool need = false;
@@ -295,20 +309,29 @@ __device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, boo
     }
     const float h = wave_dist(S, p, need, stk);
     if (need) st = m.post(h);
+    mat = 0u;
+    have_mat = false;
     return st;
 }
-__device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, bool live, CountStack& stk) {
-    DistHit h;
-    return live ? m.step(S, h, stk) : 0;
-}
+
+// Camera-ray shading (k_shade, k_shade_views, k_shade_bwd) takes the shared
+// traversal from this many objects up, the per-lane one below:
+// camera rays are coherent already, and in small scenes the shared traversal
+// opens nearly every leaf for every lane (measured on MI355X: an 8-object
+// room shaded 20% slower shared, the 262-object stress frame 12% faster).
+constexpr uint32_t kWaveShadeMinObjs = 16;
 
 // Kernel prologue: stage, then place this lane's stack column after the staged records.
-#define IFX_SETUP(LDS, LIGHTS, COUNT)                                            \
+#define IFX_SETUP(LDS, LIGHTS, COUNT) IFX_SETUP_T(LDS, LIGHTS, typename StackOf<COUNT>::T)
+// Camera-ray shading kernels (see ShadeStackOf).
+#define IFX_SETUP_W(LDS, LIGHTS, COUNT, WAVE) \
+    IFX_SETUP_T(LDS, LIGHTS, typename ShadeStackOf<COUNT, WAVE>::T)
+#define IFX_SETUP_T(LDS, LIGHTS, ...)                                           \
     extern __shared__ float4 ifx_lds[];                                         \
     SceneView S = Sg;                                                           \
     LightsView L = Lg;                                                          \
     stage_in<LDS, LIGHTS>(ifx_lds, S, L);                                       \
-    typename StackOf<COUNT>::T stk;                                             \
+    __VA_ARGS__ stk;                                                            \
     place_stack(stk, ifx_lds, P.scene_f4 + P.light_f4, P.stack_depth);          \
     stk_init(stk);                                                              \
     (void)L
@@ -544,6 +567,43 @@ void Engine::launch(const char* name, F&& f) {
         } else {                                                                         \
             if (LDS)                                                                     \
                 K<false, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);              \
+            else                                                                         \
+                K<false, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);             \
+        }                                                                                \
+    } while (0)
+
+// As LAUNCH_CL with the WAVE choice of the shading kernels (counting runs
+// keep the per-lane traversal).
+#define LAUNCH_CLW(K, COUNT, LDS, WAVE, GRID, DYN, ...)                                   \
+    do {                                                                                 \
+        if (COUNT) {                                                                     \
+            if (LDS)                                                                     \
+                K<true, true, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);        \
+            else                                                                         \
+                K<true, false, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);       \
+        } else if (WAVE) {                                                               \
+            if (LDS)                                                                     \
+                K<false, true, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);        \
+            else                                                                         \
+                K<false, false, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);       \
+        } else {                                                                         \
+            if (LDS)                                                                     \
+                K<false, true, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);       \
+            else                                                                         \
+                K<false, false, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);      \
+        }                                                                                \
+    } while (0)
+
+#define LAUNCH_LW(K, LDS, WAVE, GRID, DYN, ...)                                          \
+    do {                                                                                 \
+        if (WAVE) {                                                                      \
+            if (LDS)                                                                     \
+                K<true, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);               \
+            else                                                                         \
+                K<false, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);              \
+        } else {                                                                         \
+            if (LDS)                                                                     \
+                K<true, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);              \
             else                                                                         \
                 K<false, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);             \
         }                                                                                \

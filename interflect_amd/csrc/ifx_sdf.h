@@ -57,6 +57,15 @@ struct SceneView {
     uint32_t n_obj, n_planes, n_nodes, use_bvh;
     uint32_t n_subs;
     float blend_k;         // EXT blend-group radius, 0 = reference behaviour
+    // The same records in global memory (the pointers above may be moved to
+    // an LDS copy); the wavefront-shared traversal reads these.
+    const float4 *g_obj, *g_planes, *g_subs, *g_nodes;
+    IFX_HD void keep_global() {
+        g_obj = obj;
+        g_planes = planes;
+        g_subs = subs;
+        g_nodes = nodes;
+    }
 };
 
 struct DistHit {
@@ -176,7 +185,10 @@ constexpr uint32_t kEvalOps = 15u;
 // Each entry is a node index and that node's AABB distance, computed by its
 // parent when it was pushed (the reference recomputes the same value on pop,
 // src/bvh.rs:116; keeping it is bit-identical and skips the node's loads).
+// kShared: the device march loops over this stack answer the active lanes'
+// queries with one wavefront-shared traversal (the hooks below).
 struct ArrStack {
+    static constexpr bool kShared = false;
     uint32_t v[64];
     float dv[64];
     IFX_HD void put(int i, uint32_t x, float d) {
@@ -193,6 +205,7 @@ struct ArrStack {
 };
 
 struct LdsStack {
+    static constexpr bool kShared = true;
     uint16_t* base;
     float* dbase;
     uint32_t stride;
@@ -210,6 +223,7 @@ struct LdsStack {
 };
 
 struct CountStack {
+    static constexpr bool kShared = false;
     uint16_t* base;
     float* dbase;
     uint32_t stride;
@@ -343,15 +357,40 @@ IFX_HD float dist(const SceneView& S, f3 p, Stk& stk) {
     return dist_mat(S, p, stk).d;
 }
 
+// Hooks of the march loops: the distance at one point per active lane, and
+// whether any active lane is still marching.  These per-lane versions serve
+// the host and the instrumented device stack; device code with an LDS stack
+// gets wavefront-shared overloads (found by argument-dependent lookup where
+// the loops are instantiated) that answer all active lanes with one
+// traversal and return the same values.
+template <class Stk>
+IFX_HD float query_dist(const SceneView& S, f3 p, Stk& stk) {
+    return dist(S, p, stk);
+}
+
+// The material at a point (a trace's hit when its step did not report it).
+template <class Stk>
+IFX_HD uint32_t query_mat(const SceneView& S, f3 p, Stk& stk) {
+    return dist_mat(S, p, stk).mat;
+}
+
+IFX_HD bool any_lane(bool v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __ballot(v) != 0ull;
+#else
+    return v;
+#endif
+}
+
 // src/scene.rs:218-236: tetrahedral 4-tap gradient.
 template <class Stk>
 IFX_HD f3 scene_normal(const SceneView& S, f3 p, Stk& stk) {
     const float H = 5e-4f;
     f3 n = spl(0.0f);
-    n = n + mk(1.0f, -1.0f, -1.0f) * dist(S, p + mk(1.0f, -1.0f, -1.0f) * H, stk);
-    n = n + mk(-1.0f, -1.0f, 1.0f) * dist(S, p + mk(-1.0f, -1.0f, 1.0f) * H, stk);
-    n = n + mk(-1.0f, 1.0f, -1.0f) * dist(S, p + mk(-1.0f, 1.0f, -1.0f) * H, stk);
-    n = n + mk(1.0f, 1.0f, 1.0f) * dist(S, p + mk(1.0f, 1.0f, 1.0f) * H, stk);
+    n = n + mk(1.0f, -1.0f, -1.0f) * query_dist(S, p + mk(1.0f, -1.0f, -1.0f) * H, stk);
+    n = n + mk(-1.0f, -1.0f, 1.0f) * query_dist(S, p + mk(-1.0f, -1.0f, 1.0f) * H, stk);
+    n = n + mk(-1.0f, 1.0f, -1.0f) * query_dist(S, p + mk(-1.0f, 1.0f, -1.0f) * H, stk);
+    n = n + mk(1.0f, 1.0f, 1.0f) * query_dist(S, p + mk(1.0f, 1.0f, 1.0f) * H, stk);
     float l = len3(n);
     if (l > 1e-12f) return n / l;
     return mk(0.0f, 1.0f, 0.0f);

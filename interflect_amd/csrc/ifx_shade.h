@@ -80,21 +80,55 @@ struct TraceMarch {
 
 // src/trace.rs:27-58.  NORMAL=false skips the 4-tap normal (closure rays
 // discard it); the step count still includes its 4 evaluations.
+// One sample of a live lane's trace: 0 marching, 1 hit, 2 miss (0 for a
+// lane that is not live).  The per-lane version also reports the hit's
+// material (have_mat); see query_dist for the wavefront-shared overload.
+template <class Stk>
+IFX_HD int trace_step(const SceneView& S, TraceMarch& m, bool live, uint32_t& mat, bool& have_mat, Stk& stk) {
+    if (!live) return 0;
+    DistHit h;
+    const int st = m.step(S, h, stk);
+    mat = h.mat;
+    have_mat = true;
+    return st;
+}
+
 template <bool NORMAL, class Stk>
 IFX_HD Hit trace(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, uint32_t& steps_out, Stk& stk) {
     TraceMarch m;
     m.begin(ro, rd, tmin, tmax);
-    DistHit h;
-    int st;
-    while ((st = m.step(S, h, stk)) == 0) {
+    int st = 0;
+    bool have_mat = false;
+    uint32_t mat = 0;
+    if constexpr (Stk::kShared) {
+        bool live = true;
+        while (any_lane(live)) {
+            uint32_t mm = 0;
+            bool hm = false;
+            const int s = trace_step(S, m, live, mm, hm, stk);
+            if (live && s) {
+                st = s;
+                mat = mm;
+                have_mat = hm;
+                live = false;
+            }
+        }
+    } else {
+        DistHit h;
+        while ((st = m.step(S, h, stk)) == 0) {
+        }
+        mat = h.mat;
+        have_mat = true;
     }
     steps_out = m.steps;
     Hit r;
     if (st == 1) {
         r.t = m.t;
         r.p = ro + rd * m.t;
+        // (the sample's material: the same query as the step's)
+        if (!have_mat) mat = query_mat(S, r.p, stk);
         r.n = NORMAL ? scene_normal(S, r.p, stk) : spl(0.0f);
-        r.mat = h.mat;
+        r.mat = mat;
         r.hit = true;
     } else {
         r.t = 0.0f;
@@ -166,13 +200,31 @@ struct ShadowMarch {
 };
 
 // src/trace.rs:84-124
+// One sample of a live lane's shadow march; true when it has ended (vis
+// set).  Per-lane version; see query_dist for the wavefront-shared overload.
+template <class Stk>
+IFX_HD bool shadow_step(const SceneView& S, ShadowMarch& m, bool live, float& vis, Stk& stk) {
+    return live && m.step(S, vis, stk);
+}
+
 template <class Stk>
 IFX_HD float soft_shadow(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, float k, uint32_t& steps_out,
                          Stk& stk) {
     ShadowMarch m;
     m.begin(ro, rd, tmin, tmax, k);
-    float vis;
-    while (!m.step(S, vis, stk)) {
+    float vis = 0.0f;
+    if constexpr (Stk::kShared) {
+        bool live = true;
+        while (any_lane(live)) {
+            float v = 0.0f;
+            if (shadow_step(S, m, live, v, stk)) {
+                vis = v;
+                live = false;
+            }
+        }
+    } else {
+        while (!m.step(S, vis, stk)) {
+        }
     }
     steps_out = m.steps;
     return vis;
