@@ -6,11 +6,12 @@
 // parallel rounds:
 //   1. every seed is projected onto the surface (one lane per seed);
 //   2. the surviving seeds, in seed order, are bucketed by Poisson cell;
-//   3. each round, an undecided seed becomes rejected if an earlier
+//   3. each survivor's earlier conflicting neighbours are listed once;
+//   4. each round, an undecided seed becomes rejected if an earlier
 //      neighbour is accepted, accepted if every earlier neighbour is
 //      rejected, and waits otherwise (rounds read the previous round's
 //      states, so the result does not depend on scheduling);
-//   4. the accepted seeds are compacted in seed order and cut at the target,
+//   5. the accepted seeds are compacted in seed order and cut at the target,
 //      which is where the reference's loop stops.
 #include <hipcub/hipcub.hpp>
 
@@ -55,38 +56,57 @@ __global__ void k_iota(uint32_t* v, uint32_t n) {
     if (i < n) v[i] = i;
 }
 
-// One round over the survivors.  `items` lists the survivors by cell (seed
-// order within a cell); `starts` indexes it by cell.
-__global__ void k_mis_round(uint32_t nv, const float4* __restrict__ vpos, const uint32_t* __restrict__ vcell,
-                            const uint32_t* __restrict__ starts, const uint32_t* __restrict__ items, SurfelParams SP,
+// The conflict graph, once: each survivor's earlier neighbours (lower seed
+// index, closer than d_min) from its 27-cell neighbourhood.  `items` lists
+// the survivors by cell (seed order within a cell); `starts` indexes it by
+// cell.  FILL=false counts them, FILL=true writes them at adj_start[k].
+template <bool FILL>
+__global__ void k_earlier_nbrs(uint32_t nv, const float4* __restrict__ vpos, const uint32_t* __restrict__ vcell,
+                               const uint32_t* __restrict__ starts, const uint32_t* __restrict__ items,
+                               SurfelParams SP, const uint32_t* __restrict__ adj_start, uint32_t* out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nv) return;
+    const f3 p = ld3(vpos[k]);
+    const uint32_t cell = vcell[k];
+    const int32_t dx = SP.dim[0], dy = SP.dim[1], dz = SP.dim[2];
+    const int32_t cx = (int32_t)(cell % (uint32_t)dx), cy = (int32_t)((cell / (uint32_t)dx) % (uint32_t)dy),
+                  cz = (int32_t)(cell / ((uint32_t)dx * (uint32_t)dy));
+    uint32_t n = 0;
+    uint32_t* dst = FILL ? out + adj_start[k] : nullptr;
+    for (int32_t z = cz - 1 > 0 ? cz - 1 : 0; z <= (cz + 1 < dz - 1 ? cz + 1 : dz - 1); ++z)
+        for (int32_t y = cy - 1 > 0 ? cy - 1 : 0; y <= (cy + 1 < dy - 1 ? cy + 1 : dy - 1); ++y)
+            for (int32_t x = cx - 1 > 0 ? cx - 1 : 0; x <= (cx + 1 < dx - 1 ? cx + 1 : dx - 1); ++x) {
+                const uint32_t c = (uint32_t)(x + dx * (y + dy * z));
+                for (uint32_t q = starts[c], e = starts[c + 1]; q < e; ++q) {
+                    const uint32_t j = items[q];
+                    if (j >= k) break;  // items are in seed order within a cell
+                    const f3 dq = ld3(vpos[j]) - p;  // earlier surfel minus candidate
+                    if (!(dot3(dq, dq) < SP.d_min_sq)) continue;
+                    if (FILL) dst[n] = j;
+                    ++n;
+                }
+            }
+    if (!FILL) out[k] = n;
+}
+
+// One round over the survivors: an undecided seed is rejected when an
+// earlier neighbour is accepted, accepted when all are rejected, and waits
+// otherwise.
+__global__ void k_mis_round(uint32_t nv, const uint32_t* __restrict__ adj_start, const uint32_t* __restrict__ adj,
                             const uint8_t* __restrict__ st_in, uint8_t* st_out, uint32_t* open) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nv) return;
     uint8_t s = st_in[k];
     if (s == S_OPEN) {
-        const f3 p = ld3(vpos[k]);
-        const uint32_t cell = vcell[k];
-        const int32_t dx = SP.dim[0], dy = SP.dim[1], dz = SP.dim[2];
-        const int32_t cx = (int32_t)(cell % (uint32_t)dx), cy = (int32_t)((cell / (uint32_t)dx) % (uint32_t)dy),
-                      cz = (int32_t)(cell / ((uint32_t)dx * (uint32_t)dy));
         bool wait = false, drop = false;
-        for (int32_t z = cz - 1 > 0 ? cz - 1 : 0; z <= (cz + 1 < dz - 1 ? cz + 1 : dz - 1) && !drop; ++z)
-            for (int32_t y = cy - 1 > 0 ? cy - 1 : 0; y <= (cy + 1 < dy - 1 ? cy + 1 : dy - 1) && !drop; ++y)
-                for (int32_t x = cx - 1 > 0 ? cx - 1 : 0; x <= (cx + 1 < dx - 1 ? cx + 1 : dx - 1) && !drop; ++x) {
-                    const uint32_t c = (uint32_t)(x + dx * (y + dy * z));
-                    for (uint32_t q = starts[c], e = starts[c + 1]; q < e; ++q) {
-                        const uint32_t j = items[q];
-                        if (j >= k) break;  // items are in seed order within a cell
-                        const f3 dq = ld3(vpos[j]) - p;  // earlier surfel minus candidate
-                        if (!(dot3(dq, dq) < SP.d_min_sq)) continue;
-                        const uint8_t t = st_in[j];
-                        if (t == S_KEEP) {
-                            drop = true;
-                            break;
-                        }
-                        if (t == S_OPEN) wait = true;
-                    }
-                }
+        for (uint32_t q = adj_start[k], e = adj_start[k + 1]; q < e; ++q) {
+            const uint8_t t = st_in[adj[q]];
+            if (t == S_KEEP) {
+                drop = true;
+                break;
+            }
+            if (t == S_OPEN) wait = true;
+        }
         s = drop ? S_DROP : (wait ? S_OPEN : S_KEEP);
         if (s == S_OPEN) atomicAdd(open, 1u);
     }
@@ -140,7 +160,7 @@ int Engine::generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std
     const LightsView Lv = dev_lights();
     // 1. project every seed; survivors in seed order
     DevBuf d_pos, d_ok, d_at, d_vpos, d_vcell, d_counts, d_starts, d_keys2, d_items, d_items_in, d_st0, d_st1,
-        d_open1, d_flag, d_at2, d_op, d_on, d_oa, d_om;
+        d_open1, d_flag, d_at2, d_op, d_on, d_oa, d_om, d_adjn, d_adjs, d_adj;
     CK(d_pos.ensure(16ull * seeds));
     CK(d_ok.ensure(4ull * (seeds + 1)));
     CK(d_at.ensure(4ull * (seeds + 1)));
@@ -191,7 +211,35 @@ int Engine::generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std
     CK(d_tmp.ensure(tmp));
     CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_vcell.as<uint32_t>(), d_keys2.as<uint32_t>(),
                                           d_items_in.as<uint32_t>(), d_items.as<uint32_t>(), (int)nv, 0, kb, stream));
-    // 3. rounds until no survivor is undecided
+    // 3. the conflict graph (earlier neighbours of each survivor)
+    CK(d_adjn.ensure(4ull * (nv + 1)));
+    CK(d_adjs.ensure(4ull * (nv + 1)));
+    CK(hipMemsetAsync(d_adjn.as<uint32_t>() + nv, 0, 4, stream));
+    launch("surfel_graph", [&] {
+        k_earlier_nbrs<false><<<blocks_for(nv, 256), 256, 0, stream>>>(
+            nv, d_vpos.as<float4>(), d_vcell.as<uint32_t>(), d_starts.as<uint32_t>(), d_items.as<uint32_t>(), SP,
+            nullptr, d_adjn.as<uint32_t>());
+    });
+    tmp = 0;
+    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_adjn.as<uint32_t>(), d_adjs.as<uint32_t>(), (int)(nv + 1),
+                                        stream));
+    CK(d_tmp.ensure(tmp));
+    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_adjn.as<uint32_t>(), d_adjs.as<uint32_t>(), (int)(nv + 1),
+                                        stream));
+    uint32_t edges = 0;
+    CK(hipMemcpyAsync(&edges, d_adjs.as<uint32_t>() + nv, 4, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    if (edges >= (1u << 31)) {
+        err = "surfel conflict graph exceeds 2^31 edges";
+        return -6;
+    }
+    CK(d_adj.ensure(4ull * edges + 4));
+    launch("surfel_graph", [&] {
+        k_earlier_nbrs<true><<<blocks_for(nv, 256), 256, 0, stream>>>(
+            nv, d_vpos.as<float4>(), d_vcell.as<uint32_t>(), d_starts.as<uint32_t>(), d_items.as<uint32_t>(), SP,
+            d_adjs.as<uint32_t>(), d_adj.as<uint32_t>());
+    });
+    // 4. rounds until no survivor is undecided
     CK(d_st0.ensure(nv));
     CK(d_st1.ensure(nv));
     CK(d_open1.ensure(4ull * 64));
@@ -204,9 +252,8 @@ int Engine::generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std
         CK(hipMemsetAsync(d_open1.p, 0, 4ull * 8, stream));
         launch("surfel_rounds", [&] {
             for (int r = 0; r < 8; ++r) {
-                k_mis_round<<<blocks_for(nv, 256), 256, 0, stream>>>(
-                    nv, d_vpos.as<float4>(), d_vcell.as<uint32_t>(), d_starts.as<uint32_t>(), d_items.as<uint32_t>(),
-                    SP, st_in, st_out, d_open1.as<uint32_t>() + r);
+                k_mis_round<<<blocks_for(nv, 256), 256, 0, stream>>>(nv, d_adjs.as<uint32_t>(), d_adj.as<uint32_t>(),
+                                                                     st_in, st_out, d_open1.as<uint32_t>() + r);
                 std::swap(st_in, st_out);
             }
         });
@@ -220,7 +267,7 @@ int Engine::generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std
             return -4;
         }
     }
-    // 4. accepted in seed order, cut at the target
+    // 5. accepted in seed order, cut at the target
     CK(d_flag.ensure(4ull * (nv + 1)));
     CK(d_at2.ensure(4ull * (nv + 1)));
     CK(hipMemsetAsync(d_flag.as<uint32_t>() + nv, 0, 4, stream));
