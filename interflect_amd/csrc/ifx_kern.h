@@ -209,14 +209,19 @@ __device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, 
         const f3 e = max3(max3(ld3(nodes[2 * n]) - p, p - ld3(nodes[2 * n + 1])), spl(0.0f));
         return dot3(e, e);
     };
+    // the squared opening bound, refreshed when a leaf has been evaluated
+    auto bound2 = [&]() {
+        const float b = blend ? (fmn(hard, smin_poly(b1, b2, K)) + 1.25f * K) + 2.0f * delta
+                              : fmx(hard, 0.0f) + 2.0f * delta;
+        return b > 0.0f ? b * b : -1.0f;
+    };
+    float open2 = bound2();
     int sp = 0;
     stk.put(sp++, 0u, box2(0u));
     while (sp > 0) {
         float dn2;
         const uint32_t ni = __builtin_amdgcn_readfirstlane(stk.get(--sp, dn2));
-        const float bound = blend ? (fmn(hard, smin_poly(b1, b2, K)) + 1.25f * K) + 2.0f * delta
-                                  : fmx(hard, 0.0f) + 2.0f * delta;
-        if (__ballot(need && bound > 0.0f && dn2 < bound * bound) == 0ull) continue;
+        if (__ballot(need && dn2 < open2) == 0ull) continue;
         const uint32_t w0 = fbits(nodes[2 * ni].w), cnt = fbits(nodes[2 * ni + 1].w);
         if (cnt > 0) {
             for (uint32_t k = w0; k < w0 + cnt; ++k) {
@@ -237,6 +242,7 @@ __device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, 
                     h2 = fmn(h2, d);
                 }
             }
+            open2 = bound2();
         } else {
             const uint32_t l = ni + 1, r = w0;
             const float dl = box2(l), dr = box2(r);
