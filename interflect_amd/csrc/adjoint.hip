@@ -125,9 +125,9 @@ __device__ __forceinline__ f3 gather_scatter(const GridView& g, const float4* __
 // Adjoint of render_frame (src/main.rs:247-361) in beauty mode for diffuse
 // materials (metallic 0, roughness >= 0.45: no mirror branch, f0 constant).
 // 16x16 pixel blocks, one 8x8 tile per wavefront, as the forward shade.
-template <bool LDS, bool WAVE>
+template <bool LDS>
 __global__ __launch_bounds__(kBlock) void k_shade_bwd(SceneView Sg, LightsView Lg, LdsPlan P, ShadeArgs A, BwdArgs B) {
-    IFX_SETUP_W(LDS, true, false, WAVE);
+    IFX_SETUP(LDS, true, false);
     const uint32_t wv = threadIdx.x >> 6, l = threadIdx.x & 63u;
     const uint32_t x = blockIdx.x * 16u + (wv & 1u) * 8u + (l & 7u);
     const uint32_t y = blockIdx.y * 16u + (wv >> 1) * 8u + (l >> 3);
@@ -687,7 +687,7 @@ int Engine::backward(const float* dL_dfilm, float* d_albedo, float* d_emit, Back
         const SceneView S = dev_view();
         const Lds ld = plan_lds(true);
         dim3 grid((beauty_w_ + 15) / 16, (beauty_h_ + 15) / 16);
-        LAUNCH_LW(k_shade_bwd, ld.staged, S.n_obj >= kWaveShadeMinObjs, grid, ld.bytes, S, Lv, ld.plan, A, B);
+        LAUNCH_L(k_shade_bwd, ld.staged, grid, ld.bytes, S, Lv, ld.plan, A, B);
         if (N)
             k_fix_to_float<<<blocks_for(N, 256), 256, 0, stream>>>(N, B.uacc, B.umax, npx, kd_max,
                                                                    d_ubar.as<float4>());

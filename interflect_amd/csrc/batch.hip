@@ -45,8 +45,7 @@ __device__ __forceinline__ uint32_t slot_of(const uint32_t* __restrict__ blk, ui
 
 // Prologue of a block-partitioned tracing kernel: find the block's scene,
 // stage its records (and lights) in LDS when they fit, place the stack.
-#define IFX_BSETUP(LIGHTS) IFX_BSETUP_S(LIGHTS, LdsStack)
-#define IFX_BSETUP_S(LIGHTS, STACK)                                               \
+#define IFX_BSETUP(LIGHTS)                                                        \
     extern __shared__ float4 ifx_lds[];                                          \
     const uint32_t sid = slot_of(blk, ns, blockIdx.x);                           \
     const BSlot& T = slots[sid];                                                 \
@@ -55,7 +54,7 @@ __device__ __forceinline__ uint32_t slot_of(const uint32_t* __restrict__ blk, ui
     LightsView L = T.L;                                                          \
     const LdsPlan P = (LIGHTS) ? T.Pl : T.Pn;                                    \
     if ((LIGHTS) ? T.staged_l : T.staged_n) stage_in<true, LIGHTS>(ifx_lds, S, L); \
-    STACK stk;                                                                   \
+    LdsStack stk;                                                                \
     place_stack(stk, ifx_lds, P.scene_f4 + P.light_f4, P.stack_depth);           \
     (void)L;                                                                     \
     (void)bid
@@ -212,9 +211,7 @@ __global__ __launch_bounds__(kBlock) void kb_closure(const BSlot* __restrict__ s
                 live = true;
             }
             if (__ballot(live) == 0) break;
-            uint32_t mat = 0;
-        bool have_mat = false;
-        const int st = trace_step(S, m, live, mat, have_mat, stk);
+            const int st = miss_step(S, m, live, stk);
             if (st) {
                 if (st == 2) atomicAdd(&open[row], 1u);
                 acc += m.steps;
@@ -308,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void kb_direct_terms(const BSlot* __restric
 // Beauty pixels (k_shade): 16x16 blocks of the scene's film, 8x8 per wavefront.
 __global__ __launch_bounds__(kBlock) void kb_shade(const BSlot* __restrict__ slots, const uint32_t* __restrict__ blk,
                                                    uint32_t ns, float4* film, unsigned long long* evals) {
-    IFX_BSETUP_S(true, LaneStack);  // camera rays of mostly small scenes: per-lane traversal
+    IFX_BSETUP(true);
     const ShadeArgs& A = T.A;
     const uint32_t gx = (A.W + 15u) / 16u;
     const uint32_t bx = bid % gx, by = bid / gx;

@@ -294,9 +294,7 @@ __global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg,
             live = true;
         }
         if (__ballot(live) == 0) break;
-        uint32_t mat = 0;
-        bool have_mat = false;
-        const int st = trace_step(S, m, live, mat, have_mat, stk);
+        const int st = miss_step(S, m, live, stk);
         if (st) {
             if (st == 2) atomicAdd(&open[row], 1u);
             acc += m.steps;
@@ -1364,8 +1362,9 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
     dim3 grid((W + 15) / 16, (H + 15) / 16);
     const Lds ld = plan_lds(true);
     launch("shade", [&] {
-        LAUNCH_CLW(k_shade, count_ops_, ld.staged, S.n_obj >= kWaveShadeMinObjs, grid, ld.bytes, S, Lv, ld.plan, A,
-                   d_film.as<float4>(), ctr, count_ops_ ? ctr + 4 : nullptr);
+        const bool wave = S.n_obj > kWaveScanObjs || npx < kWaveShadeMaxPixels;
+        LAUNCH_CLW(k_shade, count_ops_, ld.staged, wave, grid, ld.bytes, S, Lv, ld.plan, A, d_film.as<float4>(), ctr,
+                   count_ops_ ? ctr + 4 : nullptr);
     });
     const bool tonemapped = mode_tonemapped(mode);
     if (rgb8) {
@@ -1433,8 +1432,9 @@ int Engine::shade_views(const std::vector<HCamera>& cams, uint32_t W, uint32_t H
     dim3 grid((W + 15) / 16, (H + 15) / 16, nv);
     const Lds ld = plan_lds(true);
     launch("shade_views", [&] {
-        LAUNCH_CLW(k_shade_views, count_ops_, ld.staged, S.n_obj >= kWaveShadeMinObjs, grid, ld.bytes, S, Lv,
-                   ld.plan, A, d_cams.as<CamV>(), d_film.as<float4>(), ctr, count_ops_ ? ctr + 4 : nullptr);
+        const bool wave = S.n_obj > kWaveScanObjs || npx < kWaveShadeMaxPixels;
+        LAUNCH_CLW(k_shade_views, count_ops_, ld.staged, wave, grid, ld.bytes, S, Lv, ld.plan, A, d_cams.as<CamV>(),
+                   d_film.as<float4>(), ctr, count_ops_ ? ctr + 4 : nullptr);
     });
     if (rgb8) {
         launch("encode", [&] {

@@ -104,7 +104,7 @@ struct StackOf<true> {
 
 // Stacks of the camera-ray shading kernels: LdsStack (wavefront-shared
 // queries) or LaneStack, an LdsStack whose march loops keep the per-lane
-// traversal.
+// traversal (kShared false).
 struct LaneStack : LdsStack {
     static constexpr bool kShared = false;
 };
@@ -121,6 +121,14 @@ template <bool WAVE>
 struct ShadeStackOf<true, WAVE> {
     using T = CountStack;
 };
+
+// Camera-ray shading of a scene small enough for the uniform scan keeps the
+// per-lane traversal from this many pixels per launch up: with the GPU full,
+// the scan's extra objects per lane cost more than its coherence saves
+// (measured on MI355X, 7-object scene: one 640x480 view 10 % faster shared,
+// twelve views in one launch 13 % slower).
+constexpr uint64_t kWaveShadeMaxPixels = 1ull << 21;
+
 __device__ __forceinline__ void stk_init(LdsStack&) {}
 __device__ __forceinline__ void stk_init(CountStack& s) { s.ops = 0ull; }
 __device__ __forceinline__ void stk_flush(LdsStack&, unsigned long long*) {}
@@ -168,10 +176,14 @@ __device__ __forceinline__ void place_stack(Stk& stk, float4* lds, uint32_t f4, 
 //    distance <= result, or below every other object's value and the bound
 //    of the unevaluated ones); a lane whose check fails takes the per-lane
 //    traversal, so every lane returns dist() bit for bit.
+//  Scenes of at most kWaveScanObjs objects skip the traversal: every lane
+//  evaluates every object (the same checks then see all of them).
 //  Records are read from their global copies (S.g_*) through
 //  constant-address-space pointers: with the node and object indices uniform,
 //  those reads are scalar loads.  The per-lane fallback reads S as given
 //  (LDS when the scene is staged).
+constexpr uint32_t kWaveScanObjs = 16;
+
 typedef float __attribute__((ext_vector_type(4))) NativeF4;
 struct ConstF4 {
     const __attribute__((address_space(4))) NativeF4* q;
@@ -185,95 +197,115 @@ __device__ __forceinline__ ConstF4 as_const(const float4* p) {
     return ConstF4{(const __attribute__((address_space(4))) NativeF4*)p};
 }
 
-template <class Stk>
-__device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, Stk& stk) {
-    if (!S.use_bvh || S.n_nodes == 0) return need ? dist(S, p, stk) : 0.0f;
+//  MAT: also the material dist_mat reports (the first object reaching the
+//  minimum in the per-lane order); a lane whose minimum is tied across leaves
+//  (where the orders differ) takes the per-lane traversal.
+template <bool MAT, class Stk>
+__device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool need, Stk& stk) {
+    if (!S.use_bvh || S.n_nodes == 0) return need ? dist_mat(S, p, stk) : DistHit{0.0f, 0u};
     const ConstF4 nodes = as_const(S.g_nodes), objs = as_const(S.g_obj), subs = as_const(S.g_subs);
     const ConstF4 planes = as_const(S.g_planes);
-    if (__ballot(need) == 0ull) return 0.0f;
+    if (__ballot(need) == 0ull) return DistHit{0.0f, 0u};
     constexpr uint32_t kNone = 0xffffffffu;
     const bool blend = S.blend_k > 0.0f;
     const float K = S.blend_k;
     const float delta = 1e-4f * (1.0f + maxe(abs3(p)));
     float hard = kFltMax, b1 = kFltMax, b2 = kFltMax, h2 = kFltMax;
-    uint32_t a = kNone, la = kNone, s1 = kNone, s2 = kNone;
+    uint32_t a = kNone, la = kNone, s1 = kNone, s2 = kNone, hmat = 0u, b1mat = 0u;
     for (uint32_t i = 0; i < S.n_planes; ++i) {
         uint32_t mat;
         bool bl;
         float d = obj_dist_at(planes + 4 * i, subs, p, mat, bl, stk);
-        if (d < hard) hard = d;
+        if (d < hard) hard = d, hmat = mat;
     }
-    // The stack holds squared box distances: the opening test only compares
-    // them with the bound, and the margins above absorb the rounding.
-    auto box2 = [&](uint32_t n) {
-        const f3 e = max3(max3(ld3(nodes[2 * n]) - p, p - ld3(nodes[2 * n + 1])), spl(0.0f));
-        return dot3(e, e);
-    };
-    // the squared opening bound, refreshed when a leaf has been evaluated
-    auto bound2 = [&]() {
-        const float b = blend ? (fmn(hard, smin_poly(b1, b2, K)) + 1.25f * K) + 2.0f * delta
-                              : fmx(hard, 0.0f) + 2.0f * delta;
-        return b > 0.0f ? b * b : -1.0f;
-    };
-    float open2 = bound2();
-    int sp = 0;
-    stk.put(sp++, 0u, box2(0u));
-    while (sp > 0) {
-        float dn2;
-        const uint32_t ni = __builtin_amdgcn_readfirstlane(stk.get(--sp, dn2));
-        if (__ballot(need && dn2 < open2) == 0ull) continue;
-        const uint32_t w0 = fbits(nodes[2 * ni].w), cnt = fbits(nodes[2 * ni + 1].w);
-        if (cnt > 0) {
-            for (uint32_t k = w0; k < w0 + cnt; ++k) {
-                uint32_t mat;
-                bool bl;
-                const float d = obj_dist_at(objs + 4 * k, subs, p, mat, bl, stk);
-                if (blend && bl) {
-                    if (d < b1) {
-                        b2 = b1, s2 = s1;
-                        b1 = d, s1 = k;
-                    } else if (d < b2) {
-                        b2 = d, s2 = k;
-                    }
-                } else if (d < hard) {
-                    if (ni != la) h2 = hard;  // everything evaluated so far lies outside this leaf
-                    hard = d, a = k, la = ni;
-                } else if (ni != la) {
-                    h2 = fmn(h2, d);
-                }
+    // object k of leaf ni at p: the lane's running minima
+    auto take = [&](uint32_t k, uint32_t ni) {
+        uint32_t mat;
+        bool bl;
+        const float d = obj_dist_at(objs + 4 * k, subs, p, mat, bl, stk);
+        if (blend && bl) {
+            if (d < b1) {
+                b2 = b1, s2 = s1;
+                b1 = d, s1 = k, b1mat = mat;
+            } else if (d < b2) {
+                b2 = d, s2 = k;
             }
-            open2 = bound2();
-        } else {
-            const uint32_t l = ni + 1, r = w0;
-            const float dl = box2(l), dr = box2(r);
-            // open first the child nearer for most lanes
-            const uint32_t nl = (uint32_t)__popcll(__ballot(need && dl < dr));
-            const uint32_t nn = (uint32_t)__popcll(__ballot(need));
-            if (2u * nl >= nn) {
-                stk.put(sp++, r, dr);
-                stk.put(sp++, l, dl);
+        } else if (d < hard) {
+            if (ni != la) h2 = hard;  // everything evaluated so far lies outside this leaf
+            hard = d, a = k, la = ni, hmat = mat;
+        } else if (ni != la) {
+            h2 = fmn(h2, d);
+        }
+    };
+    if (S.n_obj <= kWaveScanObjs) {
+        // few objects: evaluate all of them, in slot (leaf) order, no traversal
+        for (uint32_t k = 0; k < S.n_obj; ++k) take(k, S.lin[S.n_obj + k]);
+    } else {
+        // The stack holds squared box distances: the opening test only compares
+        // them with the bound, and the margins above absorb the rounding.
+        auto box2 = [&](uint32_t n) {
+            const f3 e = max3(max3(ld3(nodes[2 * n]) - p, p - ld3(nodes[2 * n + 1])), spl(0.0f));
+            return dot3(e, e);
+        };
+        // the squared opening bound, refreshed when a leaf has been evaluated
+        auto bound2 = [&]() {
+            const float b = blend ? (fmn(hard, smin_poly(b1, b2, K)) + 1.25f * K) + 2.0f * delta
+                                  : fmx(hard, 0.0f) + 2.0f * delta;
+            return b > 0.0f ? b * b : -1.0f;
+        };
+        float open2 = bound2();
+        int sp = 0;
+        stk.put(sp++, 0u, box2(0u));
+        while (sp > 0) {
+            float dn2;
+            const uint32_t ni = __builtin_amdgcn_readfirstlane(stk.get(--sp, dn2));
+            if (__ballot(need && dn2 < open2) == 0ull) continue;
+            const uint32_t w0 = fbits(nodes[2 * ni].w), cnt = fbits(nodes[2 * ni + 1].w);
+            if (cnt > 0) {
+                for (uint32_t k = w0; k < w0 + cnt; ++k) take(k, ni);
+                open2 = bound2();
             } else {
-                stk.put(sp++, l, dl);
-                stk.put(sp++, r, dr);
+                const uint32_t l = ni + 1, r = w0;
+                const float dl = box2(l), dr = box2(r);
+                // open first the child nearer for most lanes
+                const uint32_t nl = (uint32_t)__popcll(__ballot(need && dl < dr));
+                const uint32_t nn = (uint32_t)__popcll(__ballot(need));
+                if (2u * nl >= nn) {
+                    stk.put(sp++, r, dr);
+                    stk.put(sp++, l, dl);
+                } else {
+                    stk.put(sp++, l, dl);
+                    stk.put(sp++, r, dr);
+                }
             }
         }
     }
-    if (!need) return 0.0f;
+    if (!need) return DistHit{0.0f, 0u};
     auto leaf_box = [&](uint32_t slot) {
         const uint32_t n = S.lin[S.n_obj + slot];
         return aabb_dist(p, S.nodes[2 * n], S.nodes[2 * n + 1]);
     };
     bool ok = true;
     float res;
+    uint32_t rmat = hmat;
     if (!blend) {
         res = hard;
         if (a != kNone) {
             const float lb = leaf_box(a);
             ok = lb <= hard || (lb < h2 && lb < fmx(hard, 0.0f) + delta);
+            if (MAT && !(h2 > hard)) ok = false;
         }
     } else {
         const float sm = smin_poly(b1, b2, K);
         res = sm < hard ? sm : hard;
+        if (MAT) {
+            if (sm < hard) {
+                rmat = b1mat;
+                if (!(b2 > b1)) ok = false;
+            } else if (a != kNone && !(h2 > hard)) {
+                ok = false;
+            }
+        }
         // the per-lane traversal's final prune threshold, lowered by a few ulps of its terms
         float thr = fmn(hard, sm) + K;
         thr = thr - 1e-6f * (fabsf(thr) + K + fabsf(res));
@@ -283,7 +315,12 @@ __device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, 
             if (b2 - b1 < K && s2 != kNone && !(leaf_box(s2) < thr)) ok = false;
         }
     }
-    return ok ? res : dist(S, p, stk);
+    return ok ? DistHit{res, rmat} : dist_mat(S, p, stk);
+}
+
+template <class Stk>
+__device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, Stk& stk) {
+    return wave_dist_mat<false>(S, p, need, stk).d;
 }
 
 // Wavefront-shared overloads of the march hooks (the per-lane versions and
@@ -303,9 +340,9 @@ __device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, 
     if (need) ended = m.post(h, vis);
     return live && ended;
 }
-// The hit's material is left to the caller (have_mat false).
-__device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, bool live, uint32_t& mat,
-                                          bool& have_mat, LdsStack& stk) {
+// A sphere-trace step when only hit / miss matters (no material): 0
+// marching, 1 hit, 2 miss.
+__device__ __forceinline__ int miss_step(const SceneView& S, TraceMarch& m, bool live, LdsStack& stk) {
     f3 p = spl(0.0f);
     int st = 0;
     bool need = false;
@@ -315,17 +352,28 @@ __device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, boo
     }
     const float h = wave_dist(S, p, need, stk);
     if (need) st = m.post(h);
-    mat = 0u;
-    have_mat = false;
     return st;
 }
+__device__ __forceinline__ int miss_step(const SceneView& S, TraceMarch& m, bool live, CountStack& stk) {
+    DistHit h;
+    return live ? m.step(S, h, stk) : 0;
+}
 
-// Camera-ray shading (k_shade, k_shade_views, k_shade_bwd) takes the shared
-// traversal from this many objects up, the per-lane one below:
-// camera rays are coherent already, and in small scenes the shared traversal
-// opens nearly every leaf for every lane (measured on MI355X: an 8-object
-// room shaded 20% slower shared, the 262-object stress frame 12% faster).
-constexpr uint32_t kWaveShadeMinObjs = 16;
+__device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, bool live, uint32_t& mat,
+                                          bool& have_mat, LdsStack& stk) {
+    f3 p = spl(0.0f);
+    int st = 0;
+    bool need = false;
+    if (live) {
+        if (m.pre(p)) st = 2;
+        else need = true;
+    }
+    const DistHit h = wave_dist_mat<true>(S, p, need, stk);
+    if (need) st = m.post(h.d);
+    mat = h.mat;
+    have_mat = true;
+    return st;
+}
 
 // Kernel prologue: stage, then place this lane's stack column after the staged records.
 #define IFX_SETUP(LDS, LIGHTS, COUNT) IFX_SETUP_T(LDS, LIGHTS, typename StackOf<COUNT>::T)
@@ -597,21 +645,6 @@ void Engine::launch(const char* name, F&& f) {
                 K<false, true, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);       \
             else                                                                         \
                 K<false, false, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);      \
-        }                                                                                \
-    } while (0)
-
-#define LAUNCH_LW(K, LDS, WAVE, GRID, DYN, ...)                                          \
-    do {                                                                                 \
-        if (WAVE) {                                                                      \
-            if (LDS)                                                                     \
-                K<true, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);               \
-            else                                                                         \
-                K<false, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);              \
-        } else {                                                                         \
-            if (LDS)                                                                     \
-                K<true, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);              \
-            else                                                                         \
-                K<false, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);             \
         }                                                                                \
     } while (0)
 
