@@ -306,6 +306,67 @@ def test_blend_scene_dist_and_trace_bitexact():
     assert np.array_equal(g[0], o[0]) and np.array_equal(g[4], o[4])
 
 
+def tie_scene(copies, blend):
+    """Coincident duplicates and mirror pairs with different materials
+    (minima tied, within and across BVH leaves), objects inside others, and
+    optionally a blend group: the cases where the wavefront-shared query must
+    hand a lane to the per-lane traversal.  4 copies (16 objects) take the
+    shared query's uniform scan, 12 copies its traversal."""
+    L = ['render { width: 64, height: 48, surfels: 1000, bounces: 4, exposure: 1.0 }',
+         'camera { pos: [0, 2, 9], look: [0, 1, 0], up: [0, 1, 0], fov: 50 }']
+    for j in range(8):
+        L.append('material "m%d" { albedo: [%.2f, %.2f, %.2f], roughness: %s }'
+                 % (j, 0.2 + 0.1 * j, 0.9 - 0.1 * j, 0.5, "0.2, metallic: 1.0" if j == 7 else "0.8"))
+    L.append('plane { normal: [0, 1, 0], height: 0, mat: "m0" }')
+    L.append('light { verts: [[-1, 4, -1], [1, 4, -1], [1, 4, 1], [-1, 4, 1]], emit: [8, 8, 8] }')
+    ext = ", blend: 0.3" if blend else ""
+    for c in range(copies):
+        x = -3.0 + 6.0 * c / max(1, copies - 1)
+        z = -1.0 + (c % 3)
+        # the same sphere twice, two materials
+        L.append('sphere { center: [%.3f, 1, %.3f], radius: 0.6, mat: "m%d"%s }' % (x, z, 1 + c % 6, ext))
+        L.append('sphere { center: [%.3f, 1, %.3f], radius: 0.6, mat: "m%d"%s }' % (x, z, 2 + c % 5, ext))
+        # a box around the sphere's lower half (points inside both)
+        L.append('box { center: [%.3f, 0.6, %.3f], size: [1.0, 0.8, 1.0], mat: "m7" }' % (x, z))
+    # mirror pairs about x = 0 (far apart in the BVH): equal distances on x = 0
+    for y in (2.4, 3.3):
+        L.append('sphere { center: [-0.4, %.1f, 0], radius: 0.6, mat: "m3"%s }' % (y, ext))
+        L.append('sphere { center: [0.4, %.1f, 0], radius: 0.6, mat: "m5"%s }' % (y, ext))
+    return "\n".join(L) + "\n"
+
+
+@pytest.mark.parametrize("copies,blend", [(4, False), (4, True), (12, False), (12, True)])
+def test_shared_query_ties_bitexact(copies, blend):
+    say("ties", copies, blend)
+    ctx, sc = pair_text(tie_scene(copies, blend))
+    lo, hi = np.float32(sc.lo), np.float32(sc.hi)
+    # soft shadows from random points (many inside objects) to random targets
+    n = 40000
+    ro = random_points(lo, hi, n, 21)
+    tgt = random_points(lo, hi, n, 22)
+    d = tgt - ro
+    dist = np.linalg.norm(d, axis=1).astype(np.float32)
+    rd = (d / dist[:, None]).astype(np.float32)
+    k = np.random.default_rng(23).uniform(1.5, 96, n).astype(np.float32)
+    v, st = ctx.soft_shadows(ro, rd, 0.015, dist, k)
+    v0, st0 = sc.soft_shadow(ro, rd, 0.015, dist, k)
+    assert np.array_equal(bits(v), bits(v0)) and np.array_equal(st, st0)
+    # traces (hit material decided among tied duplicates), half of them in
+    # the mirror plane x = 0
+    ro, rd = camera_rays(sc, 30000, 24)
+    ro[::2, 0] = 0.0
+    rd[::2, 0] = 0.0
+    rd[::2] /= np.linalg.norm(rd[::2], axis=1, keepdims=True).astype(np.float32)
+    tmax = np.float32(sc.diag * 4)
+    g = ctx.trace_rays(ro, rd, 1e-3, tmax)
+    o = sc.trace(ro, rd, 1e-3, tmax)
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[4], o[4])
+    h = o[0]
+    assert np.array_equal(bits(g[1][h]), bits(o[1][h]))
+    assert np.array_equal(bits(g[2][h]), bits(o[2][h]))
+    assert np.array_equal(g[3][h], o[3][h])
+
+
 def _ext_pipeline(text, target, rows, w, h):
     sc = po.Scene(text)
     sc.generate(target)
