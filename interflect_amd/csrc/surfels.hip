@@ -60,13 +60,23 @@ __global__ void k_iota(uint32_t* v, uint32_t n) {
 // index, closer than d_min) from its 27-cell neighbourhood.  `items` lists
 // the survivors by cell (seed order within a cell); `starts` indexes it by
 // cell.  FILL=false counts them, FILL=true writes them at adj_start[k].
+// Threads take the survivors in cell order (t -> items[t]), so a wavefront's
+// lanes scan the same neighbour cells; `cpos` holds the positions in that
+// order.
+__global__ void k_cell_pos(uint32_t nv, const uint32_t* __restrict__ items, const float4* __restrict__ vpos,
+                           float4* cpos) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nv) cpos[q] = vpos[items[q]];
+}
+
 template <bool FILL>
-__global__ void k_earlier_nbrs(uint32_t nv, const float4* __restrict__ vpos, const uint32_t* __restrict__ vcell,
+__global__ void k_earlier_nbrs(uint32_t nv, const float4* __restrict__ cpos, const uint32_t* __restrict__ vcell,
                                const uint32_t* __restrict__ starts, const uint32_t* __restrict__ items,
                                SurfelParams SP, const uint32_t* __restrict__ adj_start, uint32_t* out) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nv) return;
-    const f3 p = ld3(vpos[k]);
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv) return;
+    const uint32_t k = items[t];
+    const f3 p = ld3(cpos[t]);
     const uint32_t cell = vcell[k];
     const int32_t dx = SP.dim[0], dy = SP.dim[1], dz = SP.dim[2];
     const int32_t cx = (int32_t)(cell % (uint32_t)dx), cy = (int32_t)((cell / (uint32_t)dx) % (uint32_t)dy),
@@ -80,7 +90,7 @@ __global__ void k_earlier_nbrs(uint32_t nv, const float4* __restrict__ vpos, con
                 for (uint32_t q = starts[c], e = starts[c + 1]; q < e; ++q) {
                     const uint32_t j = items[q];
                     if (j >= k) break;  // items are in seed order within a cell
-                    const f3 dq = ld3(vpos[j]) - p;  // earlier surfel minus candidate
+                    const f3 dq = ld3(cpos[q]) - p;  // earlier surfel minus candidate
                     if (!(dot3(dq, dq) < SP.d_min_sq)) continue;
                     if (FILL) dst[n] = j;
                     ++n;
@@ -160,7 +170,7 @@ int Engine::generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std
     const LightsView Lv = dev_lights();
     // 1. project every seed; survivors in seed order
     DevBuf d_pos, d_ok, d_at, d_vpos, d_vcell, d_counts, d_starts, d_keys2, d_items, d_items_in, d_st0, d_st1,
-        d_open1, d_flag, d_at2, d_op, d_on, d_oa, d_om, d_adjn, d_adjs, d_adj;
+        d_open1, d_flag, d_at2, d_op, d_on, d_oa, d_om, d_adjn, d_adjs, d_adj, d_cpos;
     CK(d_pos.ensure(16ull * seeds));
     CK(d_ok.ensure(4ull * (seeds + 1)));
     CK(d_at.ensure(4ull * (seeds + 1)));
@@ -214,10 +224,13 @@ int Engine::generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std
     // 3. the conflict graph (earlier neighbours of each survivor)
     CK(d_adjn.ensure(4ull * (nv + 1)));
     CK(d_adjs.ensure(4ull * (nv + 1)));
+    CK(d_cpos.ensure(16ull * nv));
     CK(hipMemsetAsync(d_adjn.as<uint32_t>() + nv, 0, 4, stream));
     launch("surfel_graph", [&] {
+        k_cell_pos<<<blocks_for(nv, 256), 256, 0, stream>>>(nv, d_items.as<uint32_t>(), d_vpos.as<float4>(),
+                                                             d_cpos.as<float4>());
         k_earlier_nbrs<false><<<blocks_for(nv, 256), 256, 0, stream>>>(
-            nv, d_vpos.as<float4>(), d_vcell.as<uint32_t>(), d_starts.as<uint32_t>(), d_items.as<uint32_t>(), SP,
+            nv, d_cpos.as<float4>(), d_vcell.as<uint32_t>(), d_starts.as<uint32_t>(), d_items.as<uint32_t>(), SP,
             nullptr, d_adjn.as<uint32_t>());
     });
     tmp = 0;
@@ -236,7 +249,7 @@ int Engine::generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std
     CK(d_adj.ensure(4ull * edges + 4));
     launch("surfel_graph", [&] {
         k_earlier_nbrs<true><<<blocks_for(nv, 256), 256, 0, stream>>>(
-            nv, d_vpos.as<float4>(), d_vcell.as<uint32_t>(), d_starts.as<uint32_t>(), d_items.as<uint32_t>(), SP,
+            nv, d_cpos.as<float4>(), d_vcell.as<uint32_t>(), d_starts.as<uint32_t>(), d_items.as<uint32_t>(), SP,
             d_adjs.as<uint32_t>(), d_adj.as<uint32_t>());
     });
     // 4. rounds until no survivor is undecided
