@@ -33,6 +33,11 @@ def say(*a):
 def diff_ctx(seed):
     ctx = ia.Context(0)
     ctx.load_scene(scenes.diff_cornell(seed, surfels=SURFELS, width=W, height=H))
+    # Host placement (bit-identical to the device placement, which
+    # test_gpu_surfels checks): the device placement of this 4000-surfel room
+    # once ran past the test's time limit, with no cause found, and stays off
+    # the GPU until one is.
+    ctx.set_tuning("surfels_gpu", 0)
     ctx.generate_surfels()
     ctx.transfer(8)
     ctx.set_solve_tolerance(0.0)
