@@ -211,9 +211,11 @@ __global__ void k_morton(const float4* __restrict__ spos, uint32_t N, f3 lo, f3 
 // and takes the next candidate from the queue as soon as its ray ends, so
 // short and long marches no longer idle each other.  Candidates are taken
 // in `order` (cluster-major, surfels in Morton order: near-parallel rays
-// from nearby origins share a wavefront).  cand_v[slot] = ff * vis for a
-// kept link, 0 for a culled one; slots are fixed, so the work order never
-// changes a result.
+// from nearby origins share a wavefront).  Each sample's distances are
+// answered for the whole wavefront by one shared BVH traversal (wave_dist;
+// the scene stays in global memory and is read through the scalar cache).
+// cand_v[slot] = ff * vis for a kept link, 0 for a culled one; slots are
+// fixed, so the work order never changes a result.
 template <bool COUNT, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView Lg, LdsPlan P,
                                                        const float4* __restrict__ spos,
