@@ -176,8 +176,8 @@ __device__ __forceinline__ void place_stack(Stk& stk, float4* lds, uint32_t f4, 
 //    distance <= result, or below every other object's value and the bound
 //    of the unevaluated ones); a lane whose check fails takes the per-lane
 //    traversal, so every lane returns dist() bit for bit.
-//  Scenes of at most kWaveScanObjs objects skip the traversal: every lane
-//  evaluates every object (the same checks then see all of them).
+//  Scenes of at most kWaveScanObjs objects skip the interior nodes: their
+//  leaves are tested in slot order, each under the same opening test.
 //  Records are read from their global copies (S.g_*) through
 //  constant-address-space pointers: with the node and object indices uniform,
 //  those reads are scalar loads.  The per-lane fallback reads S as given
@@ -200,8 +200,14 @@ __device__ __forceinline__ ConstF4 as_const(const float4* p) {
 //  MAT: also the material dist_mat reports (the first object reaching the
 //  minimum in the per-lane order); a lane whose minimum is tied across leaves
 //  (where the orders differ) takes the per-lane traversal.
+//  ub: an upper bound on the lane's scene distance at p, when the caller has
+//  one (a march's previous sample h at step s away: h + s, SDFs being
+//  1-Lipschitz), else kFltMax.  The opening bound starts from it, so nodes
+//  farther than ub (plus the same margins) are never opened: every object
+//  under them is then at least ub + delta >= the result + delta away.
 template <bool MAT, class Stk>
-__device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool need, Stk& stk) {
+__device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool need, Stk& stk,
+                                                 float ub = kFltMax) {
     if (!S.use_bvh || S.n_nodes == 0) return need ? dist_mat(S, p, stk) : DistHit{0.0f, 0u};
     const ConstF4 nodes = as_const(S.g_nodes), objs = as_const(S.g_obj), subs = as_const(S.g_subs);
     const ConstF4 planes = as_const(S.g_planes);
@@ -237,23 +243,34 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
             h2 = fmn(h2, d);
         }
     };
+    // Squared box distances: the opening test only compares them with the
+    // bound, and the margins above absorb the rounding.
+    auto box2 = [&](uint32_t n) {
+        const f3 e = max3(max3(ld3(nodes[2 * n]) - p, p - ld3(nodes[2 * n + 1])), spl(0.0f));
+        return dot3(e, e);
+    };
+    // the squared opening bound, refreshed when a leaf has been evaluated
+    const float ucap = ub < kFltMax ? (blend ? (ub + 1.25f * K) + 2.0f * delta : ub + 2.0f * delta) : kFltMax;
+    auto bound2 = [&]() {
+        float b = blend ? (fmn(hard, smin_poly(b1, b2, K)) + 1.25f * K) + 2.0f * delta
+                        : fmx(hard, 0.0f) + 2.0f * delta;
+        b = fmn(b, ucap);
+        return b > 0.0f ? b * b : -1.0f;
+    };
+    float open2 = bound2();
     if (S.n_obj <= kWaveScanObjs) {
-        // few objects: evaluate all of them, in slot (leaf) order, no traversal
-        for (uint32_t k = 0; k < S.n_obj; ++k) take(k, S.lin[S.n_obj + k]);
+        // few objects: the leaves in slot order (no interior nodes), each
+        // opened under the same test
+        for (uint32_t k = 0; k < S.n_obj;) {
+            const uint32_t ni = __builtin_amdgcn_readfirstlane(S.lin[S.n_obj + k]);
+            const uint32_t cnt = fbits(nodes[2 * ni + 1].w);
+            if (__ballot(need && box2(ni) < open2) != 0ull) {
+                for (uint32_t j = k; j < k + cnt; ++j) take(j, ni);
+                open2 = bound2();
+            }
+            k += cnt;
+        }
     } else {
-        // The stack holds squared box distances: the opening test only compares
-        // them with the bound, and the margins above absorb the rounding.
-        auto box2 = [&](uint32_t n) {
-            const f3 e = max3(max3(ld3(nodes[2 * n]) - p, p - ld3(nodes[2 * n + 1])), spl(0.0f));
-            return dot3(e, e);
-        };
-        // the squared opening bound, refreshed when a leaf has been evaluated
-        auto bound2 = [&]() {
-            const float b = blend ? (fmn(hard, smin_poly(b1, b2, K)) + 1.25f * K) + 2.0f * delta
-                                  : fmx(hard, 0.0f) + 2.0f * delta;
-            return b > 0.0f ? b * b : -1.0f;
-        };
-        float open2 = bound2();
         int sp = 0;
         stk.put(sp++, 0u, box2(0u));
         while (sp > 0) {
@@ -319,8 +336,8 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
 }
 
 template <class Stk>
-__device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, Stk& stk) {
-    return wave_dist_mat<false>(S, p, need, stk).d;
+__device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, Stk& stk, float ub = kFltMax) {
+    return wave_dist_mat<false>(S, p, need, stk, ub).d;
 }
 
 // Wavefront-shared overloads of the march hooks (the per-lane versions and
@@ -336,7 +353,7 @@ __device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, 
         ended = m.pre(vis, p);
         need = !ended;
     }
-    const float h = wave_dist(S, p, need, stk);
+    const float h = wave_dist(S, p, need, stk, m.dist_bound());
     if (need) ended = m.post(h, vis);
     return live && ended;
 }
@@ -350,7 +367,7 @@ __device__ __forceinline__ int miss_step(const SceneView& S, TraceMarch& m, bool
         if (m.pre(p)) st = 2;
         else need = true;
     }
-    const float h = wave_dist(S, p, need, stk);
+    const float h = wave_dist(S, p, need, stk, m.dist_bound());
     if (need) st = m.post(h);
     return st;
 }
@@ -368,7 +385,7 @@ __device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, boo
         if (m.pre(p)) st = 2;
         else need = true;
     }
-    const DistHit h = wave_dist_mat<true>(S, p, need, stk);
+    const DistHit h = wave_dist_mat<true>(S, p, need, stk, m.dist_bound());
     if (need) st = m.post(h.d);
     mat = h.mat;
     have_mat = true;

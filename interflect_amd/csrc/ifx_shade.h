@@ -40,15 +40,20 @@ struct Hit {
 // loop over it; persistent kernels interleave marches of many rays per lane.
 struct TraceMarch {
     f3 ro, rd;
-    float t, tmax;
+    float t, tmax, pd;
     uint32_t steps;
     IFX_HD void begin(f3 ro_, f3 rd_, float tmin, float tmax_) {
         ro = ro_;
         rd = rd_;
         t = tmin;
         tmax = tmax_;
+        pd = kFltMax;
         steps = 0;
     }
+    // An upper bound on the scene distance at the next sample: the previous
+    // sample's distance plus the step taken from it (the step is that
+    // distance), the scene SDF being 1-Lipschitz; kFltMax before the first.
+    IFX_HD float dist_bound() const { return pd < kFltMax ? pd + pd : kFltMax; }
     // One sample.  Returns 0 while marching, 1 on a hit (h.mat, point at t),
     // 2 on a miss.  On a hit `steps` already includes the normal's 4 taps.
     template <class Stk>
@@ -73,6 +78,7 @@ struct TraceMarch {
             steps += 4;
             return 1;
         }
+        pd = d;
         t += d;
         return t > tmax ? 2 : 0;
     }
@@ -155,6 +161,10 @@ struct ShadowMarch {
         res = 1.0f;
         steps = 0;
     }
+    // An upper bound on the scene distance at the next sample: the previous
+    // sample's distance plus the step taken from it (1-Lipschitz SDF);
+    // kFltMax before the first sample.
+    IFX_HD float dist_bound() const { return ph < 1e20f ? ph + clmp(ph, 0.006f, 0.12f) : kFltMax; }
     // A step in two halves around the distance query, so a caller can
     // answer the query for a whole wavefront at once.  pre: true when the
     // march has ended (vis set), else the sample point.
