@@ -250,7 +250,8 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
         return dot3(e, e);
     };
     // the squared opening bound, refreshed when a leaf has been evaluated
-    const float ucap = ub < kFltMax ? (blend ? (ub + 1.25f * K) + 2.0f * delta : ub + 2.0f * delta) : kFltMax;
+    const float ub0 = fmx(ub, 0.0f);  // objects containing p (box distance 0) stay openable
+    const float ucap = ub < kFltMax ? (blend ? (ub0 + 1.25f * K) + 2.0f * delta : ub0 + 2.0f * delta) : kFltMax;
     auto bound2 = [&]() {
         float b = blend ? (fmn(hard, smin_poly(b1, b2, K)) + 1.25f * K) + 2.0f * delta
                         : fmx(hard, 0.0f) + 2.0f * delta;
@@ -342,8 +343,8 @@ __device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, 
 
 // Wavefront-shared overloads of the march hooks (the per-lane versions and
 // their contracts sit beside the loops): all active lanes call together.
-__device__ __forceinline__ float query_dist(const SceneView& S, f3 p, LdsStack& stk) {
-    return wave_dist(S, p, true, stk);
+__device__ __forceinline__ float query_dist(const SceneView& S, f3 p, LdsStack& stk, float ub = kFltMax) {
+    return wave_dist(S, p, true, stk, ub);
 }
 __device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, bool live, float& vis,
                                             LdsStack& stk) {

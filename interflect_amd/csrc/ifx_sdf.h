@@ -363,8 +363,11 @@ IFX_HD float dist(const SceneView& S, f3 p, Stk& stk) {
 // gets wavefront-shared overloads (found by argument-dependent lookup where
 // the loops are instantiated) that answer all active lanes with one
 // traversal and return the same values.
+// (ub: an upper bound on the distance at p when the caller has one; the
+// per-lane version does not need it)
 template <class Stk>
-IFX_HD float query_dist(const SceneView& S, f3 p, Stk& stk) {
+IFX_HD float query_dist(const SceneView& S, f3 p, Stk& stk, float ub = kFltMax) {
+    (void)ub;
     return dist(S, p, stk);
 }
 
@@ -384,13 +387,15 @@ IFX_HD bool any_lane(bool v) {
 
 // src/scene.rs:218-236: tetrahedral 4-tap gradient.
 template <class Stk>
-IFX_HD f3 scene_normal(const SceneView& S, f3 p, Stk& stk) {
+IFX_HD f3 scene_normal(const SceneView& S, f3 p, Stk& stk, float d0 = kFltMax) {
     const float H = 5e-4f;
+    // d0: the distance at p, when known; each tap lies H sqrt(3) < 8.7e-4 away
+    const float ub = d0 < kFltMax ? d0 + 8.7e-4f : kFltMax;
     f3 n = spl(0.0f);
-    n = n + mk(1.0f, -1.0f, -1.0f) * query_dist(S, p + mk(1.0f, -1.0f, -1.0f) * H, stk);
-    n = n + mk(-1.0f, -1.0f, 1.0f) * query_dist(S, p + mk(-1.0f, -1.0f, 1.0f) * H, stk);
-    n = n + mk(-1.0f, 1.0f, -1.0f) * query_dist(S, p + mk(-1.0f, 1.0f, -1.0f) * H, stk);
-    n = n + mk(1.0f, 1.0f, 1.0f) * query_dist(S, p + mk(1.0f, 1.0f, 1.0f) * H, stk);
+    n = n + mk(1.0f, -1.0f, -1.0f) * query_dist(S, p + mk(1.0f, -1.0f, -1.0f) * H, stk, ub);
+    n = n + mk(-1.0f, -1.0f, 1.0f) * query_dist(S, p + mk(-1.0f, -1.0f, 1.0f) * H, stk, ub);
+    n = n + mk(-1.0f, 1.0f, -1.0f) * query_dist(S, p + mk(-1.0f, 1.0f, -1.0f) * H, stk, ub);
+    n = n + mk(1.0f, 1.0f, 1.0f) * query_dist(S, p + mk(1.0f, 1.0f, 1.0f) * H, stk, ub);
     float l = len3(n);
     if (l > 1e-12f) return n / l;
     return mk(0.0f, 1.0f, 0.0f);

@@ -40,7 +40,7 @@ struct Hit {
 // loop over it; persistent kernels interleave marches of many rays per lane.
 struct TraceMarch {
     f3 ro, rd;
-    float t, tmax, pd;
+    float t, tmax, pd, ld;  // pd: previous step's distance; ld: the last sample's
     uint32_t steps;
     IFX_HD void begin(f3 ro_, f3 rd_, float tmin, float tmax_) {
         ro = ro_;
@@ -48,6 +48,7 @@ struct TraceMarch {
         t = tmin;
         tmax = tmax_;
         pd = kFltMax;
+        ld = kFltMax;
         steps = 0;
     }
     // An upper bound on the scene distance at the next sample: the previous
@@ -73,6 +74,7 @@ struct TraceMarch {
     }
     IFX_HD int post(float d) {
         steps += 1;
+        ld = d;
         float eps = 1e-4f * fmx(t, 1.0f);
         if (d < eps) {
             steps += 4;
@@ -133,7 +135,7 @@ IFX_HD Hit trace(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, uint3
         r.p = ro + rd * m.t;
         // (the sample's material: the same query as the step's)
         if (!have_mat) mat = query_mat(S, r.p, stk);
-        r.n = NORMAL ? scene_normal(S, r.p, stk) : spl(0.0f);
+        r.n = NORMAL ? scene_normal(S, r.p, stk, m.ld) : spl(0.0f);
         r.mat = mat;
         r.hit = true;
     } else {
