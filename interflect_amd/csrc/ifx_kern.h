@@ -224,11 +224,40 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
         float d = obj_dist_at(planes + 4 * i, subs, p, mat, bl, stk);
         if (d < hard) hard = d, hmat = mat;
     }
-    // object k of leaf ni at p: the lane's running minima
+    // Squared box distances: the opening test only compares them with the
+    // bound, and the margins above absorb the rounding.
+    auto box2 = [&](uint32_t n) {
+        const f3 e = max3(max3(ld3(nodes[2 * n]) - p, p - ld3(nodes[2 * n + 1])), spl(0.0f));
+        return dot3(e, e);
+    };
+    // the opening bound (blin; open2 its square, -1 when not positive),
+    // refreshed when a leaf has been evaluated
+    const float ub0 = fmx(ub, 0.0f);  // objects containing p (box distance 0) stay openable
+    const float ucap = ub < kFltMax ? (blend ? (ub0 + 1.25f * K) + 2.0f * delta : ub0 + 2.0f * delta) : kFltMax;
+    float blin = kFltMax, open2 = kFltMax;
+    auto refresh = [&]() {
+        float b = blend ? (fmn(hard, smin_poly(b1, b2, K)) + 1.25f * K) + 2.0f * delta
+                        : fmx(hard, 0.0f) + 2.0f * delta;
+        blin = fmn(b, ucap);
+        open2 = blin > 0.0f ? blin * blin : -1.0f;
+    };
+    refresh();
+    // object k of leaf ni at p: the lane's running minima.  A CSG object is
+    // skipped when its base primitive (a lower bound: subtraction only raises
+    // the value) is beyond every lane's bound, as an unopened node is.
     auto take = [&](uint32_t k, uint32_t ni) {
-        uint32_t mat;
-        bool bl;
-        const float d = obj_dist_at(objs + 4 * k, subs, p, mat, bl, stk);
+        const ConstF4 rec = objs + 4 * k;
+        const float4 g0 = rec[0], g1 = rec[1], g2 = rec[2], g3 = rec[3];
+        const uint32_t mat = fbits(g3.x), sb = fbits(g3.y), sc = fbits(g3.z);
+        const bool bl = fbits(g3.w) != 0u;
+        float d = prim_dist(g0, g1, g2, p);
+        if (sc > 0) {
+            if (__ballot(need && d < blin) == 0ull) return;
+            for (uint32_t q = sb; q < sb + sc; ++q) {
+                const ConstF4 r = subs + 4 * q;
+                d = fmx(d, -prim_dist(r[0], r[1], r[2], p));
+            }
+        }
         if (blend && bl) {
             if (d < b1) {
                 b2 = b1, s2 = s1;
@@ -243,22 +272,6 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
             h2 = fmn(h2, d);
         }
     };
-    // Squared box distances: the opening test only compares them with the
-    // bound, and the margins above absorb the rounding.
-    auto box2 = [&](uint32_t n) {
-        const f3 e = max3(max3(ld3(nodes[2 * n]) - p, p - ld3(nodes[2 * n + 1])), spl(0.0f));
-        return dot3(e, e);
-    };
-    // the squared opening bound, refreshed when a leaf has been evaluated
-    const float ub0 = fmx(ub, 0.0f);  // objects containing p (box distance 0) stay openable
-    const float ucap = ub < kFltMax ? (blend ? (ub0 + 1.25f * K) + 2.0f * delta : ub0 + 2.0f * delta) : kFltMax;
-    auto bound2 = [&]() {
-        float b = blend ? (fmn(hard, smin_poly(b1, b2, K)) + 1.25f * K) + 2.0f * delta
-                        : fmx(hard, 0.0f) + 2.0f * delta;
-        b = fmn(b, ucap);
-        return b > 0.0f ? b * b : -1.0f;
-    };
-    float open2 = bound2();
     if (S.n_obj <= kWaveScanObjs) {
         // few objects: the leaves in slot order (no interior nodes), each
         // opened under the same test
@@ -267,7 +280,7 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
             const uint32_t cnt = fbits(nodes[2 * ni + 1].w);
             if (__ballot(need && box2(ni) < open2) != 0ull) {
                 for (uint32_t j = k; j < k + cnt; ++j) take(j, ni);
-                open2 = bound2();
+                refresh();
             }
             k += cnt;
         }
@@ -281,7 +294,7 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
             const uint32_t w0 = fbits(nodes[2 * ni].w), cnt = fbits(nodes[2 * ni + 1].w);
             if (cnt > 0) {
                 for (uint32_t k = w0; k < w0 + cnt; ++k) take(k, ni);
-                open2 = bound2();
+                refresh();
             } else {
                 const uint32_t l = ni + 1, r = w0;
                 const float dl = box2(l), dr = box2(r);
