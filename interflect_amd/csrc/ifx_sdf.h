@@ -157,6 +157,31 @@ IFX_HD float obj_dist(const float4* __restrict__ rec, const float4* __restrict__
     return obj_dist_at(rec, subs, p, mat, blended, stk);
 }
 
+// obj_dist for a caller that only compares the value against a running
+// minimum: `cut` for a hard object, `cut_bl` for a blended one.  When the
+// base primitive is already >= that cut, the subtractions (which only raise
+// the value) cannot bring it below, so they are skipped and the base value
+// returned: the caller's comparisons come out the same, bit for bit.  The
+// counting stack evaluates everything (the reference algorithm's count).
+template <class Stk>
+IFX_HD float obj_dist_cut(const float4* __restrict__ rec, const float4* __restrict__ subs, f3 p, float cut,
+                          float cut_bl, uint32_t& mat, bool& blended, Stk& stk) {
+    if (!Stk::kSkipCsg) return obj_dist_at(rec, subs, p, mat, blended, stk);
+    float4 g0 = rec[0], g1 = rec[1], g2 = rec[2], g3 = rec[3];
+    mat = fbits(g3.x);
+    uint32_t sb = fbits(g3.y), sc = fbits(g3.z);
+    blended = fbits(g3.w) != 0u;
+    float d = prim_dist(g0, g1, g2, p);
+    stk.count_prim(fbits(g2.z), 0u);
+    if (d >= (blended ? cut_bl : cut)) return d;
+    for (uint32_t s = sb; s < sb + sc; ++s) {
+        const float4* q = subs + 4 * s;
+        d = fmx(d, -prim_dist(q[0], q[1], q[2], p));
+        stk.count_prim(fbits(q[2].z), 2u);
+    }
+    return d;
+}
+
 // src/bvh.rs:45-50
 IFX_HD float aabb_dist(f3 p, float4 lo, float4 hi) {
     f3 d = max3(max3(ld3(lo) - p, p - ld3(hi)), spl(0.0f));
@@ -189,6 +214,7 @@ constexpr uint32_t kEvalOps = 15u;
 // queries with one wavefront-shared traversal (the hooks below).
 struct ArrStack {
     static constexpr bool kShared = false;
+    static constexpr bool kSkipCsg = true;
     uint32_t v[64];
     float dv[64];
     IFX_HD void put(int i, uint32_t x, float d) {
@@ -206,6 +232,7 @@ struct ArrStack {
 
 struct LdsStack {
     static constexpr bool kShared = true;
+    static constexpr bool kSkipCsg = true;
     uint16_t* base;
     float* dbase;
     uint32_t stride;
@@ -224,6 +251,7 @@ struct LdsStack {
 
 struct CountStack {
     static constexpr bool kShared = false;
+    static constexpr bool kSkipCsg = false;
     uint16_t* base;
     float* dbase;
     uint32_t stride;
@@ -280,7 +308,7 @@ IFX_HD void bvh_query(const SceneView& S, f3 p, BlendState& bs, Stk& stk) {
             for (uint32_t k = w0; k < w0 + cnt; ++k) {
                 uint32_t mat;
                 bool bl;
-                float d = obj_dist(S.obj + 4 * k, S.subs, p, mat, bl, stk);
+                float d = obj_dist_cut(S.obj + 4 * k, S.subs, p, bs.hard.d, BLEND ? bs.b2 : bs.hard.d, mat, bl, stk);
                 if (BLEND) {
                     blend_take(bs, d, mat, bl);
                 } else if (d < bs.hard.d) {
@@ -315,7 +343,7 @@ IFX_HD DistHit dist_mat(const SceneView& S, f3 p, Stk& stk) {
     for (uint32_t i = 0; i < S.n_planes; ++i) {
         uint32_t mat;
         bool bl;
-        float d = obj_dist(S.planes + 4 * i, S.subs, p, mat, bl, stk);
+        float d = obj_dist_cut(S.planes + 4 * i, S.subs, p, bs.hard.d, bs.hard.d, mat, bl, stk);
         if (d < bs.hard.d) {
             bs.hard.d = d;
             bs.hard.mat = mat;
@@ -328,7 +356,7 @@ IFX_HD DistHit dist_mat(const SceneView& S, f3 p, Stk& stk) {
             for (uint32_t i = 0; i < S.n_obj; ++i) {
                 uint32_t mat;
                 bool bl;
-                float d = obj_dist(S.obj + 4 * S.lin[i], S.subs, p, mat, bl, stk);
+                float d = obj_dist_cut(S.obj + 4 * S.lin[i], S.subs, p, bs.hard.d, bs.hard.d, mat, bl, stk);
                 if (d < bs.hard.d) {
                     bs.hard.d = d;
                     bs.hard.mat = mat;
@@ -343,7 +371,7 @@ IFX_HD DistHit dist_mat(const SceneView& S, f3 p, Stk& stk) {
         for (uint32_t i = 0; i < S.n_obj; ++i) {
             uint32_t mat;
             bool bl;
-            float d = obj_dist(S.obj + 4 * S.lin[i], S.subs, p, mat, bl, stk);
+            float d = obj_dist_cut(S.obj + 4 * S.lin[i], S.subs, p, bs.hard.d, bs.b2, mat, bl, stk);
             blend_take(bs, d, mat, bl);
         }
     }
