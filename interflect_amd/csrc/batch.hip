@@ -435,7 +435,22 @@ __global__ __launch_bounds__(kBlock) void kb_jacobi_rows(BJacobi A, uint32_t it)
             if (!bj_active(A, s, it)) continue;
             f3 g = spl(0.0f);
             uint32_t e = A.lstart[i + 1];
-            for (uint32_t k = A.lstart[i]; k < e; ++k) g = g + ld3(A.bc[A.idx[k]]) * A.w[k];
+            uint32_t k = A.lstart[i];
+            for (; k + kRowBatch <= e; k += kRowBatch) {
+                uint32_t ix[kRowBatch];
+                float wv[kRowBatch];
+                f3 bv[kRowBatch];
+#pragma unroll
+                for (uint32_t j = 0; j < kRowBatch; ++j) {
+                    ix[j] = A.idx[k + j];
+                    wv[j] = A.w[k + j];
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kRowBatch; ++j) bv[j] = ld3(A.bc[ix[j]]);
+#pragma unroll
+                for (uint32_t j = 0; j < kRowBatch; ++j) g = g + bv[j] * wv[j];
+            }
+            for (; k < e; ++k) g = g + ld3(A.bc[A.idx[k]]) * A.w[k];
             f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
             uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
             A.irr[i] = st4(g);

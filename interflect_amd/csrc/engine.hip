@@ -495,7 +495,22 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_rows(JacobiArgs A, uint32_t i
             f3 g = spl(0.0f);
             const uint32_t len = A.lstart[i + 1] - A.lstart[i];
             const uint64_t b = 64ull * A.sbase[i >> 6] + (i & 63u);
-            for (uint32_t k = 0; k < len; ++k) g = g + ld3(A.bc[A.sidx[b + 64ull * k]]) * A.sw[b + 64ull * k];
+            uint32_t k = 0;
+            for (; k + kRowBatch <= len; k += kRowBatch) {
+                uint32_t ix[kRowBatch];
+                float wv[kRowBatch];
+                f3 bv[kRowBatch];
+#pragma unroll
+                for (uint32_t j = 0; j < kRowBatch; ++j) {
+                    ix[j] = A.sidx[b + 64ull * (k + j)];
+                    wv[j] = A.sw[b + 64ull * (k + j)];
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kRowBatch; ++j) bv[j] = ld3(A.bc[ix[j]]);
+#pragma unroll
+                for (uint32_t j = 0; j < kRowBatch; ++j) g = g + bv[j] * wv[j];
+            }
+            for (; k < len; ++k) g = g + ld3(A.bc[A.sidx[b + 64ull * k]]) * A.sw[b + 64ull * k];
             f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
             uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
             delta = d > delta ? d : delta;
@@ -505,8 +520,23 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_rows(JacobiArgs A, uint32_t i
     } else {
         for (uint32_t i = gt; i < A.N; i += nt) {
             f3 g = spl(0.0f);
-            uint32_t e = A.lstart[i + 1];
-            for (uint32_t k = A.lstart[i]; k < e; ++k) g = g + ld3(A.bc[A.idx[k]]) * A.w[k];
+            const uint32_t e = A.lstart[i + 1];
+            uint32_t k = A.lstart[i];
+            for (; k + kRowBatch <= e; k += kRowBatch) {
+                uint32_t ix[kRowBatch];
+                float wv[kRowBatch];
+                f3 bv[kRowBatch];
+#pragma unroll
+                for (uint32_t j = 0; j < kRowBatch; ++j) {
+                    ix[j] = A.idx[k + j];
+                    wv[j] = A.w[k + j];
+                }
+#pragma unroll
+                for (uint32_t j = 0; j < kRowBatch; ++j) bv[j] = ld3(A.bc[ix[j]]);
+#pragma unroll
+                for (uint32_t j = 0; j < kRowBatch; ++j) g = g + bv[j] * wv[j];
+            }
+            for (; k < e; ++k) g = g + ld3(A.bc[A.idx[k]]) * A.w[k];
             f3 nr = ld3(A.direct[i]) + ld3(A.salb[i]) * g;
             uint32_t d = fbits(maxe(abs3(nr - ld3(A.rad[i]))));
             delta = d > delta ? d : delta;

@@ -406,6 +406,11 @@ __device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, boo
     return st;
 }
 
+// Links per batch of a Jacobi row: the batch's index, weight and cluster
+// loads are issued together, then summed in link order (the order, and so
+// every bit, of the serial sum).
+constexpr uint32_t kRowBatch = 8;
+
 // Kernel prologue: stage, then place this lane's stack column after the staged records.
 #define IFX_SETUP(LDS, LIGHTS, COUNT) IFX_SETUP_T(LDS, LIGHTS, typename StackOf<COUNT>::T)
 // Camera-ray shading kernels (see ShadeStackOf).
