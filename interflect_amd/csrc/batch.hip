@@ -1075,7 +1075,7 @@ int Batch::frame(bool eager, std::string& err) {
     }
     fill(d_lcount, N + 1, 0u);
     if (N)
-        k_row_sum<<<blocks_for(N, 256), 256, 0, st>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
+        k_row_sum<<<blocks_for(64ull * N, 256), 256, 0, st>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
                                                       d_open.as<uint32_t>(), d_sky.as<float>(),
                                                       d_lcount.as<uint32_t>(), d_scale.as<float>(),
                                                       d_rowsum.as<float>());
@@ -1089,7 +1089,7 @@ int Batch::frame(bool eager, std::string& err) {
         BK(d_w.ensure(4ull * L + 4));
     }
     if (N)
-        k_compact<<<blocks_for(N, 256), 256, 0, st>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
+        k_compact<<<blocks_for(64ull * N, 256), 256, 0, st>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
                                                       d_cand_ci.as<uint32_t>(), d_lstart.as<uint32_t>(),
                                                       d_scale.as<float>(), d_idx.as<uint32_t>(), d_w.as<float>());
 

@@ -312,17 +312,24 @@ __global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg,
 __global__ void k_row_sum(uint32_t N, const uint32_t* __restrict__ cstart, const float* __restrict__ cand_v,
                           const uint32_t* __restrict__ open, float* sky_out, uint32_t* lcount, float* scale,
                           float* row_sum) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wavefront per row: its candidates are read coalesced, and the kept
+    // values are added in candidate order (readlane), the serial sum
+    const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = lane_id();
     if (i >= N) return;
     float sum = 0.0f;
     uint32_t cnt = 0;
-    for (uint32_t k = cstart[i]; k < cstart[i + 1]; ++k) {
-        float v = cand_v[k];
-        if (v > 0.0f) {
-            sum += v;
-            ++cnt;
+    for (uint32_t b = cstart[i], e = cstart[i + 1]; b < e; b += 64u) {
+        const uint32_t k = b + lane;
+        const float v = k < e ? cand_v[k] : 0.0f;
+        uint64_t m = __ballot(k < e && v > 0.0f);
+        cnt += (uint32_t)__popcll(m);
+        while (m) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(m);
+            sum += bitsf(__builtin_amdgcn_readlane(fbits(v), j));
+            m &= m - 1ull;
         }
     }
+    if (lane) return;
     float sky = (float)open[i] / 16.0f;
     sky_out[i] = sky;
     float target = clmp(1.0f - sky, 0.0f, 1.0f);
@@ -336,20 +343,25 @@ __global__ void k_row_sum(uint32_t N, const uint32_t* __restrict__ cstart, const
     row_sum[i] = cnt ? fmn(sum, 1.0f) : -1.0f;
 }
 
+// One wavefront per row: kept candidates in candidate order (ballot ranks).
 __global__ void k_compact(uint32_t N, const uint32_t* __restrict__ cstart, const float* __restrict__ cand_v,
                           const uint32_t* __restrict__ cand_ci, const uint32_t* __restrict__ lstart,
                           const float* __restrict__ scale, uint32_t* idx, float* w) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = lane_id();
     if (i >= N) return;
     uint32_t pos = lstart[i];
-    float sc = scale[i];
-    for (uint32_t k = cstart[i]; k < cstart[i + 1]; ++k) {
-        float v = cand_v[k];
-        if (v > 0.0f) {
-            idx[pos] = cand_ci[k];
-            w[pos] = v * sc;
-            ++pos;
+    const float sc = scale[i];
+    for (uint32_t b = cstart[i], e = cstart[i + 1]; b < e; b += 64u) {
+        const uint32_t k = b + lane;
+        const float v = k < e ? cand_v[k] : 0.0f;
+        const bool keep = k < e && v > 0.0f;
+        const uint64_t m = __ballot(keep);
+        if (keep) {
+            const uint32_t at = pos + lanes_below(m);
+            idx[at] = cand_ci[k];
+            w[at] = v * sc;
         }
+        pos += (uint32_t)__popcll(m);
     }
 }
 
@@ -1104,7 +1116,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     CK(d_rowsum.ensure(4ull * N));
     CK(hipMemsetAsync(d_lcount.as<uint32_t>() + N, 0, 4, stream));
     launch("row_sum", [&] {
-        k_row_sum<<<blocks_for(N, 256), 256, 0, stream>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
+        k_row_sum<<<blocks_for(64ull * N, 256), 256, 0, stream>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
                                                           d_open.as<uint32_t>(), d_sky.as<float>(),
                                                           d_lcount.as<uint32_t>(), d_scale.as<float>(),
                                                           d_rowsum.as<float>());
@@ -1121,7 +1133,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     CK(d_idx.ensure(4ull * L));
     CK(d_w.ensure(4ull * L));
     launch("compact", [&] {
-        k_compact<<<blocks_for(N, 256), 256, 0, stream>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
+        k_compact<<<blocks_for(64ull * N, 256), 256, 0, stream>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
                                                           d_cand_ci.as<uint32_t>(), d_lstart.as<uint32_t>(),
                                                           d_scale.as<float>(), d_idx.as<uint32_t>(), d_w.as<float>());
     });
