@@ -950,7 +950,7 @@ int Batch::frame(bool eager, std::string& err) {
     if (nseg) {
         k_seg_begin<<<blocks_for(N, 256), 256, 0, st>>>(d_cflag.as<uint32_t>(), d_cseg.as<uint32_t>(), N,
                                                         d_cbegin.as<uint32_t>());
-        k_cluster_build<<<blocks_for(nseg, 64), 64, 0, st>>>(d_cbegin.as<uint32_t>(), nseg, members.as<uint32_t>(),
+        k_cluster_build<<<blocks_for(64ull * nseg, 256), 256, 0, st>>>(d_cbegin.as<uint32_t>(), nseg, members.as<uint32_t>(),
                                                              spos, snrm, d_craw_pos.as<float4>(),
                                                              d_craw_nrm.as<float4>(), d_cvalid.as<uint32_t>());
         BK(gsort::exclusive_scan(d_cvalid.as<uint32_t>(), d_cidx.as<uint32_t>(), nseg, d_scratch.as<uint32_t>(), st));

@@ -121,20 +121,39 @@ __global__ void k_seg_begin(const uint32_t* flags, const uint32_t* seg, uint32_t
 }
 
 // Serial area-weighted sums in ascending member order (src/formfactor.rs:120-145).
+// One wavefront per cluster: the members' records are loaded 64 at a time
+// and their weighted terms added in member order through readlane (the
+// serial sums, bit for bit).
 __global__ void k_cluster_build(const uint32_t* begin, uint32_t nseg, const uint32_t* members,
                                 const float4* __restrict__ spos, const float4* __restrict__ snrm, float4* cpos,
                                 float4* cnrm, uint32_t* valid) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = lane_id();
     if (c >= nseg) return;
     f3 p = spl(0.0f), n = spl(0.0f);
     float area = 0.0f;
-    for (uint32_t k = begin[c]; k < begin[c + 1]; ++k) {
-        uint32_t m = members[k];
-        float4 P = spos[m];
-        p = p + ld3(P) * P.w;
-        n = n + ld3(snrm[m]) * P.w;
-        area += P.w;
+    for (uint32_t b = begin[c], e = begin[c + 1]; b < e; b += 64u) {
+        const uint32_t k = b + lane;
+        f3 tp = spl(0.0f), tn = spl(0.0f);
+        float ta = 0.0f;
+        if (k < e) {
+            const uint32_t m = members[k];
+            const float4 P = spos[m];
+            tp = ld3(P) * P.w;
+            tn = ld3(snrm[m]) * P.w;
+            ta = P.w;
+        }
+        const uint32_t cnt = e - b < 64u ? e - b : 64u;
+        for (uint32_t j = 0; j < cnt; ++j) {
+            p = p + mk(bitsf(__builtin_amdgcn_readlane(fbits(tp.x), j)),
+                       bitsf(__builtin_amdgcn_readlane(fbits(tp.y), j)),
+                       bitsf(__builtin_amdgcn_readlane(fbits(tp.z), j)));
+            n = n + mk(bitsf(__builtin_amdgcn_readlane(fbits(tn.x), j)),
+                       bitsf(__builtin_amdgcn_readlane(fbits(tn.y), j)),
+                       bitsf(__builtin_amdgcn_readlane(fbits(tn.z), j)));
+            area += bitsf(__builtin_amdgcn_readlane(fbits(ta), j));
+        }
     }
+    if (lane) return;
     valid[c] = area <= 1e-12f ? 0u : 1u;
     f3 nn = dot3(n, n) > 1e-12f ? nrm3(n) : mk(0.0f, 1.0f, 0.0f);
     cpos[c] = st4(p / area, area);
@@ -975,7 +994,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
                                                             seg_begin.as<uint32_t>());
     });
     launch("cluster_build", [&] {
-        k_cluster_build<<<blocks_for(nseg, 64), 64, 0, stream>>>(seg_begin.as<uint32_t>(), nseg, members, spos, snrm,
+        k_cluster_build<<<blocks_for(64ull * nseg, 256), 256, 0, stream>>>(seg_begin.as<uint32_t>(), nseg, members, spos, snrm,
                                                                  raw_pos.as<float4>(), raw_nrm.as<float4>(),
                                                                  d_cvalid.as<uint32_t>());
     });
