@@ -410,6 +410,10 @@ int Engine::set_material_albedo(uint32_t m, const float* rgb, std::string& err) 
     have_solve = have_gterm_ = have_beauty_ = false;
     if (!has_device_) return 0;
     CK(hipSetDevice(device_));
+    // the side stream's precomputed direct terms used the old value
+    if (side_) CK(hipStreamSynchronize(side_));
+    join_side();
+    direct_pre_ = false;
     CK(hipMemcpy(d_mat.p, scene.mat_rec.data(), 16ull * scene.mat_rec.size(), hipMemcpyHostToDevice));
     if (!surfels_dirty && !surfels.empty()) {
         std::vector<float4> ha(surfels.size());
@@ -439,6 +443,10 @@ int Engine::set_light_emit(uint32_t li, const float* rgb, std::string& err) {
     have_solve = have_gterm_ = have_beauty_ = false;
     if (!has_device_) return 0;
     CK(hipSetDevice(device_));
+    // the side stream's precomputed direct terms used the old value
+    if (side_) CK(hipStreamSynchronize(side_));
+    join_side();
+    direct_pre_ = false;
     if (nq) CK(hipMemcpy(d_quad.p, scene.quad_rec.data(), 16ull * scene.quad_rec.size(), hipMemcpyHostToDevice));
     if (ns) CK(hipMemcpy(d_sph.p, scene.sph_rec.data(), 16ull * scene.sph_rec.size(), hipMemcpyHostToDevice));
     if (np) CK(hipMemcpy(d_pnt.p, scene.pnt_rec.data(), 16ull * scene.pnt_rec.size(), hipMemcpyHostToDevice));

@@ -149,6 +149,9 @@ class Engine {
     uint32_t jacobi_rows = 0;  // 0 automatic, 1 lane per row on the CSR, 2 wavefront per row, 3 sliced rows
     float solve_tol = 1e-4f;   // Jacobi stops once max |delta| < solve_tol (src/solve.rs:98)
     bool place_on_gpu = true;  // ifx_generate_surfels on the device when there is one
+    // transfer() also launches the direct-light terms on a second stream, so
+    // they run beside the tracing kernels; solve() then waits for them.
+    bool overlap_direct = true;
     uint32_t spp = 512;        // paths per pixel of the reference mode (--spp, src/main.rs:81)
     uint32_t adj_max_iters = 1000;  // adjoint sweeps at most
     float adj_tol = 1e-6f;          // adjoint stops once max |delta mu| <= adj_tol * max |mu|
@@ -179,8 +182,20 @@ class Engine {
     SceneView dev_view() const;
     LightsView dev_lights() const;
     template <class F>
-    void launch(const char* name, F&& f);
+    void launch(const char* name, F&& f) {
+        launch_on(stream, name, static_cast<F&&>(f));
+    }
+    template <class F>
+    void launch_on(hipStream_t s, const char* name, F&& f);
     void drain_events();
+    // direct-light terms launched by transfer() on side_ (see overlap_direct)
+    int launch_direct(hipStream_t s, unsigned long long* evals, unsigned long long* ops, std::string& err);
+    void join_side();  // the main stream waits for side_'s work
+    hipStream_t side_ = nullptr;
+    hipEvent_t fork_ = nullptr, join_ = nullptr;
+    bool direct_pre_ = false;    // terms from the last transfer are pending on side_ and still valid
+    bool side_pending_ = false;  // side_ holds work the main stream has not waited for
+    DevBuf d_dctr;
 
     int device_ = 0;
     int n_cu_ = 1;

@@ -721,9 +721,13 @@ int Engine::check(hipError_t e, const char* what, std::string& err) {
 Engine::~Engine() {
     if (stream) {
         (void)hipSetDevice(device_);
+        if (side_) (void)hipStreamSynchronize(side_);
         (void)hipStreamSynchronize(stream);
         drain_events();
         for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
+        if (fork_) (void)hipEventDestroy(fork_);
+        if (join_) (void)hipEventDestroy(join_);
+        if (side_) (void)hipStreamDestroy(side_);
         (void)hipStreamDestroy(stream);
     }
 }
@@ -742,6 +746,9 @@ int Engine::init(int device, std::string& err) {
     device_ = device;
     CK(hipSetDevice(device));
     CK(hipStreamCreate(&stream));
+    CK(hipStreamCreate(&side_));
+    CK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
     {
         int cus = 0;
         CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -788,17 +795,31 @@ int Engine::need_device(std::string& err) const {
 }
 
 void Engine::drain_events() {
+    std::vector<Pending> keep;
     for (Pending& p : pending_) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+        const hipError_t e = hipEventElapsedTime(&ms, p.a, p.b);
+        if (e == hipErrorNotReady) {  // still running on the side stream
+            (void)hipGetLastError();
+            keep.push_back(p);
+            continue;
+        }
+        if (e == hipSuccess) {
             KernelStat& k = kstats_[p.name];
             k.launches += 1;
             k.ms += ms;
         }
+        (void)hipGetLastError();
         event_pool_.push_back(p.a);
         event_pool_.push_back(p.b);
     }
-    pending_.clear();
+    pending_.swap(keep);
+}
+
+void Engine::join_side() {
+    if (!side_pending_) return;
+    (void)hipStreamWaitEvent(stream, join_, 0);
+    side_pending_ = false;
 }
 
 SceneView Engine::dev_view() const {
@@ -852,6 +873,8 @@ int Engine::load_scene(const std::string& src, std::string& err) {
         err = "scene has more than 65535 BVH nodes";
         return -6;
     }
+    if (side_) (void)hipStreamSynchronize(side_);  // no side work may still read the old records
+    direct_pre_ = side_pending_ = false;
     if (!has_device_) {
         scene = std::move(s);
         surfels.clear();
@@ -905,6 +928,8 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         return -1;
     }
     CK(hipSetDevice(device_));
+    join_side();  // earlier side work may still read the surfel records
+    direct_pre_ = false;
     const uint32_t N = (uint32_t)surfels.size();
     have_transfer = have_solve = have_beauty_ = have_gterm_ = false;
     ++transfer_gen_;
@@ -1035,6 +1060,18 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         CK(d_tmp.ensure(tmp));
         CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_morton.as<uint32_t>(), d_mkey_s.as<uint32_t>(),
                                               d_mval.as<uint32_t>(), d_rorder.as<uint32_t>(), (int)N, 0, 15, stream));
+    }
+    // Direct-light terms depend only on the surfels, scene and lights: launch
+    // them now on the side stream, beside the tracing kernels; solve() joins.
+    if (overlap_direct && !count_ops_ && n_lights(dev_lights()) > 0) {
+        CK(d_dctr.ensure(64));
+        CK(hipMemsetAsync(d_dctr.p, 0, 8, stream));
+        CK(hipEventRecord(fork_, stream));
+        CK(hipStreamWaitEvent(side_, fork_, 0));
+        if (int rc = launch_direct(side_, d_dctr.as<unsigned long long>(), nullptr, err)) return rc;
+        CK(hipEventRecord(join_, side_));
+        side_pending_ = true;
+        direct_pre_ = true;
     }
     CK(d_queue.ensure(16));
     CK(hipMemsetAsync(d_queue.p, 0, 16, stream));
@@ -1177,6 +1214,26 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     return 0;
 }
 
+// Direct-light terms (k_direct_terms) on stream s; terms, geometry terms and
+// the evaluation count go to d_terms, d_gterm and evals.
+int Engine::launch_direct(hipStream_t s, unsigned long long* evals, unsigned long long* ops, std::string& err) {
+    const uint32_t N = (uint32_t)surfels.size();
+    const LightsView Lv = dev_lights();
+    const uint32_t nl = n_lights(Lv);
+    CK(d_terms.ensure(16ull * N * (nl ? nl : 1)));
+    CK(d_gterm.ensure(4ull * N * (nl ? nl : 1)));
+    if (!N || !nl) return 0;
+    const SceneView S = dev_view();
+    const Lds ld = plan_lds(true);
+    hipStream_t stream = s;  // the launch macro's stream
+    launch_on(stream, "direct_e", [&] {
+        LAUNCH_CL(k_direct_terms, count_ops_, ld.staged, blocks_for((uint64_t)N * nl, kBlock), ld.bytes, S, Lv,
+                  ld.plan, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, nl,
+                  d_rorder.as<uint32_t>(), d_terms.as<float4>(), d_gterm.as<float>(), evals, ops);
+    });
+    return 0;
+}
+
 int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
     if (!have_transfer) {
         err = "solve called before transfer";
@@ -1184,6 +1241,10 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
     }
     CK(hipSetDevice(device_));
     out = SolveOut();
+    // direct-light terms precomputed by transfer() on the side stream
+    const bool pre = direct_pre_ && !count_ops_;
+    join_side();
+    direct_pre_ = false;
     const uint32_t N = (uint32_t)surfels.size();
     const LightsView Lv = dev_lights();
     const uint32_t nl = n_lights(Lv);
@@ -1199,14 +1260,8 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
     CK(hipMemsetAsync(d_resid.p, 0, 4ull * (bounces + 1), stream));
     const SceneView S = dev_view();
     if (N) {
-        const Lds ld = plan_lds(true);
-        if (nl) {
-            launch("direct_e", [&] {
-                LAUNCH_CL(k_direct_terms, count_ops_, ld.staged, blocks_for((uint64_t)N * nl, kBlock), ld.bytes, S,
-                          Lv, ld.plan, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, nl,
-                          d_rorder.as<uint32_t>(), d_terms.as<float4>(), d_gterm.as<float>(), ctr,
-                          count_ops_ ? ctr + 3 : nullptr);
-            });
+        if (nl && !pre) {
+            if (int rc = launch_direct(stream, ctr, count_ops_ ? ctr + 3 : nullptr, err)) return rc;
         }
         launch("direct_sum", [&] {
             k_direct_sum<<<blocks_for(N, 256), 256, 0, stream>>>(N, nl, d_terms.as<float4>(), d_direct.as<float4>(),
@@ -1274,6 +1329,11 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
     CK(hipStreamSynchronize(stream));
     drain_events();
     unsigned long long ev = cv[0];
+    if (pre) {  // the direct-light evaluations counted on the side stream
+        unsigned long long dv = 0;
+        CK(hipMemcpy(&dv, d_dctr.p, 8, hipMemcpyDeviceToHost));
+        ev += dv;
+    }
     if (count_ops_) kops_["direct_e"] += cv[3];
     out.evals = ev;
     have_beauty_ = false;

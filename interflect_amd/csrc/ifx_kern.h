@@ -624,7 +624,7 @@ inline uint32_t bits_for(uint64_t maxv) {
 }
 
 template <class F>
-void Engine::launch(const char* name, F&& f) {
+void Engine::launch_on(hipStream_t stream, const char* name, F&& f) {
     if (!prof_) {
         f();
         return;
