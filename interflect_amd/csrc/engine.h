@@ -149,8 +149,10 @@ class Engine {
     uint32_t jacobi_rows = 0;  // 0 automatic, 1 lane per row on the CSR, 2 wavefront per row, 3 sliced rows
     float solve_tol = 1e-4f;   // Jacobi stops once max |delta| < solve_tol (src/solve.rs:98)
     bool place_on_gpu = true;  // ifx_generate_surfels on the device when there is one
-    // transfer() also launches the direct-light terms on a second stream, so
-    // they run beside the tracing kernels; solve() then waits for them.
+    // transfer() launches the closure rays and the direct-light terms on a
+    // second stream, beside the cluster build, the candidate sort and the
+    // visibility kernel; the link calibration waits for the closure and
+    // solve() for the terms.
     bool overlap_direct = true;
     uint32_t spp = 512;        // paths per pixel of the reference mode (--spp, src/main.rs:81)
     uint32_t adj_max_iters = 1000;  // adjoint sweeps at most
@@ -188,11 +190,11 @@ class Engine {
     template <class F>
     void launch_on(hipStream_t s, const char* name, F&& f);
     void drain_events();
-    // direct-light terms launched by transfer() on side_ (see overlap_direct)
+    // closure and direct-light terms launched by transfer() on side_ (see overlap_direct)
     int launch_direct(hipStream_t s, unsigned long long* evals, unsigned long long* ops, std::string& err);
     void join_side();  // the main stream waits for side_'s work
     hipStream_t side_ = nullptr;
-    hipEvent_t fork_ = nullptr, join_ = nullptr;
+    hipEvent_t fork_ = nullptr, join_ = nullptr, closed_ = nullptr;  // closed_: the closure kernel is done
     bool direct_pre_ = false;    // terms from the last transfer are pending on side_ and still valid
     bool side_pending_ = false;  // side_ holds work the main stream has not waited for
     DevBuf d_dctr;

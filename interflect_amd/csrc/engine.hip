@@ -727,6 +727,7 @@ Engine::~Engine() {
         for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
         if (fork_) (void)hipEventDestroy(fork_);
         if (join_) (void)hipEventDestroy(join_);
+        if (closed_) (void)hipEventDestroy(closed_);
         if (side_) (void)hipStreamDestroy(side_);
         (void)hipStreamDestroy(stream);
     }
@@ -749,6 +750,7 @@ int Engine::init(int device, std::string& err) {
     CK(hipStreamCreate(&side_));
     CK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
     CK(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&closed_, hipEventDisableTiming));
     {
         int cus = 0;
         CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -968,6 +970,71 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     const float4* spos = d_spos.as<float4>();
     const float4* snrm = d_snrm.as<float4>();
 
+    // ---- work order: Morton cell of every surfel (coherent wavefronts only)
+    const float diag = scene.diag();
+    const SceneView S = dev_view();
+    CK(d_morton.ensure(4ull * N));
+    CK(d_mval.ensure(4ull * N));
+    CK(d_mkey_s.ensure(4ull * N));
+    CK(d_rorder.ensure(4ull * N));
+    size_t tmp = 0;
+    {
+        const f3 ext = scene.hi - scene.lo;
+        const f3 sc = mk(32.0f / fmx(ext.x, 1e-6f), 32.0f / fmx(ext.y, 1e-6f), 32.0f / fmx(ext.z, 1e-6f));
+        launch("morton", [&] {
+            k_morton<<<blocks_for(N, 256), 256, 0, stream>>>(spos, N, scene.lo, sc, d_morton.as<uint32_t>(),
+                                                             d_mval.as<uint32_t>());
+        });
+        CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_morton.as<uint32_t>(), d_mkey_s.as<uint32_t>(),
+                                              d_mval.as<uint32_t>(), d_rorder.as<uint32_t>(), (int)N, 0, 15, stream));
+        CK(d_tmp.ensure(tmp));
+        CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_morton.as<uint32_t>(), d_mkey_s.as<uint32_t>(),
+                                              d_mval.as<uint32_t>(), d_rorder.as<uint32_t>(), (int)N, 0, 15, stream));
+    }
+    CK(d_queue.ensure(16));
+    CK(hipMemsetAsync(d_queue.p, 0, 16, stream));
+    uint32_t* queue = d_queue.as<uint32_t>();
+    const uint32_t persist = (uint32_t)n_cu_ * 8u;
+
+    // ---- closure rays and direct-light terms: they need only the surfels,
+    // the scene and the lights, so they run on the side stream while this
+    // stream builds clusters and candidates (whose host reads of counts
+    // would otherwise leave the device idle) and traces the links.  The
+    // link calibration waits for the closure; solve() for the terms.
+    CK(d_sky.ensure(4ull * N));
+    CK(d_open.ensure(4ull * N));
+    CK(hipMemsetAsync(d_open.p, 0, 4ull * N, stream));
+    const bool side = overlap_direct && !count_ops_;
+    const bool pre = side && n_lights(dev_lights()) > 0;
+    if (pre) {
+        CK(d_dctr.ensure(64));
+        CK(hipMemsetAsync(d_dctr.p, 0, 8, stream));
+    }
+    if (side) {
+        CK(hipEventRecord(fork_, stream));
+        CK(hipStreamWaitEvent(side_, fork_, 0));
+    }
+    {
+        hipStream_t cs = side ? side_ : stream;
+        const Lds ld = plan_lds(false, false);
+        const LightsView Lg{};
+        const uint32_t grid = std::max(1u, std::min<uint32_t>(persist, blocks_for(16ull * N, kBlock)));
+        launch_on(cs, "closure", [&] {
+            hipStream_t stream = cs;  // the launch macro's stream
+            LAUNCH_CL(k_closure, count_ops_, ld.staged, grid, ld.bytes, S, Lg, ld.plan, spos, snrm, N,
+                      d_rorder.as<uint32_t>(), d_dirs.as<float4>(), diag * 2.0f, d_open.as<uint32_t>(), queue + 1,
+                      wave_chunk(16ull * N, grid), ctr, count_ops_ ? ctr + 2 : nullptr);
+        });
+    }
+    if (side) {
+        CK(hipEventRecord(closed_, side_));
+        const int rc = pre ? launch_direct(side_, d_dctr.as<unsigned long long>(), nullptr, err) : 0;
+        CK(hipEventRecord(join_, side_));
+        side_pending_ = true;
+        direct_pre_ = pre && rc == 0;
+        if (rc) return rc;
+    }
+
     // ---- clusters: key, stable sort, segments, serial sums, compaction
     const f3 lo = scene.lo, ext = scene.hi - scene.lo;
     const f3 inv = mk((float)res / fmx(ext.x, 1e-6f), (float)res / fmx(ext.y, 1e-6f), (float)res / fmx(ext.z, 1e-6f));
@@ -983,7 +1050,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         k_cluster_keys<<<blocks_for(N, 256), 256, 0, stream>>>(spos, snrm, N, lo, inv, res, d_ckey.as<uint64_t>(),
                                                                d_cval.as<uint32_t>());
     });
-    size_t tmp = 0;
+    tmp = 0;
     CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_ckey.as<uint64_t>(), d_ckey2.as<uint64_t>(),
                                           d_cval.as<uint32_t>(), d_cval2.as<uint32_t>(), (int)N, 0, (int)key_bits,
                                           stream));
@@ -1040,58 +1107,6 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     const uint32_t C = tail[0] + tail[1];
     n_clusters = C;
     out.clusters = C;
-
-    // ---- work order: Morton cell of every surfel (coherent wavefronts only)
-    const float diag = scene.diag();
-    const SceneView S = dev_view();
-    CK(d_morton.ensure(4ull * N));
-    CK(d_mval.ensure(4ull * N));
-    CK(d_mkey_s.ensure(4ull * N));
-    CK(d_rorder.ensure(4ull * N));
-    {
-        const f3 sc = mk(32.0f / fmx(ext.x, 1e-6f), 32.0f / fmx(ext.y, 1e-6f), 32.0f / fmx(ext.z, 1e-6f));
-        launch("morton", [&] {
-            k_morton<<<blocks_for(N, 256), 256, 0, stream>>>(spos, N, lo, sc, d_morton.as<uint32_t>(),
-                                                             d_mval.as<uint32_t>());
-        });
-        tmp = 0;
-        CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_morton.as<uint32_t>(), d_mkey_s.as<uint32_t>(),
-                                              d_mval.as<uint32_t>(), d_rorder.as<uint32_t>(), (int)N, 0, 15, stream));
-        CK(d_tmp.ensure(tmp));
-        CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_morton.as<uint32_t>(), d_mkey_s.as<uint32_t>(),
-                                              d_mval.as<uint32_t>(), d_rorder.as<uint32_t>(), (int)N, 0, 15, stream));
-    }
-    // Direct-light terms depend only on the surfels, scene and lights: launch
-    // them now on the side stream, beside the tracing kernels; solve() joins.
-    if (overlap_direct && !count_ops_ && n_lights(dev_lights()) > 0) {
-        CK(d_dctr.ensure(64));
-        CK(hipMemsetAsync(d_dctr.p, 0, 8, stream));
-        CK(hipEventRecord(fork_, stream));
-        CK(hipStreamWaitEvent(side_, fork_, 0));
-        if (int rc = launch_direct(side_, d_dctr.as<unsigned long long>(), nullptr, err)) return rc;
-        CK(hipEventRecord(join_, side_));
-        side_pending_ = true;
-        direct_pre_ = true;
-    }
-    CK(d_queue.ensure(16));
-    CK(hipMemsetAsync(d_queue.p, 0, 16, stream));
-    uint32_t* queue = d_queue.as<uint32_t>();
-    const uint32_t persist = (uint32_t)n_cu_ * 8u;
-
-    // ---- closure: independent of the links, launched first
-    CK(d_sky.ensure(4ull * N));
-    CK(d_open.ensure(4ull * N));
-    CK(hipMemsetAsync(d_open.p, 0, 4ull * N, stream));
-    {
-        const Lds ld = plan_lds(false, false);
-        const LightsView Lg{};
-        const uint32_t grid = std::max(1u, std::min<uint32_t>(persist, blocks_for(16ull * N, kBlock)));
-        launch("closure", [&] {
-            LAUNCH_CL(k_closure, count_ops_, ld.staged, grid, ld.bytes, S, Lg, ld.plan, spos, snrm, N,
-                      d_rorder.as<uint32_t>(), d_dirs.as<float4>(), diag * 2.0f, d_open.as<uint32_t>(), queue + 1,
-                      wave_chunk(16ull * N, grid), ctr, count_ops_ ? ctr + 2 : nullptr);
-        });
-    }
 
     // ---- candidates: count, scan, emit
     if ((uint64_t)N * C >= (1ull << 32)) {
@@ -1171,6 +1186,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     CK(d_scale.ensure(4ull * N));
     CK(d_rowsum.ensure(4ull * N));
     CK(hipMemsetAsync(d_lcount.as<uint32_t>() + N, 0, 4, stream));
+    if (side) CK(hipStreamWaitEvent(stream, closed_, 0));  // the closure's open-ray counts
     launch("row_sum", [&] {
         k_row_sum<<<blocks_for(64ull * N, 256), 256, 0, stream>>>(N, d_cstart.as<uint32_t>(), d_cand_v.as<float>(),
                                                           d_open.as<uint32_t>(), d_sky.as<float>(),
