@@ -33,6 +33,16 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// Pinned host staging for read-backs of images: a copy into pageable memory
+// goes through the runtime's own bounce buffers piece by piece.
+struct PinBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+    ~PinBuf() { release(); }
+};
+
 struct TransferOut {
     uint32_t clusters = 0;
     uint64_t candidates = 0, links = 0, evals = 0;
@@ -175,6 +185,10 @@ class Engine {
     int adjoint_solve(BackwardOut& out, std::string& err);
     int adjoint_params(bool pixels, uint64_t npx, float* d_albedo, float* d_emit, std::string& err);
     int check_backward(std::string& err) const;
+    // image read-back: enqueue into h_img_, then copy out once the stream is synchronised
+    int stage_images(uint64_t npx, bool rgb8, bool film, std::string& err);
+    void finish_images(uint64_t npx, uint8_t* rgb8, float* film) const;
+    PinBuf h_img_;
     int copy3_d2h(float* host, const void* dev, uint64_t n, std::string& err);
     int copy3_h2d(void* dev, const float* host, uint64_t n, std::string& err);
     void upload_materials();

@@ -44,6 +44,27 @@ void DevBuf::release() {
     cap = 0;
 }
 
+hipError_t PinBuf::ensure(size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (bytes <= cap) return hipSuccess;
+    release();
+    hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        p = nullptr;
+        cap = 0;
+        (void)hipGetLastError();
+        return e;
+    }
+    cap = bytes;
+    return hipSuccess;
+}
+
+void PinBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
 // ====================================================================== probes
 template <bool LDS>
 __global__ __launch_bounds__(kBlock) void k_probe_dist(SceneView Sg, LightsView Lg, LdsPlan P, const float* pts,
@@ -855,6 +876,25 @@ int Engine::copy3_d2h(float* host, const void* dev, uint64_t n, std::string& err
     return 0;
 }
 
+int Engine::stage_images(uint64_t npx, bool rgb8, bool film, std::string& err) {
+    const size_t rgb_bytes = rgb8 ? (3ull * npx + 255) & ~255ull : 0;
+    if (!rgb8 && !film) return 0;
+    CK(h_img_.ensure(rgb_bytes + (film ? 12ull * npx : 0)));
+    if (rgb8) CK(hipMemcpyAsync(h_img_.p, d_rgb8.p, 3ull * npx, hipMemcpyDeviceToHost, stream));
+    if (film) {
+        CK(d_stage.ensure(12ull * npx));
+        k_pack3<<<blocks_for(npx, 256), 256, 0, stream>>>(d_film.as<float4>(), npx, d_stage.as<float>());
+        CK(hipMemcpyAsync((char*)h_img_.p + rgb_bytes, d_stage.p, 12ull * npx, hipMemcpyDeviceToHost, stream));
+    }
+    return 0;
+}
+
+void Engine::finish_images(uint64_t npx, uint8_t* rgb8, float* film) const {
+    const size_t rgb_bytes = rgb8 ? (3ull * npx + 255) & ~255ull : 0;
+    if (rgb8) memcpy(rgb8, h_img_.p, 3ull * npx);
+    if (film) memcpy(film, (const char*)h_img_.p + rgb_bytes, 12ull * npx);
+}
+
 int Engine::copy3_h2d(void* dev, const float* host, uint64_t n, std::string& err) {
     if (!n || !host) return 0;
     CK(d_stage.ensure(12ull * n));
@@ -1512,14 +1552,12 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
                                                                scene.cfg.exposure, tonemapped ? 1 : 0,
                                                                d_srgb.as<float>(), d_rgb8.as<uint8_t>());
         });
-        CK(hipMemcpyAsync(rgb8, d_rgb8.p, 3ull * npx, hipMemcpyDeviceToHost, stream));
     }
-    if (film) {
-        if (int rc = copy3_d2h(film, d_film.p, npx, err)) return rc;
-    }
+    if (int rc = stage_images(npx, rgb8 != nullptr, film != nullptr, err)) return rc;
     unsigned long long cv[8] = {0};
     CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
+    finish_images(npx, rgb8, film);
     drain_events();
     if (count_ops_) kops_["shade"] += cv[4];
     evals = cv[0];
@@ -1581,13 +1619,12 @@ int Engine::shade_views(const std::vector<HCamera>& cams, uint32_t W, uint32_t H
                                                                scene.cfg.exposure, mode_tonemapped(mode) ? 1 : 0,
                                                                d_srgb.as<float>(), d_rgb8.as<uint8_t>());
         });
-        CK(hipMemcpyAsync(rgb8, d_rgb8.p, 3ull * npx, hipMemcpyDeviceToHost, stream));
     }
-    if (film)
-        if (int rc = copy3_d2h(film, d_film.p, npx, err)) return rc;
+    if (int rc = stage_images(npx, rgb8 != nullptr, film != nullptr, err)) return rc;
     unsigned long long cv[8] = {0};
     CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
+    finish_images(npx, rgb8, film);
     drain_events();
     if (count_ops_) kops_["shade"] += cv[4];
     evals = cv[0];
