@@ -160,6 +160,8 @@ ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
  *                   1 = one lane per row on the CSR, 2 = one wavefront per row,
  *                   3 = sliced rows (one lane per row, links stored per 64-row slice column by column)
  *   "surfels_gpu"   1 = place surfels on the device (default), 0 = on the host
+ *   "side_stream"   1 = closure rays and direct-light terms on a second stream beside the
+ *                   link visibility (default), 0 = everything on one stream
  *   "spp"           paths per pixel of IFX_MODE_REFERENCE (--spp, default 512) */
 ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value);
 ifx_status ifx_kernel_ops(ifx_ctx* ctx, const char* name, uint64_t* ops);

@@ -245,16 +245,16 @@ def _frame_bytes(ctx):
 @pytest.mark.parametrize("name", ["product.rad", "stress.rad"])
 def test_launch_config_invariance(name):
     """Thread-count invariance (ci.yml:81-93) restated for launch shapes: LDS
-    staging on/off, visibility / cull / Jacobi grid sizes, and the
-    instrumented counting kernels all give the same bytes."""
+    staging on/off, visibility / cull / Jacobi grid sizes, one stream or
+    two, and the instrumented counting kernels all give the same bytes."""
     say("launch configs", name)
     src = scene_text(name)
     ctx = ia.Context(0)
     ctx.load_scene(src)
     ctx.generate_surfels(4000)
     base = _frame_bytes(ctx)
-    for key, val in [("lds", 0), ("vis_blocks", 7), ("ff_blocks", 3), ("jacobi_blocks", 5), ("jacobi_rows", 1),
-                     ("jacobi_rows", 2), ("jacobi_rows", 3)]:
+    for key, val in [("side_stream", 0), ("lds", 0), ("vis_blocks", 7), ("ff_blocks", 3), ("jacobi_blocks", 5),
+                     ("jacobi_rows", 1), ("jacobi_rows", 2), ("jacobi_rows", 3), ("side_stream", 1)]:
         ctx.set_tuning(key, val)
         got = _frame_bytes(ctx)
         assert all(np.array_equal(a, b) for a, b in zip(base, got)), key
