@@ -40,6 +40,17 @@ ALL_KERNELS = ["cluster_keys", "seg_flags", "seg_begin", "cluster_build", "clust
 PEAK_VALU_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per dispatch of the product frame's kernels, read from two
+# rocprofv3 --pmc runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, scaled against a
+# plain 256 MiB device copy).  The figures and the commands that made them
+# are in profiles/product_hbm_traffic_pmc.md; they hold for the product
+# workload only, so other workloads report null.
+PRODUCT_TRAFFIC = {
+    "visibility": {"bytes": 2.484e8, "profile": "profiles/product_hbm_traffic_pmc.md"},
+    "closure": {"bytes": 1.379e7, "profile": "profiles/product_hbm_traffic_pmc.md"},
+    "shade": {"bytes": 1.284e7, "profile": "profiles/product_hbm_traffic_pmc.md"},
+}
+
 
 def _cfg(name):
     from interflect_amd import scenes
@@ -211,8 +222,11 @@ def measure(name, steps, warmup, with_cpu):
     roof = None
     if dominant in TRACE_KERNELS and ops.get(dominant):
         tfl = ops[dominant] / (ktime[dominant]["avg_ms"] * 1e-3) / 1e12
+        measured = PRODUCT_TRAFFIC.get(dominant) if name == "product" else None
         roof = {"kernel": dominant, "bound": "valu", "achieved": tfl, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
-                "frac": tfl / PEAK_VALU_TFLOPS, "traffic": None, "ops_per_launch": ops[dominant]}
+                "frac": tfl / PEAK_VALU_TFLOPS, "traffic": measured["bytes"] if measured else None,
+                "traffic_unit": "bytes per dispatch (rocprofv3 PMC)",
+                "traffic_profile": measured["profile"] if measured else None, "ops_per_launch": ops[dominant]}
     elif dominant == "jacobi":
         roof = jac_roof
     trace_roofs = {}
