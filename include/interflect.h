@@ -111,6 +111,19 @@ ifx_status ifx_get_clusters(ifx_ctx* ctx, float* p, float* n, float* area, float
 /* solve::light_surfels + solve::solve (src/solve.rs:27, :47). */
 ifx_status ifx_solve(ifx_ctx* ctx, uint32_t max_bounces, ifx_solve_stats* out);
 ifx_status ifx_get_radiosity(ifx_ctx* ctx, float* direct, float* irr, float* rad);
+/* No counterpart in the reference.  ifx_solve for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc): it allocates nothing.  Outputs go to caller-owned DEVICE
+ * arrays d_direct, d_irr, d_rad of N float4 records each (xyz + one unused
+ * float per surfel), equal bit for bit to ifx_solve's; every scratch buffer
+ * (direct-light terms, cluster radiosity, residuals, counters) lives in one
+ * static 32 MiB device array per GPU, so N x lights x 20 B + clusters x 16 B
+ * + 16 KiB must fit in it (IFX_E_LIMIT otherwise).  Needs a transfer in the
+ * context; the context's own radiosity (ifx_get_radiosity, ifx_shade) is left
+ * as the last ifx_solve made it.  Calls that overlap on one GPU, from any
+ * context, race on the shared array. */
+ifx_status ifx_solve_noalloc(ifx_ctx* ctx, uint32_t max_bounces, float* d_direct, float* d_irr, float* d_rad,
+                             ifx_solve_stats* out);
 /* Replace the solved radiosity (e.g. to shade from an externally solved cache). */
 ifx_status ifx_set_radiosity(ifx_ctx* ctx, const float* direct, const float* irr, const float* rad);
 

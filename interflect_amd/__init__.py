@@ -41,7 +41,7 @@ EXPORTS = [
     "ifx_batch_create", "ifx_batch_destroy", "ifx_batch_last_error", "ifx_batch_add_scene",
     "ifx_batch_generate_surfels", "ifx_batch_set_graph", "ifx_batch_set_tuning", "ifx_batch_autotune",
     "ifx_batch_run", "ifx_batch_get_stats", "ifx_batch_get_image", "ifx_batch_get_radiosity",
-    "ifx_batch_get_transfer", "ifx_shade_views", "ifx_turntable_cameras",
+    "ifx_batch_get_transfer", "ifx_shade_views", "ifx_turntable_cameras", "ifx_solve_noalloc",
 ]
 
 
@@ -115,6 +115,7 @@ def lib():
         "ifx_get_transfer": [vp, P, P, P, P, P],
         "ifx_get_clusters": [vp, P, P, P, P, P, P],
         "ifx_solve": [vp, u32, C.POINTER(SolveStats)],
+        "ifx_solve_noalloc": [vp, u32, P, P, P, C.POINTER(SolveStats)],
         "ifx_get_radiosity": [vp, P, P, P],
         "ifx_set_radiosity": [vp, P, P, P],
         "ifx_shade": [vp, C.POINTER(Camera), u32, u32, i32, P, P, C.POINTER(ShadeStats)],
@@ -287,6 +288,16 @@ class Context:
         st = SolveStats()
         b = int(max_bounces or self.info()["bounces"])
         self._check(lib().ifx_solve(self._h, b, C.byref(st)))
+        return {"iterations": int(st.iterations), "residual": float(st.residual), "evals": int(st.evals),
+                "ms": float(st.ms)}
+
+    def solve_noalloc(self, d_direct, d_irr, d_rad, max_bounces=None):
+        """ifx_solve_noalloc: outputs into caller-owned device arrays (integer
+        device addresses of N x 4 float32 each); allocates nothing."""
+        st = SolveStats()
+        b = int(max_bounces or self.info()["bounces"])
+        self._check(lib().ifx_solve_noalloc(self._h, b, C.c_void_p(d_direct), C.c_void_p(d_irr), C.c_void_p(d_rad),
+                                            C.byref(st)))
         return {"iterations": int(st.iterations), "residual": float(st.residual), "evals": int(st.evals),
                 "ms": float(st.ms)}
 

@@ -272,6 +272,25 @@ ifx_status ifx_solve(ifx_ctx* ctx, uint32_t max_bounces, ifx_solve_stats* o) {
     });
 }
 
+ifx_status ifx_solve_noalloc(ifx_ctx* ctx, uint32_t max_bounces, float* d_direct, float* d_irr, float* d_rad,
+                             ifx_solve_stats* o) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        auto t0 = std::chrono::steady_clock::now();
+        ifx::SolveOut r;
+        int rc = ctx->eng.solve_noalloc(max_bounces, (float4*)d_direct, (float4*)d_irr, (float4*)d_rad, r, ctx->err);
+        if (rc) return rc;
+        if (o) {
+            std::memset(o, 0, sizeof(*o));
+            o->iterations = r.iterations;
+            o->residual = r.residual;
+            o->evals = r.evals;
+            o->ms = (float)ms_since(t0);
+        }
+        return IFX_OK;
+    });
+}
+
 ifx_status ifx_get_radiosity(ifx_ctx* ctx, float* direct, float* irr, float* rad) {
     return guarded(ctx, [&]() -> ifx_status { return ctx->eng.download_radiosity(direct, irr, rad, ctx->err); });
 }

@@ -102,6 +102,12 @@ class Engine {
     uint32_t place_rounds() const { return last_place_rounds_; }
     int transfer(int32_t res, TransferOut& out, std::string& err);
     int solve(uint32_t bounces, SolveOut& out, std::string& err);
+    // solve() into caller-owned device arrays (N float4 records each) with
+    // every scratch buffer (direct-light terms, cluster radiosity, residuals,
+    // counters) in the library's static device array: nothing is allocated.
+    // Calls that overlap on one device race on that array.
+    int solve_noalloc(uint32_t bounces, float4* direct, float4* irr, float4* rad, SolveOut& out, std::string& err);
+    static constexpr size_t kScratchBytes = 32ull << 20;  // the static array, per device
     // Many cameras over one solve, one launch (film / rgb8: views x h x w x 3).
     int shade_views(const std::vector<HCamera>& cams, uint32_t w, uint32_t h, int32_t mode, float* film,
                     uint8_t* rgb8, uint64_t& evals, std::string& err);
@@ -213,6 +219,7 @@ class Engine {
     bool side_pending_ = false;  // side_ holds work the main stream has not waited for
     DevBuf d_dctr;
 
+    unsigned char* scratch_ = nullptr;  // device address of the static scratch array on device_
     int device_ = 0;
     int n_cu_ = 1;
     bool has_device_ = false;

@@ -17,12 +17,41 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "ifx_kern.h"
 
 namespace ifx {
 
 // ====================================================================== memory
+// Scratch of the allocation-free entry point (solve_noalloc): one array per
+// device, held until the process ends.  Host code reaches it only through
+// the device address hipGetSymbolAddress returns on that device (the
+// array's name is a host-side stand-in); scratch_device() looks that address
+// up once per device, at context creation.
+static __device__ __align__(256) unsigned char scratch[Engine::kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_at;
+
+// The scratch array's address on `device`, which must be current.
+hipError_t scratch_device(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_at.find(device);
+    if (it != g_scratch_at.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_at[device] = (unsigned char*)p;
+    *out = (unsigned char*)p;
+    return hipSuccess;
+}
+}  // namespace
+
 hipError_t DevBuf::ensure(size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (bytes <= cap) return hipSuccess;
@@ -682,6 +711,12 @@ __global__ void k_encode(const float4* __restrict__ film, uint32_t npx, float ex
     encode_px(film[i], exposure, tonemap, thr, out + 3ull * i);
 }
 
+// Clears n words (a kernel, not a memset, so the clear can sit in a captured graph).
+__global__ void k_zero_words(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 // Packed xyz triples <-> float4 records, so host copies stay contiguous.
 __global__ void k_pack3(const float4* __restrict__ src, uint64_t n, float* dst) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -807,6 +842,7 @@ int Engine::init(int device, std::string& err) {
     CK(d_srgb.ensure(sizeof(thr)));
     CK(hipMemcpy(d_srgb.p, thr, sizeof(thr), hipMemcpyHostToDevice));
     CK(d_counter.ensure(64));
+    CK(scratch_device(device, &scratch_));
     has_device_ = true;
     return 0;
 }
@@ -1404,6 +1440,132 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
         out.residual = bitsf(res[out.iterations - 1]);
     }
     have_solve = have_gterm_ = true;
+    return 0;
+}
+
+// solve() with caller-owned outputs and every scratch buffer in the static
+// scratch array (no allocation): the same kernels in the same order, so the
+// results equal solve()'s bit for bit.  Layout of the array (256-byte
+// aligned sections): evaluation counter, per-bounce residuals, cluster
+// radiosity, direct-light terms, light factors.  Counter and residuals are
+// cleared by a kernel at every call; every other section is written before
+// it is read.
+int Engine::solve_noalloc(uint32_t bounces, float4* direct, float4* irr, float4* rad, SolveOut& out,
+                          std::string& err) {
+    if (!have_transfer) {
+        err = "solve called before transfer";
+        return -3;
+    }
+    if (!scratch_) {
+        err = "no scratch array on this device";
+        return -3;
+    }
+    if (bounces == 0 || bounces > 4096) {
+        err = "bounces must be in 1..4096";
+        return -1;
+    }
+    const uint32_t N = (uint32_t)surfels.size();
+    const LightsView Lv = dev_lights();
+    const uint32_t nl = n_lights(Lv);
+    const uint32_t C = n_clusters;
+    auto al = [](uint64_t b) { return (b + 255ull) & ~255ull; };
+    const uint64_t o_res = 256, o_bc = o_res + al(4ull * (bounces + 1)), o_terms = o_bc + al(16ull * (C + 1)),
+                   o_g = o_terms + al(16ull * N * nl), total = o_g + al(4ull * N * nl);
+    if (total > kScratchBytes) {
+        err = "solve_noalloc needs " + std::to_string(total) + " bytes of scratch (" + std::to_string(N) +
+              " surfels x " + std::to_string(nl) + " lights, " + std::to_string(C) +
+              " clusters); the static scratch array holds " + std::to_string(kScratchBytes);
+        return -6;
+    }
+    if (N && (!direct || !irr || !rad)) {
+        err = "null output array";
+        return -1;
+    }
+    CK(hipSetDevice(device_));
+    out = SolveOut();
+    join_side();  // the side stream's closure / terms of the last transfer
+    unsigned long long* ctr = (unsigned long long*)scratch_;
+    uint32_t* resid = (uint32_t*)(scratch_ + o_res);
+    float4* bc = (float4*)(scratch_ + o_bc);
+    float4* terms = (float4*)(scratch_ + o_terms);
+    float* gterm = (float*)(scratch_ + o_g);
+    const uint32_t clear_words = (uint32_t)(o_bc / 4);
+    k_zero_words<<<blocks_for(clear_words, 256), 256, 0, stream>>>((uint32_t*)scratch_, clear_words);
+    const SceneView S = dev_view();
+    if (N) {
+        if (nl) {
+            const Lds ld = plan_lds(true);
+            LAUNCH_CL(k_direct_terms, false, ld.staged, blocks_for((uint64_t)N * nl, kBlock), ld.bytes, S, Lv,
+                      ld.plan, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, nl,
+                      d_rorder.as<uint32_t>(), terms, gterm, ctr, nullptr);
+        }
+        k_direct_sum<<<blocks_for(N, 256), 256, 0, stream>>>(N, nl, terms, direct, rad, irr);
+    }
+    uint32_t res[4097];
+    if (C > 0 && N > 0) {
+        JacobiArgs A;
+        A.N = N;
+        A.C = C;
+        A.tol = solve_tol;
+        A.wave_rows = jacobi_rows == 2 ? 1u : 0u;
+        A.sbase = nullptr;
+        A.sidx = nullptr;
+        A.sw = nullptr;
+        // the sliced copy only when this transfer's is already built (building it allocates)
+        const bool sliced = jacobi_rows == 3 || (jacobi_rows == 0 && n_links >= 64ull * N);
+        if (sliced && sliced_gen_ == transfer_gen_ && !A.wave_rows) {
+            A.sbase = d_sbase.as<uint32_t>();
+            A.sidx = d_sidx.as<uint32_t>();
+            A.sw = d_sw.as<float>();
+        }
+        A.range = d_crange.as<uint2>();
+        A.members = d_cval2.as<uint32_t>();
+        A.spos = d_spos.as<float4>();
+        A.cpos = d_cpos.as<float4>();
+        A.lstart = d_lstart.as<uint32_t>();
+        A.idx = d_idx.as<uint32_t>();
+        A.w = d_w.as<float>();
+        A.direct = direct;
+        A.salb = d_salb.as<float4>();
+        A.rad = rad;
+        A.irr = irr;
+        A.bc = bc;
+        A.resid = resid;
+        const uint32_t g_agg = jacobi_blocks ? jacobi_blocks : blocks_for(64ull * C, kBlock);
+        const uint32_t g_rows =
+            jacobi_blocks ? jacobi_blocks : blocks_for(A.wave_rows ? 64ull * N : (uint64_t)N, kBlock);
+        const uint32_t group = 8;
+        bool done = false;
+        for (uint32_t it0 = 0; it0 < bounces && !done; it0 += group) {
+            const uint32_t it1 = std::min(bounces, it0 + group);
+            for (uint32_t it = it0; it < it1; ++it) {
+                k_jacobi_agg<<<g_agg, kBlock, 0, stream>>>(A, it);
+                k_jacobi_rows<<<g_rows, kBlock, 0, stream>>>(A, it);
+            }
+            if (it1 < bounces) {
+                uint32_t last = 0;
+                CK(hipMemcpyAsync(&last, resid + it1 - 1, 4, hipMemcpyDeviceToHost, stream));
+                CK(hipStreamSynchronize(stream));
+                done = bitsf(last) < solve_tol;
+            }
+        }
+        CK(hipMemcpyAsync(res, resid, 4ull * bounces, hipMemcpyDeviceToHost, stream));
+    }
+    CK(hipGetLastError());
+    unsigned long long ev = 0;
+    CK(hipMemcpyAsync(&ev, ctr, 8, hipMemcpyDeviceToHost, stream));
+    CK(hipStreamSynchronize(stream));
+    out.evals = ev;
+    if (C > 0 && N > 0) {
+        out.iterations = bounces;
+        for (uint32_t it = 0; it < bounces; ++it) {
+            if (bitsf(res[it]) < solve_tol) {
+                out.iterations = it + 1;
+                break;
+            }
+        }
+        out.residual = bitsf(res[out.iterations - 1]);
+    }
     return 0;
 }
 
