@@ -37,8 +37,21 @@ def hipcc():
     return "hipcc"
 
 
-def _compile(src):
+def _stale(src, obj):
+    """An object is stale when it is missing or older than its source or
+    any header of the source tree."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    deps.append(os.path.join(ROOT, "include", "interflect.h"))
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(OUT, src + ".o")
+    if incremental and not _stale(src, obj):
+        return obj
     cmd = [hipcc()] + COMMON
     if src.endswith(".hip"):
         cmd += ["-x", "hip"] + DEVICE
@@ -51,10 +64,12 @@ def _compile(src):
     return obj
 
 
-def build_lib(verbose=True):
+def build_lib(verbose=True, incremental=False):
+    """Compile every source (incremental=True: only those whose object is
+    stale) and link the library."""
     os.makedirs(OUT, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=4) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+        objs = list(ex.map(lambda s: _compile(s, incremental), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -75,7 +90,7 @@ def build_oracle(verbose=True):
 
 
 def main():
-    build_lib()
+    build_lib(incremental="--incremental" in sys.argv)
     if "--no-oracle" not in sys.argv:
         build_oracle()
 
