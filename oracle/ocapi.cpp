@@ -355,6 +355,41 @@ float orc_quad_ff(const float* verts, const float* p, const float* n) {
 }
 
 float orc_sphere_ff(float cos_t, float s2) { return sphere_form_factor(cos_t, s2); }
+
+// BSDF helpers and direct light (src/shade.rs:8-125), emitters (src/shade.rs:168-181,
+// src/trace.rs:128-148)
+void orc_fresnel(float cos_theta, const float* f0, float* out) { st(out, fresnel_schlick(cos_theta, ld(f0))); }
+float orc_d_ggx(float n_dot_h, float alpha) { return d_ggx(n_dot_h, alpha); }
+float orc_v_smith(float n_dot_v, float n_dot_l, float alpha) { return v_smith(n_dot_v, n_dot_l, alpha); }
+void orc_f0_of(const float* albedo, float metallic, float* out) {
+    Material m;
+    m.albedo = ld(albedo);
+    m.metallic = metallic;
+    st(out, f0_of(m));
+}
+void orc_direct(void* h, const float* p, const float* n, const float* v, const float* albedo, float roughness,
+                float metallic, float* out, uint32_t* steps) {
+    Material m;
+    m.albedo = ld(albedo);
+    m.roughness = roughness;
+    m.metallic = metallic;
+    uint32_t s = 0;
+    st(out, direct(((Ctx*)h)->sc, ld(p), ld(n), ld(v), m, s));
+    *steps = s;
+}
+int orc_emitter_hit(void* h, const float* ro, const float* rd, float tmax, float* emit) {
+    V3 e = splat(0.0f);
+    const bool hit = emitter_hit(((Ctx*)h)->sc, ld(ro), ld(rd), tmax, e);
+    st(emit, e);
+    return hit ? 1 : 0;
+}
+int orc_intersect_quad(const float* verts, const float* ro, const float* rd, float* t) {
+    V3 vv[4] = {ld(verts), ld(verts + 3), ld(verts + 6), ld(verts + 9)};
+    float tt = 0.0f;
+    const bool hit = intersect_quad(vv, ld(ro), ld(rd), tt);
+    *t = tt;
+    return hit ? 1 : 0;
+}
 uint32_t orc_srgb8(float c) { return srgb8(c); }
 void orc_aces(const float* in, float* out) { st(out, aces(ld(in))); }
 float orc_radical_inverse(uint32_t i, uint32_t base) { return radical_inverse(i, base); }

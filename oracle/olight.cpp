@@ -185,13 +185,13 @@ V3 fresnel_schlick(float cos_theta, V3 f0) {
     return f0 + (splat(1.0f) - f0) * (m2 * m2 * m);
 }
 
-static float d_ggx(float n_dot_h, float alpha) {
+float d_ggx(float n_dot_h, float alpha) {
     float a2 = alpha * alpha;
     float d = n_dot_h * n_dot_h * (a2 - 1.0f) + 1.0f;
     return a2 / fmax_(PI_F * d * d, 1e-9f);
 }
 
-static float v_smith(float n_dot_v, float n_dot_l, float alpha) {
+float v_smith(float n_dot_v, float n_dot_l, float alpha) {
     float a2 = alpha * alpha;
     float gv = n_dot_l * std::sqrt(n_dot_v * n_dot_v * (1.0f - a2) + a2);
     float gl = n_dot_v * std::sqrt(n_dot_l * n_dot_l * (1.0f - a2) + a2);

@@ -35,6 +35,9 @@ V3 closest_on_quad(const V3 verts[4], V3 q);
 
 // src/shade.rs
 V3 fresnel_schlick(float cos_theta, V3 f0);
+// src/shade.rs:17-31
+float d_ggx(float n_dot_h, float alpha);
+float v_smith(float n_dot_v, float n_dot_l, float alpha);
 V3 f0_of(const Material& m);
 V3 diffuse_albedo(const Material& m);
 V3 direct(const Scene& sc, V3 p, V3 n, V3 v, const Material& m, uint32_t& steps);

@@ -93,6 +93,17 @@ def lib():
     L.orc_radical_inverse.argtypes = [C.c_uint32, C.c_uint32]
     L.orc_frame.argtypes = [f32p, f32p, f32p]
     L.orc_set_spp.argtypes = [vp, C.c_uint32]
+    L.orc_fresnel.argtypes = [C.c_float, f32p, f32p]
+    L.orc_d_ggx.restype = C.c_float
+    L.orc_d_ggx.argtypes = [C.c_float, C.c_float]
+    L.orc_v_smith.restype = C.c_float
+    L.orc_v_smith.argtypes = [C.c_float, C.c_float, C.c_float]
+    L.orc_f0_of.argtypes = [f32p, C.c_float, f32p]
+    L.orc_direct.argtypes = [vp, f32p, f32p, f32p, f32p, C.c_float, C.c_float, f32p, u32p]
+    L.orc_emitter_hit.restype = C.c_int
+    L.orc_emitter_hit.argtypes = [vp, f32p, f32p, C.c_float, f32p]
+    L.orc_intersect_quad.restype = C.c_int
+    L.orc_intersect_quad.argtypes = [f32p, f32p, f32p, f32p]
     L.orc_rng.argtypes = [C.c_uint64, C.c_uint32, f32p]
     _LIB = L
     return L
@@ -290,6 +301,18 @@ class Scene:
                               1 if use_gi else 0, self.threads, y0, y1, film)
         return film.reshape(h, w, 3), int(ev)
 
+    def direct(self, p, n, v, albedo, roughness, metallic):
+        """shade::direct (src/shade.rs:50-125) at one point: (radiance, shadow steps)."""
+        out = np.zeros(3, np.float32)
+        st = np.zeros(1, np.uint32)
+        lib().orc_direct(self.h, _f(p), _f(n), _f(v), _f(albedo), roughness, metallic, out, st)
+        return out, int(st[0])
+
+    def emitter_hit(self, ro, rd, tmax):
+        """shade::emitter_hit (src/shade.rs:168-181): emission of the nearest front-facing emitter, or None."""
+        e = np.zeros(3, np.float32)
+        return e if lib().orc_emitter_hit(self.h, _f(ro), _f(rd), tmax, e) else None
+
     def gather(self, p, n, radius):
         p = _f(p, (-1, 3))
         out = np.zeros_like(p)
@@ -320,6 +343,31 @@ def prim_bounds(kind, params):
 
 def quad_ff(verts, p, n):
     return float(lib().orc_quad_ff(_f(verts), _f(p), _f(n)))
+
+
+def fresnel(cos_theta, f0):
+    out = np.zeros(3, np.float32)
+    lib().orc_fresnel(cos_theta, _f(f0), out)
+    return out
+
+
+def d_ggx(n_dot_h, alpha):
+    return float(lib().orc_d_ggx(n_dot_h, alpha))
+
+
+def v_smith(n_dot_v, n_dot_l, alpha):
+    return float(lib().orc_v_smith(n_dot_v, n_dot_l, alpha))
+
+
+def f0_of(albedo, metallic):
+    out = np.zeros(3, np.float32)
+    lib().orc_f0_of(_f(albedo), metallic, out)
+    return out
+
+
+def intersect_quad(verts, ro, rd):
+    t = np.zeros(1, np.float32)
+    return float(t[0]) if lib().orc_intersect_quad(_f(verts), _f(ro), _f(rd), t) else None
 
 
 def sphere_ff(cos_t, s2):

@@ -636,3 +636,145 @@ def test_path_tracer_energy_ratio_gate_on_the_bench_scenes():
         lg = po.encode(g, sc.exposure, True).astype(np.float64) @ np.array([0.2126, 0.7152, 0.0722])
         ratio = la.mean() / lg.mean()
         assert 0.85 < ratio < 1.25, (name, ratio)
+
+
+# --------------------------------------------------------------------------- shade.rs BSDF + emitters
+# Closed forms for the specular lobe of shade::direct (src/shade.rs:8-125) and
+# the emitter test (src/shade.rs:168-181, src/trace.rs:128-148): the published
+# pins count evaluations only, which do not depend on these values.
+def test_fresnel_schlick_closed_forms():
+    """src/shade.rs:8-14: F(1) = f0, F(0) = 1, F(1/2) = f0 + (1 - f0)/32, decreasing in cos."""
+    f0 = np.array([0.04, 0.5, 0.95], np.float32)
+    assert np.array_equal(po.fresnel(1.0, f0), f0)
+    assert np.allclose(po.fresnel(0.0, f0), 1.0, rtol=0, atol=1.2e-7)
+    assert np.allclose(po.fresnel(0.5, f0), f0 + (1 - f0) / 32.0, rtol=1e-6)
+    cs = np.linspace(0, 1, 41)
+    vals = np.array([po.fresnel(c, f0)[0] for c in cs])
+    assert np.all(np.diff(vals) <= 0)
+    # clamp: cos outside [0, 1] behaves as the nearest end
+    assert np.array_equal(po.fresnel(1.5, f0), f0)
+
+
+@pytest.mark.parametrize("alpha", [0.05, 0.3, 1.0])
+def test_ggx_distribution_is_normalised(alpha):
+    """src/shade.rs:16-22: the projected GGX distribution integrates to one over
+    the hemisphere, 2 pi int_0^1 D(mu) mu dmu = pi int_0^1 D(sqrt x) dx = 1
+    (quadrature of the oracle's f32 D on a grid refined towards mu = 1)."""
+    s = np.concatenate([[0.0], np.geomspace(1e-9, 1.0, 6000)])  # s = 1 - x
+    d = np.array([po.d_ggx(math.sqrt(1.0 - v), alpha) for v in s])
+    integral = math.pi * float(np.sum(0.5 * (d[1:] + d[:-1]) * np.diff(s)))
+    assert integral == pytest.approx(1.0, abs=2e-3)
+    # peak and grazing values: D(1) = 1/(pi a^2), D(0) = a^2/pi
+    assert po.d_ggx(1.0, alpha) == pytest.approx(1.0 / (math.pi * alpha * alpha), rel=1e-5)
+    assert po.d_ggx(0.0, alpha) == pytest.approx(alpha * alpha / math.pi, rel=1e-5)
+
+
+def test_smith_visibility_closed_forms():
+    """src/shade.rs:24-31: height-correlated Smith over 4 nl nv.  alpha = 1:
+    0.5 / (nl + nv) exactly; alpha -> 0: 1 / (4 nl nv); symmetric in (nv, nl)."""
+    rng = np.random.default_rng(5)
+    for nv, nl in rng.uniform(0.05, 1.0, (20, 2)).astype(np.float32):
+        assert po.v_smith(nv, nl, 1.0) == float(np.float32(0.5) / (nv + nl))
+        assert po.v_smith(nv, nl, 1e-4) == pytest.approx(1.0 / (4.0 * nv * nl), rel=2e-5)
+        for a in (0.1, 0.5, 0.9):
+            assert po.v_smith(nv, nl, a) == po.v_smith(nl, nv, a)
+    # between the limits, larger roughness lowers the visibility term
+    assert po.v_smith(0.5, 0.7, 0.2) > po.v_smith(0.5, 0.7, 0.8)
+
+
+def test_f0_is_the_lerp_of_dielectric_and_albedo():
+    """src/shade.rs:35-38: f0 = lerp(0.04, albedo, metallic)."""
+    alb = np.array([0.9, 0.5, 0.2], np.float32)
+    assert np.array_equal(po.f0_of(alb, 0.0), np.full(3, 0.04, np.float32))
+    assert np.array_equal(po.f0_of(alb, 1.0), alb)
+    assert np.allclose(po.f0_of(alb, 0.5), (0.04 + alb) / 2, rtol=1e-6)
+
+
+def lit_floor(a, h, emit):
+    return ("render { width: 8, height: 8 }\ncamera { pos: [0, 1, 3], look: [0, 0, 0], fov: 40 }\n"
+            "material \"m\" { albedo: [0.5, 0.5, 0.5], roughness: 1.0 }\n"
+            "plane { normal: [0, 1, 0], height: 0, mat: \"m\" }\n"
+            "light { verts: [[%r, %r, %r], [%r, %r, %r], [%r, %r, %r], [%r, %r, %r]], emit: [%r, %r, %r] }\n"
+            % (-a, h, -a, a, h, -a, a, h, a, -a, h, a, emit, emit, emit))
+
+
+def howell_b3(a, h):
+    """Point under the centre of a parallel square of half-side a at height h."""
+    x = a / h
+    c = (1 / (2 * math.pi)) * 2 * (x / math.sqrt(1 + x * x) * math.atan(x / math.sqrt(1 + x * x)))
+    return 4 * c
+
+
+@pytest.mark.parametrize("theta_deg", [0.0, 20.0, 35.0])
+@pytest.mark.parametrize("rough,metal", [(0.3, 1.0), (1.0, 0.0), (0.5, 0.5), (0.08, 1.0)])
+def test_direct_light_closed_form_on_an_open_floor(theta_deg, rough, metal):
+    """shade::direct (src/shade.rs:50-125) at a floor point under the centre of
+    a downward square light, viewed at angle theta in the xy plane.  Nothing
+    occludes (vis = 1), the mirror ray meets the light plane at
+    (-h tan theta, h, 0) inside the quad, so the representative point is that
+    point, l = reflect(-v) and h = n: D = 1/(pi w^2), F = Schlick at cos theta,
+    V = 0.5 / (2 c sqrt(c^2 (1 - w^2) + w^2)), energy (alpha/w)^2 with
+    w = clamp(alpha + radius cos(theta) / (2h), alpha, 1), plus the diffuse
+    kd L F_B3."""
+    a, h, L = 1.6, 2.0, 7.0  # h tan(35 deg) = 1.40 < a
+    sc = po.Scene(lit_floor(a, h, L))
+    th = math.radians(theta_deg)
+    v = np.array([math.sin(th), math.cos(th), 0.0], np.float32)
+    v /= np.linalg.norm(v)
+    alb = np.array([0.9, 0.5, 0.2])
+    got, steps = sc.direct([0, 0, 0], [0, 1, 0], v, alb, rough, metal)
+    assert steps > 0
+    c = math.cos(th)
+    alpha = max(rough * rough, 1e-3)
+    radius = math.sqrt((2 * a) ** 2 / math.pi)
+    w = min(max(alpha + radius / (2 * h / c), alpha), 1.0)
+    energy = (alpha / w) ** 2
+    D = 1.0 / (math.pi * w * w)
+    V = 0.5 / (2 * c * math.sqrt(c * c * (1 - w * w) + w * w))
+    f0 = 0.04 * (1 - metal) + alb * metal
+    F = f0 + (1 - f0) * (1 - c) ** 5
+    spec = F * (D * V * energy * c) * L
+    diff = alb * (1 - metal) * L * howell_b3(a, h)
+    want = diff + spec
+    assert np.allclose(got, want, rtol=2e-5, atol=1e-6), (got, want)
+
+
+def test_direct_light_skips_an_occluded_light_and_a_back_facing_one():
+    """A slab between floor and light zeroes the term; a light facing away contributes nothing."""
+    src = lit_floor(0.6, 2.0, 7.0) + "box { center: [0, 1, 0], size: [3, 0.1, 3], mat: \"m\" }\n"
+    got, _ = po.Scene(src).direct([0, 0, 0], [0, 1, 0], [0, 1, 0], [0.5, 0.5, 0.5], 0.5, 0.0)
+    assert np.array_equal(got, np.zeros(3, np.float32))
+    up = lit_floor(0.6, 2.0, 7.0).replace("[[-0.6, 2.0, -0.6], [0.6, 2.0, -0.6], [0.6, 2.0, 0.6], [-0.6, 2.0, 0.6]]",
+                                          "[[-0.6, 2.0, 0.6], [0.6, 2.0, 0.6], [0.6, 2.0, -0.6], [-0.6, 2.0, -0.6]]")
+    assert up != lit_floor(0.6, 2.0, 7.0)
+    got, _ = po.Scene(up).direct([0, 0, 0], [0, 1, 0], [0, 1, 0], [0.5, 0.5, 0.5], 0.5, 0.0)
+    assert np.array_equal(got, np.zeros(3, np.float32))
+
+
+def test_intersect_quad_closed_forms():
+    """src/trace.rs:128-148: hit distance inside the quad, misses outside it,
+    parallel rays and hits closer than 1e-4."""
+    q = ceiling(0.5, 2.0)
+    assert po.intersect_quad(q, [0, 0, 0], [0, 1, 0]) == pytest.approx(2.0, abs=1e-6)
+    d = np.array([0.2, 1.0, 0.1])
+    d /= np.linalg.norm(d)
+    assert po.intersect_quad(q, [0, 0, 0], d) == pytest.approx(2.0 / d[1], rel=1e-6)
+    assert po.intersect_quad(q, [0.6, 0, 0], [0, 1, 0]) is None  # outside
+    assert po.intersect_quad(q, [0, 0, 0], [1, 0, 0]) is None    # parallel
+    assert po.intersect_quad(q, [0, 2.0 - 5e-5, 0], [0, 1, 0]) is None  # t < 1e-4
+    assert po.intersect_quad(q, [0, 3, 0], [0, 1, 0]) is None    # behind
+
+
+def test_emitter_hit_front_face_nearest_and_tmax():
+    """src/shade.rs:168-181: only the emitting face is visible, the nearest
+    hit wins, and hits at or beyond tmax are ignored."""
+    src = lit_floor(0.6, 2.0, 7.0) + (
+        "light { verts: [[-0.3, 1.0, -0.3], [0.3, 1.0, -0.3], [0.3, 1.0, 0.3], [-0.3, 1.0, 0.3]], "
+        "emit: [3, 4, 5] }\n")
+    sc = po.Scene(src)
+    e = sc.emitter_hit([0, 0, 0], [0, 1, 0], 100.0)
+    assert e is not None and np.array_equal(e, np.array([3, 4, 5], np.float32))  # the nearer (y = 1)
+    e = sc.emitter_hit([0.45, 0, 0], [0, 1, 0], 100.0)
+    assert e is not None and np.array_equal(e, np.full(3, 7.0, np.float32))  # only the larger covers x = .45
+    assert sc.emitter_hit([0, 0, 0], [0, 1, 0], 0.99) is None  # both beyond tmax
+    assert sc.emitter_hit([0, 5, 0], [0, -1, 0], 100.0) is None  # back faces
