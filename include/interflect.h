@@ -158,10 +158,15 @@ ifx_status ifx_set_profiling(ifx_ctx* ctx, int32_t on);
 /* Kernel `name`'s launch count and summed duration since the last reset. */
 ifx_status ifx_kernel_time(ifx_ctx* ctx, const char* name, uint64_t* launches, double* total_ms);
 ifx_status ifx_reset_kernel_times(ifx_ctx* ctx);
-/* Instrumented mode: the tracing kernels (visibility, closure, direct_e,
- * shade) run a counting instantiation that tallies algorithmic FP32
- * operations; results are identical, timings are not representative.
- * No counterpart in the reference; feeds the roofline report. */
+/* Instrumented modes of the tracing kernels (visibility, closure, direct_e,
+ * shade); results are identical, timings are not representative.  No
+ * counterpart in the reference; feeds the roofline report.
+ *   1: a counting instantiation with the reference's per-lane traversal
+ *      tallies algorithmic FP32 operations (ifx_kernel_ops "name") and
+ *      primitive evaluations ("name.prims");
+ *   2: the timed kernels tally the primitive evaluations they execute, every
+ *      lane of a shared traversal counted ("name.exec_prims");
+ *   0: off. */
 ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
 /* Launch-configuration knobs (no counterpart in the reference; results are
  * identical for every setting, only speed changes):

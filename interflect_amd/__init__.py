@@ -397,7 +397,10 @@ class Context:
         self._check(lib().ifx_set_tuning(self._h, key.encode(), int(value)))
 
     def set_counting(self, on):
-        self._check(lib().ifx_set_counting(self._h, 1 if on else 0))
+        """True / 1: counting instantiation (FP32 operations and primitive
+        evaluations of the reference traversal); 2: the timed kernels count
+        the primitive evaluations they execute; False / 0: off."""
+        self._check(lib().ifx_set_counting(self._h, int(on)))
 
     def kernel_ops(self, name):
         n = C.c_uint64()

@@ -20,8 +20,6 @@
 // launch shapes: transposed links are summed per fixed-size segment by a
 // fixed lane tree, the gather scatter adds 64-bit fixed-point integers, and
 // parameter sums are per-block f64 trees that the host adds in block order.
-#include <hipcub/hipcub.hpp>
-
 #include <cmath>
 
 #include "ifx_kern.h"
@@ -500,37 +498,26 @@ int Engine::adjoint_prepare(std::string& err) {
                                                                 d_perm.as<uint32_t>());
             k_col_count<<<blocks_for(L, 256), 256, 0, stream>>>(L, d_idx.as<uint32_t>(), d_colcnt.as<uint32_t>());
         });
-        size_t tmp = 0;
         const int kb = (int)bits_for(C ? C - 1 : 1);
-        CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_idx.as<uint32_t>(), d_ckeyt.as<uint32_t>(),
-                                              d_perm.as<uint32_t>(), d_perm2.as<uint32_t>(), (int)L, 0, kb, stream));
-        CK(d_tmp.ensure(tmp));
-        CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_idx.as<uint32_t>(), d_ckeyt.as<uint32_t>(),
-                                              d_perm.as<uint32_t>(), d_perm2.as<uint32_t>(), (int)L, 0, kb, stream));
+        if (int rc_s = sort_u32(d_idx.as<uint32_t>(), d_perm.as<uint32_t>(), d_ckeyt.as<uint32_t>(),
+                           d_perm2.as<uint32_t>(), L, (uint32_t)kb, err))
+            return rc_s;
         launch("adj_prep", [&] {
             k_csc_fill<<<blocks_for(L, 256), 256, 0, stream>>>(L, d_perm2.as<uint32_t>(), d_lrow.as<uint32_t>(),
                                                                d_w.as<float>(), d_crow.as<uint32_t>(),
                                                                d_cw.as<float>());
         });
     }
-    size_t tmp = 0;
-    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_colcnt.as<uint32_t>(), d_colstart.as<uint32_t>(),
-                                        (int)(C + 1), stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_colcnt.as<uint32_t>(), d_colstart.as<uint32_t>(),
-                                        (int)(C + 1), stream));
+    if (int rc_s = scan_u32(d_colcnt.as<uint32_t>(), d_colstart.as<uint32_t>(), (C + 1), err))
+        return rc_s;
     CK(hipMemsetAsync(d_segcnt.p, 0, 4ull * (C + 1), stream));
     if (C)
         launch("adj_prep", [&] {
             k_seg_count<<<blocks_for(C, 256), 256, 0, stream>>>(C, d_colstart.as<uint32_t>(), S,
                                                                 d_segcnt.as<uint32_t>());
         });
-    tmp = 0;
-    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_segcnt.as<uint32_t>(), d_segfirst.as<uint32_t>(),
-                                        (int)(C + 1), stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_segcnt.as<uint32_t>(), d_segfirst.as<uint32_t>(),
-                                        (int)(C + 1), stream));
+    if (int rc_s = scan_u32(d_segcnt.as<uint32_t>(), d_segfirst.as<uint32_t>(), (C + 1), err))
+        return rc_s;
     uint32_t nseg = 0;
     CK(hipMemcpyAsync(&nseg, d_segfirst.as<uint32_t>() + C, 4, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));

@@ -422,7 +422,11 @@ ifx_status ifx_reset_kernel_times(ifx_ctx* ctx) {
 
 ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on) {
     return guarded(ctx, [&]() -> ifx_status {
-        ctx->eng.set_counting(on != 0);
+        if (on < 0 || on > 2) {
+            ctx->err = "counting mode must be 0, 1 or 2";
+            return IFX_E_INVALID;
+        }
+        ctx->eng.set_counting(on);
         return IFX_OK;
     });
 }

@@ -143,8 +143,14 @@ class Engine {
     int n_cu() const { return n_cu_; }
     int device() const { return device_; }
     bool has_device() const { return has_device_; }
-    // Instrumented launches of the tracing kernels (algorithmic FP32 operation tallies).
-    void set_counting(bool on) { count_ops_ = on; }
+    // Instrumented launches of the tracing kernels: 1 = the counting
+    // instantiation (algorithmic FP32 operations and primitive evaluations of
+    // the reference traversal), 2 = the timed kernels tally the primitive
+    // evaluations they execute; 0 = off.
+    void set_counting(int mode) {
+        count_ops_ = mode == 1;
+        count_exec_ = mode == 2;
+    }
     const std::map<std::string, uint64_t>& kernel_ops() const { return kops_; }
     const std::map<std::string, KernelStat>& kernel_stats() const { return kstats_; }
     void reset_kernel_stats() {
@@ -165,10 +171,9 @@ class Engine {
     uint32_t jacobi_rows = 0;  // 0 automatic, 1 lane per row on the CSR, 2 wavefront per row, 3 sliced rows
     float solve_tol = 1e-4f;   // Jacobi stops once max |delta| < solve_tol (src/solve.rs:98)
     bool place_on_gpu = true;  // ifx_generate_surfels on the device when there is one
-    // transfer() launches the closure rays and the direct-light terms on a
-    // second stream, beside the cluster build, the candidate sort and the
-    // visibility kernel; the link calibration waits for the closure and
-    // solve() for the terms.
+    // transfer() launches the direct-light terms and the closure rays on a
+    // second stream, beside the cluster build and the candidate culls and
+    // sort; the link visibility waits for them.
     bool overlap_direct = true;
     uint32_t spp = 512;        // paths per pixel of the reference mode (--spp, src/main.rs:81)
     uint32_t adj_max_iters = 1000;  // adjoint sweeps at most
@@ -200,6 +205,14 @@ class Engine {
     void upload_materials();
     void upload_lights();
     int check(hipError_t e, const char* what, std::string& err);
+    // Stable radix sort of (key, u32 value) pairs by the low `bits` key bits
+    // into the output arrays (inputs untouched), and an exclusive u32 scan,
+    // on `stream`: kernels only (ifx_sort.h), scratch in d_tmp / d_sort_*.
+    int sort_u32(const uint32_t* k, const uint32_t* v, uint32_t* ko, uint32_t* vo, uint64_t n, uint32_t bits,
+                 std::string& err);
+    int sort_u64(const uint64_t* k, const uint32_t* v, uint64_t* ko, uint32_t* vo, uint64_t n, uint32_t bits,
+                 std::string& err);
+    int scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, std::string& err);
     int need_device(std::string& err) const;
     SceneView dev_view() const;
     LightsView dev_lights() const;
@@ -225,6 +238,11 @@ class Engine {
     bool has_device_ = false;
     bool prof_ = false;
     bool count_ops_ = false;
+    bool count_exec_ = false;
+    bool tally() const { return count_ops_ || count_exec_; }
+    int tally_mode() const { return count_ops_ ? 1 : (count_exec_ ? 2 : 0); }
+    // kops_[name] (+ ".prims", ".exec_prims") from a kernel's counter slot
+    void tally_ops(const char* name, const unsigned long long* cv, int slot);
     std::map<std::string, uint64_t> kops_;
     std::map<std::string, KernelStat> kstats_;
     struct Pending {
@@ -251,7 +269,7 @@ class Engine {
     // shade
     DevBuf d_gkey, d_gkey2, d_gval, d_gval2, d_gstart, d_film, d_rgb8;
     // misc
-    DevBuf d_counter, d_tmp, d_probe_in, d_probe_out;
+    DevBuf d_counter, d_tmp, d_probe_in, d_probe_out, d_sort_k, d_sort_v;
     // adjoint: surfel materials, transposed links (cluster-major), segments, iterates, parameter partials
     DevBuf d_smat, d_scl, d_sscale, d_lrow, d_perm, d_perm2, d_ckeyt, d_ckeyt2, d_crow, d_cw, d_colcnt, d_colstart,
         d_segcnt, d_segfirst, d_seg, d_segsum, d_ubar, d_uacc, d_umax, d_mu, d_v, d_dmax, d_pmat, d_pdr, d_pde, d_sdr,

@@ -12,14 +12,13 @@
 //   shade:    Grid::build     src/surfel.rs:71-114       -> k_grid_keys + radix sort
 //             render_frame    src/main.rs:229-367        -> k_shade
 //             Film::encode    src/film.rs:51-61          -> k_encode
-#include <hipcub/hipcub.hpp>
-
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 
 #include "ifx_kern.h"
+#include "ifx_sort.h"
 
 namespace ifx {
 
@@ -279,24 +278,27 @@ __global__ void k_morton(const float4* __restrict__ spos, uint32_t N, f3 lo, f3 
 // kernel: every lane marches one candidate's shadow ray a sample at a time
 // and takes the next candidate from the queue as soon as its ray ends, so
 // short and long marches no longer idle each other.  Candidates are taken
-// in `order` (cluster-major, surfels in Morton order: near-parallel rays
-// from nearby origins share a wavefront).  Each sample's distances are
-// answered for the whole wavefront by one shared BVH traversal (wave_dist;
-// the scene stays in global memory and is read through the scalar cache).
-// cand_v[slot] = ff * vis for a kept link, 0 for a culled one; slots are
-// fixed, so the work order never changes a result.
-template <bool COUNT, bool LDS>
+// in visiting order (cluster-major, surfels in Morton order: near-parallel
+// rays from nearby origins share a wavefront), and their inputs and results
+// are stored in that order too, so every index read and result write is
+// coalesced: item q reads its row crow[q] and cluster ckey[q] >> shift and
+// writes vis[q] = ff * vis for a kept link, 0 for a culled one (k_vis_order
+// and k_vis_scatter move between visiting and row-major order).  Each
+// sample's distances are answered for the whole wavefront by one shared BVH
+// traversal (wave_dist; the scene stays in global memory and is read
+// through the scalar cache).  Results sit at fixed positions, so the work
+// order never changes one.
+template <int MODE, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView Lg, LdsPlan P,
                                                        const float4* __restrict__ spos,
                                                        const float4* __restrict__ snrm,
                                                        const float4* __restrict__ cpos,
                                                        const float4* __restrict__ cnrm,
-                                                       const uint32_t* __restrict__ cand_ci,
-                                                       const uint32_t* __restrict__ cand_row,
-                                                       const uint32_t* __restrict__ order, uint32_t M, float off,
-                                                       float tmin, float* cand_v, uint32_t* queue, uint32_t chunk,
+                                                       const uint32_t* __restrict__ ckey, uint32_t shift,
+                                                       const uint32_t* __restrict__ crow, uint32_t M, float off,
+                                                       float tmin, float* vis_out, uint32_t* queue, uint32_t chunk,
                                                        unsigned long long* evals, unsigned long long* ops) {
-    IFX_SETUP(LDS, false, COUNT);
+    IFX_SETUP(LDS, false, MODE);
     uint32_t acc = 0;
     bool live = false;
     WaveQueue wq{0u, 0u, chunk, false};
@@ -307,8 +309,8 @@ __global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView 
     for (;;) {
         uint32_t q;
         if (refill(live, wq, queue, M, q)) {
-            slot = order ? order[q] : q;
-            uint32_t i = cand_row[slot], c = cand_ci[slot];
+            slot = q;
+            const uint32_t i = crow[q], c = ckey[q] >> shift;
             f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
             float4 CN = cnrm[c];
             float r = 0.0f;
@@ -321,7 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView 
         if (__ballot(live) == 0) break;
         float vis = 0.0f;
         if (shadow_step(S, m, live, vis, stk)) {
-            cand_v[slot] = vis <= 1e-3f ? 0.0f : ff * vis;
+            vis_out[slot] = vis <= 1e-3f ? 0.0f : ff * vis;
             acc += m.steps;
             live = false;
         }
@@ -330,12 +332,30 @@ __global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView 
     stk_flush(stk, ops);
 }
 
+// Candidates up to which the link visibility may share the GPU with the
+// side stream's closure and direct-light launches (measured, see transfer()).
+constexpr uint32_t kVisOverlapMax = 1u << 23;
+
+// Visiting order <-> row-major candidates: the row of each visited
+// candidate, and its result back to the candidate's row-major slot.
+__global__ void k_vis_order(uint32_t M, const uint32_t* __restrict__ order, const uint32_t* __restrict__ cand_row,
+                            uint32_t* vrow) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < M) vrow[q] = cand_row[order[q]];
+}
+
+__global__ void k_vis_scatter(uint32_t M, const uint32_t* __restrict__ order, const float* __restrict__ vis,
+                              float* cand_v) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < M) cand_v[order[q]] = vis[q];
+}
+
 // Hemisphere closure (src/trace.rs:165-186), persistent like k_visibility.
 // Item q = direction-major over surfels in Morton order, so a wavefront
 // traces one Halton direction from 64 neighbouring surfels; a missed ray
 // adds one to its surfel's integer open count.  dirs[k] = (r cos phi,
 // r sin phi, sqrt(1 - u1)).
-template <bool COUNT, bool LDS>
+template <int MODE, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg, LdsPlan P,
                                                     const float4* __restrict__ spos,
                                                     const float4* __restrict__ snrm, uint32_t N,
@@ -343,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg,
                                                     const float4* __restrict__ dirs, float tmax, uint32_t* open,
                                                     uint32_t* queue, uint32_t chunk, unsigned long long* evals,
                                                     unsigned long long* ops) {
-    IFX_SETUP(LDS, false, COUNT);
+    IFX_SETUP(LDS, false, MODE);
     uint32_t acc = 0;
     bool live = false;
     WaveQueue wq{0u, 0u, chunk, false};
@@ -774,6 +794,32 @@ int Engine::check(hipError_t e, const char* what, std::string& err) {
 }
 
 
+int Engine::scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, std::string& err) {
+    CK(d_tmp.ensure(4ull * gsort::scan_scratch_words(n)));
+    CK(gsort::exclusive_scan(in, out, n, d_tmp.as<uint32_t>(), stream));
+    return 0;
+}
+
+int Engine::sort_u32(const uint32_t* k, const uint32_t* v, uint32_t* ko, uint32_t* vo, uint64_t n, uint32_t bits,
+                     std::string& err) {
+    CK(d_tmp.ensure(4ull * gsort::sort_scratch_words(n)));
+    CK(d_sort_k.ensure(4ull * n));
+    CK(d_sort_v.ensure(4ull * n));
+    CK(gsort::sort_pairs_to<uint32_t>(k, v, ko, vo, d_sort_k.as<uint32_t>(), d_sort_v.as<uint32_t>(), n, bits,
+                                      d_tmp.as<uint32_t>(), stream));
+    return 0;
+}
+
+int Engine::sort_u64(const uint64_t* k, const uint32_t* v, uint64_t* ko, uint32_t* vo, uint64_t n, uint32_t bits,
+                     std::string& err) {
+    CK(d_tmp.ensure(4ull * gsort::sort_scratch_words(n)));
+    CK(d_sort_k.ensure(8ull * n));
+    CK(d_sort_v.ensure(4ull * n));
+    CK(gsort::sort_pairs_to<uint64_t>(k, v, ko, vo, d_sort_k.as<uint64_t>(), d_sort_v.as<uint32_t>(), n, bits,
+                                      d_tmp.as<uint32_t>(), stream));
+    return 0;
+}
+
 Engine::~Engine() {
     if (stream) {
         (void)hipSetDevice(device_);
@@ -841,7 +887,7 @@ int Engine::init(int device, std::string& err) {
     }
     CK(d_srgb.ensure(sizeof(thr)));
     CK(hipMemcpy(d_srgb.p, thr, sizeof(thr), hipMemcpyHostToDevice));
-    CK(d_counter.ensure(64));
+    CK(d_counter.ensure(256));
     CK(scratch_device(device, &scratch_));
     has_device_ = true;
     return 0;
@@ -873,6 +919,16 @@ void Engine::drain_events() {
         event_pool_.push_back(p.b);
     }
     pending_.swap(keep);
+}
+
+void Engine::tally_ops(const char* name, const unsigned long long* cv, int slot) {
+    const std::string k(name);
+    if (count_ops_) {
+        kops_[k] += cv[slot];
+        kops_[k + ".prims"] += cv[slot + 4];
+    } else if (count_exec_) {
+        kops_[k + ".exec_prims"] += cv[slot + 8];
+    }
 }
 
 void Engine::join_side() {
@@ -1036,7 +1092,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         surfels_dirty = false;
     }
     unsigned long long* ctr = d_counter.as<unsigned long long>();
-    CK(hipMemsetAsync(ctr, 0, 64, stream));
+    CK(hipMemsetAsync(ctr, 0, 256, stream));
     n_clusters = 0;
     n_links = 0;
     if (N == 0) {
@@ -1053,7 +1109,6 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     CK(d_mval.ensure(4ull * N));
     CK(d_mkey_s.ensure(4ull * N));
     CK(d_rorder.ensure(4ull * N));
-    size_t tmp = 0;
     {
         const f3 ext = scene.hi - scene.lo;
         const f3 sc = mk(32.0f / fmx(ext.x, 1e-6f), 32.0f / fmx(ext.y, 1e-6f), 32.0f / fmx(ext.z, 1e-6f));
@@ -1061,26 +1116,29 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
             k_morton<<<blocks_for(N, 256), 256, 0, stream>>>(spos, N, scene.lo, sc, d_morton.as<uint32_t>(),
                                                              d_mval.as<uint32_t>());
         });
-        CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_morton.as<uint32_t>(), d_mkey_s.as<uint32_t>(),
-                                              d_mval.as<uint32_t>(), d_rorder.as<uint32_t>(), (int)N, 0, 15, stream));
-        CK(d_tmp.ensure(tmp));
-        CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_morton.as<uint32_t>(), d_mkey_s.as<uint32_t>(),
-                                              d_mval.as<uint32_t>(), d_rorder.as<uint32_t>(), (int)N, 0, 15, stream));
+        if (int rc_s = sort_u32(d_morton.as<uint32_t>(), d_mval.as<uint32_t>(), d_mkey_s.as<uint32_t>(),
+                           d_rorder.as<uint32_t>(), N, 15u, err))
+            return rc_s;
     }
     CK(d_queue.ensure(16));
     CK(hipMemsetAsync(d_queue.p, 0, 16, stream));
     uint32_t* queue = d_queue.as<uint32_t>();
     const uint32_t persist = (uint32_t)n_cu_ * 8u;
 
-    // ---- closure rays and direct-light terms: they need only the surfels,
+    // ---- direct-light terms and closure rays: they need only the surfels,
     // the scene and the lights, so they run on the side stream while this
-    // stream builds clusters and candidates (whose host reads of counts
-    // would otherwise leave the device idle) and traces the links.  The
-    // link calibration waits for the closure; solve() for the terms.
+    // stream builds clusters and candidates, whose host reads of counts
+    // would otherwise leave the device idle.  A long link-visibility launch
+    // waits for all of it: sharing the GPU with that persistent launch
+    // slowed it by more than the side work's own time (stress scene, 68M
+    // candidates: 343 against 313 ms with the terms beside it, 345 with the
+    // closure).  A short one (up to kVisOverlapMax candidates) runs beside
+    // the side work, which then fills its tail (product scene, 1.4M
+    // candidates: frame 4.0 against 4.7 ms).
     CK(d_sky.ensure(4ull * N));
     CK(d_open.ensure(4ull * N));
     CK(hipMemsetAsync(d_open.p, 0, 4ull * N, stream));
-    const bool side = overlap_direct && !count_ops_;
+    const bool side = overlap_direct && !tally();
     const bool pre = side && n_lights(dev_lights()) > 0;
     if (pre) {
         CK(d_dctr.ensure(64));
@@ -1089,6 +1147,10 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     if (side) {
         CK(hipEventRecord(fork_, stream));
         CK(hipStreamWaitEvent(side_, fork_, 0));
+        const int rc = pre ? launch_direct(side_, d_dctr.as<unsigned long long>(), nullptr, err) : 0;
+        side_pending_ = true;
+        direct_pre_ = pre && rc == 0;
+        if (rc) return rc;
     }
     {
         hipStream_t cs = side ? side_ : stream;
@@ -1097,18 +1159,14 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         const uint32_t grid = std::max(1u, std::min<uint32_t>(persist, blocks_for(16ull * N, kBlock)));
         launch_on(cs, "closure", [&] {
             hipStream_t stream = cs;  // the launch macro's stream
-            LAUNCH_CL(k_closure, count_ops_, ld.staged, grid, ld.bytes, S, Lg, ld.plan, spos, snrm, N,
+            LAUNCH_CM(k_closure, tally_mode(), ld.staged, grid, ld.bytes, S, Lg, ld.plan, spos, snrm, N,
                       d_rorder.as<uint32_t>(), d_dirs.as<float4>(), diag * 2.0f, d_open.as<uint32_t>(), queue + 1,
-                      wave_chunk(16ull * N, grid), ctr, count_ops_ ? ctr + 2 : nullptr);
+                      wave_chunk(16ull * N, grid), ctr, tally() ? ctr + 2 : nullptr);
         });
     }
     if (side) {
         CK(hipEventRecord(closed_, side_));
-        const int rc = pre ? launch_direct(side_, d_dctr.as<unsigned long long>(), nullptr, err) : 0;
         CK(hipEventRecord(join_, side_));
-        side_pending_ = true;
-        direct_pre_ = pre && rc == 0;
-        if (rc) return rc;
     }
 
     // ---- clusters: key, stable sort, segments, serial sums, compaction
@@ -1126,22 +1184,15 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         k_cluster_keys<<<blocks_for(N, 256), 256, 0, stream>>>(spos, snrm, N, lo, inv, res, d_ckey.as<uint64_t>(),
                                                                d_cval.as<uint32_t>());
     });
-    tmp = 0;
-    CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_ckey.as<uint64_t>(), d_ckey2.as<uint64_t>(),
-                                          d_cval.as<uint32_t>(), d_cval2.as<uint32_t>(), (int)N, 0, (int)key_bits,
-                                          stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_ckey.as<uint64_t>(), d_ckey2.as<uint64_t>(),
-                                          d_cval.as<uint32_t>(), d_cval2.as<uint32_t>(), (int)N, 0, (int)key_bits,
-                                          stream));
+    if (int rc_s = sort_u64(d_ckey.as<uint64_t>(), d_cval.as<uint32_t>(), d_ckey2.as<uint64_t>(),
+                       d_cval2.as<uint32_t>(), N, key_bits, err))
+        return rc_s;
     const uint32_t* members = d_cval2.as<uint32_t>();
     launch("seg_flags", [&] {
         k_seg_flags<<<blocks_for(N, 256), 256, 0, stream>>>(d_ckey2.as<uint64_t>(), N, d_cflag.as<uint32_t>());
     });
-    tmp = 0;
-    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_cflag.as<uint32_t>(), d_cseg.as<uint32_t>(), (int)N, stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_cflag.as<uint32_t>(), d_cseg.as<uint32_t>(), (int)N, stream));
+    if (int rc_s = scan_u32(d_cflag.as<uint32_t>(), d_cseg.as<uint32_t>(), N, err))
+        return rc_s;
     uint32_t tail[2];
     CK(hipMemcpyAsync(&tail[0], d_cseg.as<uint32_t>() + N - 1, 4, hipMemcpyDeviceToHost, stream));
     CK(hipMemcpyAsync(&tail[1], d_cflag.as<uint32_t>() + N - 1, 4, hipMemcpyDeviceToHost, stream));
@@ -1166,12 +1217,8 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
                                                                  raw_pos.as<float4>(), raw_nrm.as<float4>(),
                                                                  d_cvalid.as<uint32_t>());
     });
-    tmp = 0;
-    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_cvalid.as<uint32_t>(), d_cidx.as<uint32_t>(), (int)nseg,
-                                        stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_cvalid.as<uint32_t>(), d_cidx.as<uint32_t>(), (int)nseg,
-                                        stream));
+    if (int rc_s = scan_u32(d_cvalid.as<uint32_t>(), d_cidx.as<uint32_t>(), nseg, err))
+        return rc_s;
     launch("cluster_compact", [&] {
         k_cluster_compact<<<blocks_for(nseg, 256), 256, 0, stream>>>(
             nseg, d_cvalid.as<uint32_t>(), d_cidx.as<uint32_t>(), seg_begin.as<uint32_t>(), raw_pos.as<float4>(),
@@ -1198,12 +1245,8 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
                                                  d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr,
                                                  nullptr);
     });
-    tmp = 0;
-    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_ccount.as<uint32_t>(), d_cstart.as<uint32_t>(), (int)(N + 1),
-                                        stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_ccount.as<uint32_t>(), d_cstart.as<uint32_t>(), (int)(N + 1),
-                                        stream));
+    if (int rc_s = scan_u32(d_ccount.as<uint32_t>(), d_cstart.as<uint32_t>(), (N + 1), err))
+        return rc_s;
     uint32_t M = 0;
     CK(hipMemcpyAsync(&M, d_cstart.as<uint32_t>() + N, 4, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
@@ -1232,28 +1275,40 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
                                                 sort_cand ? d_vval.as<uint32_t>() : nullptr);
     });
     if (sort_cand) {
-        tmp = 0;
-        CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_vkey.as<uint32_t>(), d_vkey_s.as<uint32_t>(),
-                                              d_vval.as<uint32_t>(), d_vorder.as<uint32_t>(), (int)M, 0,
-                                              (int)vkey_bits, stream));
-        CK(d_tmp.ensure(tmp));
-        CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_vkey.as<uint32_t>(), d_vkey_s.as<uint32_t>(),
-                                              d_vval.as<uint32_t>(), d_vorder.as<uint32_t>(), (int)M, 0,
-                                              (int)vkey_bits, stream));
+        if (int rc_s = sort_u32(d_vkey.as<uint32_t>(), d_vval.as<uint32_t>(), d_vkey_s.as<uint32_t>(),
+                           d_vorder.as<uint32_t>(), M, vkey_bits, err))
+            return rc_s;
     }
 
-    // ---- link visibility
+    // ---- link visibility, in visiting order (the unsorted fallback visits row-major)
+    if (side && M > kVisOverlapMax) CK(hipStreamWaitEvent(stream, join_, 0));  // the side stream is done
     if (M) {
         const uint32_t visb =
             vis_blocks ? vis_blocks : std::max(1u, std::min<uint32_t>(persist, blocks_for(M, kBlock)));
         const Lds ld = plan_lds(false, false);
         const LightsView Lg{};
+        // the sort's input keys and values are dead: they hold the visiting-order rows and results
+        uint32_t* vrow = d_vkey.as<uint32_t>();
+        float* vis = sort_cand ? d_vval.as<float>() : d_cand_v.as<float>();
+        if (sort_cand) {
+            launch("vis_order", [&] {
+                k_vis_order<<<blocks_for(M, 256), 256, 0, stream>>>(M, d_vorder.as<uint32_t>(),
+                                                                    d_cand_row.as<uint32_t>(), vrow);
+            });
+        }
         launch("visibility", [&] {
-            LAUNCH_CL(k_visibility, count_ops_, ld.staged, visb, ld.bytes, S, Lg, ld.plan, spos, snrm,
-                      d_cpos.as<float4>(), d_cnrm.as<float4>(), d_cand_ci.as<uint32_t>(), d_cand_row.as<uint32_t>(),
-                      sort_cand ? d_vorder.as<uint32_t>() : nullptr, M, diag * 5e-4f, diag * 1e-3f,
-                      d_cand_v.as<float>(), queue, wave_chunk(M, visb), ctr, count_ops_ ? ctr + 1 : nullptr);
+            LAUNCH_CM(k_visibility, tally_mode(), ld.staged, visb, ld.bytes, S, Lg, ld.plan, spos, snrm,
+                      d_cpos.as<float4>(), d_cnrm.as<float4>(),
+                      sort_cand ? d_vkey_s.as<uint32_t>() : d_cand_ci.as<uint32_t>(), sort_cand ? 15u : 0u,
+                      sort_cand ? vrow : d_cand_row.as<uint32_t>(), M, diag * 5e-4f, diag * 1e-3f, vis, queue,
+                      wave_chunk(M, visb), ctr, tally() ? ctr + 1 : nullptr);
         });
+        if (sort_cand) {
+            launch("vis_scatter", [&] {
+                k_vis_scatter<<<blocks_for(M, 256), 256, 0, stream>>>(M, d_vorder.as<uint32_t>(), vis,
+                                                                      d_cand_v.as<float>());
+            });
+        }
     }
 
     // ---- calibration, link scan, compaction
@@ -1269,12 +1324,8 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
                                                           d_lcount.as<uint32_t>(), d_scale.as<float>(),
                                                           d_rowsum.as<float>());
     });
-    tmp = 0;
-    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_lcount.as<uint32_t>(), d_lstart.as<uint32_t>(), (int)(N + 1),
-                                        stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_lcount.as<uint32_t>(), d_lstart.as<uint32_t>(), (int)(N + 1),
-                                        stream));
+    if (int rc_s = scan_u32(d_lcount.as<uint32_t>(), d_lstart.as<uint32_t>(), (N + 1), err))
+        return rc_s;
     uint32_t L = 0;
     CK(hipMemcpyAsync(&L, d_lstart.as<uint32_t>() + N, 4, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
@@ -1287,12 +1338,13 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     });
     // mean_row_sum: serial f32 sum over rows with links, in row order.
     std::vector<float> rs(N);
-    unsigned long long cv[8] = {0};
+    unsigned long long cv[32] = {0};
     CK(hipMemcpyAsync(rs.data(), d_rowsum.p, 4ull * N, hipMemcpyDeviceToHost, stream));
-    CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(cv, ctr, 256, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
     unsigned long long ev = cv[0];
-    if (count_ops_) kops_["visibility"] += cv[1], kops_["closure"] += cv[2];
+    tally_ops("visibility", cv, 1);
+    tally_ops("closure", cv, 2);
     drain_events();
     float acc = -0.0f;
     uint64_t interior = 0;
@@ -1334,7 +1386,7 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
     CK(hipSetDevice(device_));
     out = SolveOut();
     // direct-light terms precomputed by transfer() on the side stream
-    const bool pre = direct_pre_ && !count_ops_;
+    const bool pre = direct_pre_ && !tally();
     join_side();
     direct_pre_ = false;
     const uint32_t N = (uint32_t)surfels.size();
@@ -1348,12 +1400,12 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
     CK(d_terms.ensure(16ull * N * (nl ? nl : 1)));
     CK(d_gterm.ensure(4ull * N * (nl ? nl : 1)));
     unsigned long long* ctr = d_counter.as<unsigned long long>();
-    CK(hipMemsetAsync(ctr, 0, 64, stream));
+    CK(hipMemsetAsync(ctr, 0, 256, stream));
     CK(hipMemsetAsync(d_resid.p, 0, 4ull * (bounces + 1), stream));
     const SceneView S = dev_view();
     if (N) {
         if (nl && !pre) {
-            if (int rc = launch_direct(stream, ctr, count_ops_ ? ctr + 3 : nullptr, err)) return rc;
+            if (int rc = launch_direct(stream, ctr, tally() ? ctr + 3 : nullptr, err)) return rc;
         }
         launch("direct_sum", [&] {
             k_direct_sum<<<blocks_for(N, 256), 256, 0, stream>>>(N, nl, d_terms.as<float4>(), d_direct.as<float4>(),
@@ -1415,9 +1467,9 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
         }
     }
     std::vector<uint32_t> res(bounces + 1, 0);
-    unsigned long long cv[8] = {0};
+    unsigned long long cv[32] = {0};
     CK(hipMemcpyAsync(res.data(), d_resid.p, 4ull * bounces, hipMemcpyDeviceToHost, stream));
-    CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(cv, ctr, 256, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
     drain_events();
     unsigned long long ev = cv[0];
@@ -1426,7 +1478,7 @@ int Engine::solve(uint32_t bounces, SolveOut& out, std::string& err) {
         CK(hipMemcpy(&dv, d_dctr.p, 8, hipMemcpyDeviceToHost));
         ev += dv;
     }
-    if (count_ops_) kops_["direct_e"] += cv[3];
+    tally_ops("direct_e", cv, 3);
     out.evals = ev;
     have_beauty_ = false;
     if (C > 0 && N > 0) {
@@ -1581,12 +1633,8 @@ int Engine::build_sliced(std::string& err) {
     if (N)
         k_sell_widths<<<blocks_for(64ull * ns, 256), 256, 0, stream>>>(N, d_lstart.as<uint32_t>(),
                                                                        width.as<uint32_t>());
-    size_t tmp = 0;
-    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, width.as<uint32_t>(), d_sbase.as<uint32_t>(), (int)(ns + 1),
-                                        stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, width.as<uint32_t>(), d_sbase.as<uint32_t>(), (int)(ns + 1),
-                                        stream));
+    if (int rc_s = scan_u32(width.as<uint32_t>(), d_sbase.as<uint32_t>(), (ns + 1), err))
+        return rc_s;
     uint32_t cols = 0;
     CK(hipMemcpyAsync(&cols, d_sbase.as<uint32_t>() + ns, 4, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
@@ -1661,22 +1709,13 @@ int Engine::shade_setup(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode
                                                                     counts.as<uint32_t>());
             });
         }
-        size_t tmp = 0;
-        CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, counts.as<uint32_t>(), d_gstart.as<uint32_t>(),
-                                            (int)(ncells + 1), stream));
-        CK(d_tmp.ensure(tmp));
-        CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, counts.as<uint32_t>(), d_gstart.as<uint32_t>(),
-                                            (int)(ncells + 1), stream));
+        if (int rc_s = scan_u32(counts.as<uint32_t>(), d_gstart.as<uint32_t>(), (ncells + 1), err))
+            return rc_s;
         if (N) {
             const int kb = (int)bits_for(ncells - 1 ? ncells - 1 : 1);
-            tmp = 0;
-            CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_gkey.as<uint32_t>(), d_gkey2.as<uint32_t>(),
-                                                  d_gval.as<uint32_t>(), d_gval2.as<uint32_t>(), (int)N, 0, kb,
-                                                  stream));
-            CK(d_tmp.ensure(tmp));
-            CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_gkey.as<uint32_t>(), d_gkey2.as<uint32_t>(),
-                                                  d_gval.as<uint32_t>(), d_gval2.as<uint32_t>(), (int)N, 0, kb,
-                                                  stream));
+            if (int rc_s = sort_u32(d_gkey.as<uint32_t>(), d_gval.as<uint32_t>(), d_gkey2.as<uint32_t>(),
+                               d_gval2.as<uint32_t>(), N, (uint32_t)kb, err))
+                return rc_s;
         }
         A.grid.inv_cell = inv_cell;
         A.grid.dim[0] = dx;
@@ -1694,7 +1733,7 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
     if (int rc = shade_setup(cam, W, H, mode, A, err)) return rc;
     have_beauty_ = false;
     unsigned long long* ctr = d_counter.as<unsigned long long>();
-    CK(hipMemsetAsync(ctr, 0, 64, stream));
+    CK(hipMemsetAsync(ctr, 0, 256, stream));
     const uint64_t npx = (uint64_t)W * H;
     CK(d_film.ensure(16ull * npx));
     CK(d_rgb8.ensure(3ull * npx));
@@ -1705,7 +1744,7 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
     launch("shade", [&] {
         const bool wave = S.n_obj > kWaveScanObjs || npx < kWaveShadeMaxPixels;
         LAUNCH_CLW(k_shade, count_ops_, ld.staged, wave, grid, ld.bytes, S, Lv, ld.plan, A, d_film.as<float4>(), ctr,
-                   count_ops_ ? ctr + 4 : nullptr);
+                   tally() ? ctr + 4 : nullptr);
     });
     const bool tonemapped = mode_tonemapped(mode);
     if (rgb8) {
@@ -1716,12 +1755,12 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
         });
     }
     if (int rc = stage_images(npx, rgb8 != nullptr, film != nullptr, err)) return rc;
-    unsigned long long cv[8] = {0};
-    CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
+    unsigned long long cv[32] = {0};
+    CK(hipMemcpyAsync(cv, ctr, 256, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
     finish_images(npx, rgb8, film);
     drain_events();
-    if (count_ops_) kops_["shade"] += cv[4];
+    tally_ops("shade", cv, 4);
     evals = cv[0];
     if (mode == M_BEAUTY) {
         have_beauty_ = true;
@@ -1759,7 +1798,7 @@ int Engine::shade_views(const std::vector<HCamera>& cams, uint32_t W, uint32_t H
     std::vector<CamV> hc(nv);
     for (uint32_t v = 0; v < nv; ++v) hc[v] = cam_basis(cams[v], W, H);
     unsigned long long* ctr = d_counter.as<unsigned long long>();
-    CK(hipMemsetAsync(ctr, 0, 64, stream));
+    CK(hipMemsetAsync(ctr, 0, 256, stream));
     DevBuf& d_cams = d_probe_in;
     CK(d_cams.ensure(sizeof(CamV) * nv));
     CK(hipMemcpyAsync(d_cams.p, hc.data(), sizeof(CamV) * nv, hipMemcpyHostToDevice, stream));
@@ -1773,7 +1812,7 @@ int Engine::shade_views(const std::vector<HCamera>& cams, uint32_t W, uint32_t H
     launch("shade_views", [&] {
         const bool wave = S.n_obj > kWaveScanObjs || npx < kWaveShadeMaxPixels;
         LAUNCH_CLW(k_shade_views, count_ops_, ld.staged, wave, grid, ld.bytes, S, Lv, ld.plan, A, d_cams.as<CamV>(),
-                   d_film.as<float4>(), ctr, count_ops_ ? ctr + 4 : nullptr);
+                   d_film.as<float4>(), ctr, tally() ? ctr + 4 : nullptr);
     });
     if (rgb8) {
         launch("encode", [&] {
@@ -1783,12 +1822,12 @@ int Engine::shade_views(const std::vector<HCamera>& cams, uint32_t W, uint32_t H
         });
     }
     if (int rc = stage_images(npx, rgb8 != nullptr, film != nullptr, err)) return rc;
-    unsigned long long cv[8] = {0};
-    CK(hipMemcpyAsync(cv, ctr, 64, hipMemcpyDeviceToHost, stream));
+    unsigned long long cv[32] = {0};
+    CK(hipMemcpyAsync(cv, ctr, 256, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
     finish_images(npx, rgb8, film);
     drain_events();
-    if (count_ops_) kops_["shade"] += cv[4];
+    tally_ops("shade", cv, 4);
     evals = cv[0];
     return 0;
 }

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <functional>
 #include <sstream>
+#include <stdexcept>
 #include <unordered_map>
 
 namespace ifx {
@@ -423,6 +424,23 @@ void finalize_scene(HostScene& s) {
     for (size_t ni = 0; ni < nodes.size(); ++ni)
         for (uint32_t k = nodes[ni].start; nodes[ni].count > 0 && k < nodes[ni].start + nodes[ni].count; ++k)
             s.lin[n_obj + k] = (uint32_t)ni;
+    // The device traversals rely on this layout: every slot lies in exactly
+    // one non-empty leaf, and each interior node's children (node + 1 and
+    // `right`) lie after it, so a traversal always ends.
+    {
+        std::vector<uint32_t> cover(n_obj, 0u);
+        for (size_t ni = 0; ni < nodes.size(); ++ni) {
+            const BuildNode& n = nodes[ni];
+            if (n.count > 0) {
+                for (uint32_t k = n.start; k < n.start + n.count; ++k)
+                    if (k >= n_obj || cover[k]++) throw std::runtime_error("BVH leaf ranges overlap or overrun");
+            } else if (!(ni + 1 < nodes.size() && n.right > ni + 1 && n.right < nodes.size())) {
+                throw std::runtime_error("BVH interior node has an out-of-order child");
+            }
+        }
+        for (size_t k = 0; k < n_obj; ++k)
+            if (cover[k] != 1u) throw std::runtime_error("BVH leaves do not cover every object");
+    }
     s.plane_rec.assign(4 * s.planes.size(), make_float4(0, 0, 0, 0));
     for (size_t i = 0; i < s.planes.size(); ++i) object_rec(s.planes[i], &s.plane_rec[4 * i]);
     s.blend_k = 0.0f;
@@ -669,7 +687,12 @@ bool load_rad(const std::string& src, HostScene& s, std::string& err) {
         return false;
     }
     if (!validate_cfg(s.cfg, err)) return false;
-    finalize_scene(s);
+    try {
+        finalize_scene(s);
+    } catch (const std::runtime_error& e) {
+        err = e.what();
+        return false;
+    }
     return true;
 }
 

@@ -91,15 +91,21 @@ __device__ __forceinline__ void stage_in(float4* lds, SceneView& S, LightsView& 
     }
 }
 
-// Timed kernels traverse with LdsStack; the instrumented instantiation
-// (COUNT=true) uses CountStack and adds its operation tally to `ops`.
-template <bool COUNT>
+// Timed kernels traverse with LdsStack; the instrumented instantiations use
+// CountStack (MODE 1: the reference traversal's operation tally) or
+// ExecStack (MODE 2: primitive evaluations of the timed traversal) and add
+// their tallies to `ops`.
+template <int MODE>
 struct StackOf {
     using T = LdsStack;
 };
 template <>
-struct StackOf<true> {
+struct StackOf<1> {
     using T = CountStack;
+};
+template <>
+struct StackOf<2> {
+    using T = ExecStack;
 };
 
 // Stacks of the camera-ray shading kernels: LdsStack (wavefront-shared
@@ -129,11 +135,25 @@ struct ShadeStackOf<true, WAVE> {
 // twelve views in one launch 13 % slower).
 constexpr uint64_t kWaveShadeMaxPixels = 1ull << 21;
 
+// Counter slots of a tracing kernel, from its `ops` pointer: ops[0] counted
+// FP32 operations and ops[4] primitive evaluations of the reference
+// traversal (CountStack), ops[8] primitive evaluations executed by the
+// timed traversal (ExecStack).  Every lane of the wavefront calls stk_flush.
 __device__ __forceinline__ void stk_init(LdsStack&) {}
-__device__ __forceinline__ void stk_init(CountStack& s) { s.ops = 0ull; }
+__device__ __forceinline__ void stk_init(ExecStack& s) { s.xe = s.xw = 0u; }
+__device__ __forceinline__ void stk_init(CountStack& s) {
+    s.ops = 0ull;
+    s.prims = 0u;
+}
 __device__ __forceinline__ void stk_flush(LdsStack&, unsigned long long*) {}
+__device__ __forceinline__ void stk_flush(ExecStack& s, unsigned long long* ops) {
+    const uint32_t t = wave_sum(s.xe);
+    if (lane_id() == (uint32_t)__builtin_ctzll(__ballot(true)) && t + s.xw)
+        atomicAdd(ops + 8, (unsigned long long)t + s.xw);
+}
 __device__ __forceinline__ void stk_flush(CountStack& s, unsigned long long* ops) {
     if (s.ops) atomicAdd(ops, s.ops);
+    if (s.prims) atomicAdd(ops + 4, (unsigned long long)s.prims);
 }
 
 // LDS bytes of a block's lane columns: per lane `depth` f32 distances, then
@@ -216,6 +236,7 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
     const bool blend = S.blend_k > 0.0f;
     const float K = S.blend_k;
     const float delta = 1e-4f * (1.0f + maxe(abs3(p)));
+    const uint32_t lanes = (uint32_t)__popcll(__ballot(true));  // lanes evaluating each opened object
     float hard = kFltMax, b1 = kFltMax, b2 = kFltMax, h2 = kFltMax;
     uint32_t a = kNone, la = kNone, s1 = kNone, s2 = kNone, hmat = 0u, b1mat = 0u;
     for (uint32_t i = 0; i < S.n_planes; ++i) {
@@ -251,11 +272,13 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
         const uint32_t mat = fbits(g3.x), sb = fbits(g3.y), sc = fbits(g3.z);
         const bool bl = fbits(g3.w) != 0u;
         float d = prim_dist(g0, g1, g2, p);
+        stk.count_wave_prims(lanes);
         if (sc > 0) {
             if (__ballot(need && d < blin) == 0ull) return;
             for (uint32_t q = sb; q < sb + sc; ++q) {
                 const ConstF4 r = subs + 4 * q;
                 d = fmx(d, -prim_dist(r[0], r[1], r[2], p));
+                stk.count_wave_prims(lanes);
             }
         }
         if (blend && bl) {
@@ -282,7 +305,7 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
                 for (uint32_t j = k; j < k + cnt; ++j) take(j, ni);
                 refresh();
             }
-            k += cnt;
+            k += cnt ? cnt : 1u;  // leaves are non-empty (checked when the scene is loaded)
         }
     } else {
         int sp = 0;
@@ -356,11 +379,12 @@ __device__ __forceinline__ float wave_dist(const SceneView& S, f3 p, bool need, 
 
 // Wavefront-shared overloads of the march hooks (the per-lane versions and
 // their contracts sit beside the loops): all active lanes call together.
-__device__ __forceinline__ float query_dist(const SceneView& S, f3 p, LdsStack& stk, float ub = kFltMax) {
-    return wave_dist(S, p, true, stk, ub);
-}
-__device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, bool live, float& vis,
-                                            LdsStack& stk) {
+// The bodies are templates over the shared stack types (LdsStack,
+// ExecStack); the non-template overloads below select them, since an exact
+// match beats the per-lane templates.
+template <class Stk>
+__device__ __forceinline__ bool shadow_step_shared(const SceneView& S, ShadowMarch& m, bool live, float& vis,
+                                                   Stk& stk) {
     f3 p = spl(0.0f);
     bool ended = false, need = false;
     if (live) {
@@ -373,7 +397,8 @@ __device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, 
 }
 // A sphere-trace step when only hit / miss matters (no material): 0
 // marching, 1 hit, 2 miss.
-__device__ __forceinline__ int miss_step(const SceneView& S, TraceMarch& m, bool live, LdsStack& stk) {
+template <class Stk>
+__device__ __forceinline__ int miss_step_shared(const SceneView& S, TraceMarch& m, bool live, Stk& stk) {
     f3 p = spl(0.0f);
     int st = 0;
     bool need = false;
@@ -385,13 +410,9 @@ __device__ __forceinline__ int miss_step(const SceneView& S, TraceMarch& m, bool
     if (need) st = m.post(h);
     return st;
 }
-__device__ __forceinline__ int miss_step(const SceneView& S, TraceMarch& m, bool live, CountStack& stk) {
-    DistHit h;
-    return live ? m.step(S, h, stk) : 0;
-}
-
-__device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, bool live, uint32_t& mat,
-                                          bool& have_mat, LdsStack& stk) {
+template <class Stk>
+__device__ __forceinline__ int trace_step_shared(const SceneView& S, TraceMarch& m, bool live, uint32_t& mat,
+                                                 bool& have_mat, Stk& stk) {
     f3 p = spl(0.0f);
     int st = 0;
     bool need = false;
@@ -404,6 +425,39 @@ __device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, boo
     mat = h.mat;
     have_mat = true;
     return st;
+}
+
+__device__ __forceinline__ float query_dist(const SceneView& S, f3 p, LdsStack& stk, float ub = kFltMax) {
+    return wave_dist(S, p, true, stk, ub);
+}
+__device__ __forceinline__ float query_dist(const SceneView& S, f3 p, ExecStack& stk, float ub = kFltMax) {
+    return wave_dist(S, p, true, stk, ub);
+}
+__device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, bool live, float& vis,
+                                            LdsStack& stk) {
+    return shadow_step_shared(S, m, live, vis, stk);
+}
+__device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, bool live, float& vis,
+                                            ExecStack& stk) {
+    return shadow_step_shared(S, m, live, vis, stk);
+}
+__device__ __forceinline__ int miss_step(const SceneView& S, TraceMarch& m, bool live, LdsStack& stk) {
+    return miss_step_shared(S, m, live, stk);
+}
+__device__ __forceinline__ int miss_step(const SceneView& S, TraceMarch& m, bool live, ExecStack& stk) {
+    return miss_step_shared(S, m, live, stk);
+}
+__device__ __forceinline__ int miss_step(const SceneView& S, TraceMarch& m, bool live, CountStack& stk) {
+    DistHit h;
+    return live ? m.step(S, h, stk) : 0;
+}
+__device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, bool live, uint32_t& mat,
+                                          bool& have_mat, LdsStack& stk) {
+    return trace_step_shared(S, m, live, mat, have_mat, stk);
+}
+__device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, bool live, uint32_t& mat,
+                                          bool& have_mat, ExecStack& stk) {
+    return trace_step_shared(S, m, live, mat, have_mat, stk);
 }
 
 // Links per batch of a Jacobi row: the batch's index, weight and cluster
@@ -681,6 +735,28 @@ void Engine::launch_on(hipStream_t stream, const char* name, F&& f) {
                 K<false, true, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);       \
             else                                                                         \
                 K<false, false, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);      \
+        }                                                                                \
+    } while (0)
+
+// Three-way instrumentation of the persistent tracing kernels: MODE 0 timed,
+// 1 counting (CountStack), 2 executed-evaluation tally (ExecStack).
+#define LAUNCH_CM(K, MODE, LDS, GRID, DYN, ...)                                          \
+    do {                                                                                 \
+        if ((MODE) == 1) {                                                               \
+            if (LDS)                                                                     \
+                K<1, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);                  \
+            else                                                                         \
+                K<1, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);                 \
+        } else if ((MODE) == 2) {                                                        \
+            if (LDS)                                                                     \
+                K<2, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);                  \
+            else                                                                         \
+                K<2, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);                 \
+        } else {                                                                         \
+            if (LDS)                                                                     \
+                K<0, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);                  \
+            else                                                                         \
+                K<0, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);                 \
         }                                                                                \
     } while (0)
 

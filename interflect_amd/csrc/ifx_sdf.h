@@ -247,6 +247,19 @@ struct LdsStack {
     __device__ __forceinline__ void count_prim(uint32_t, uint32_t) {}
     __device__ __forceinline__ void count_aabb(uint32_t) {}
     __device__ __forceinline__ void count_eval() {}
+    __device__ __forceinline__ void count_wave_prims(uint32_t) {}
+};
+
+// LdsStack of the instrumented instantiation that tallies the primitive
+// evaluations the timed traversal executes, for comparison with the
+// reference traversal's (CountStack): `xe` per lane on the per-lane paths,
+// `xw` per wavefront for the shared traversal (each object it opens is
+// evaluated by every active lane).  A separate instantiation: the counters
+// cost the timed kernels registers.
+struct ExecStack : LdsStack {
+    uint32_t xe, xw;
+    __device__ __forceinline__ void count_prim(uint32_t, uint32_t) { ++xe; }
+    __device__ __forceinline__ void count_wave_prims(uint32_t lanes) { xw += lanes; }
 };
 
 struct CountStack {
@@ -256,6 +269,7 @@ struct CountStack {
     float* dbase;
     uint32_t stride;
     unsigned long long ops;
+    uint32_t prims;  // primitive evaluations of the reference traversal
     __device__ __forceinline__ void put(int i, uint32_t x, float d) {
         base[i * stride] = (uint16_t)x;
         dbase[i * stride] = d;
@@ -264,7 +278,10 @@ struct CountStack {
         d = dbase[i * stride];
         return base[i * stride];
     }
-    __device__ __forceinline__ void count_prim(uint32_t kind, uint32_t extra) { ops += kPrimOps[kind & 7u] + extra; }
+    __device__ __forceinline__ void count_prim(uint32_t kind, uint32_t extra) {
+        ops += kPrimOps[kind & 7u] + extra;
+        ++prims;
+    }
     __device__ __forceinline__ void count_aabb(uint32_t n) { ops += kAabbOps * n; }
     __device__ __forceinline__ void count_eval() { ops += kEvalOps; }
 };

@@ -28,6 +28,9 @@ inline uint64_t sort_scratch_words(uint64_t n) {
 inline uint64_t scan_scratch_words(uint64_t n) { return tiles(n) + 64; }
 
 namespace detail {
+// Internal linkage: every translation unit that includes this header keeps
+// its own copy of the kernels.
+namespace {
 
 __device__ __forceinline__ uint32_t lane() { return threadIdx.x & 63u; }
 
@@ -161,6 +164,7 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const K* __restrict__ keys
     }
 }
 
+}  // namespace
 }  // namespace detail
 
 // out[i] = sum of in[0..i) for i < n.  `scratch` holds scan_scratch_words(n).
@@ -200,6 +204,39 @@ inline hipError_t sort_pairs(K* keys, uint32_t* vals, K* keys_alt, uint32_t* val
         vi = vo;
         vo = tv;
         in_alt = !in_alt;
+    }
+    return hipGetLastError();
+}
+
+// Sorts (keys, vals) by the low `bits` bits of the keys, stably, into
+// (keys_out, vals_out), leaving the inputs untouched: the passes alternate
+// between the output pair and (keys_tmp, vals_tmp) so that the last one
+// lands in the output.  `scratch` holds sort_scratch_words(n).
+template <class K>
+inline hipError_t sort_pairs_to(const K* keys, const uint32_t* vals, K* keys_out, uint32_t* vals_out, K* keys_tmp,
+                                uint32_t* vals_tmp, uint64_t n, uint32_t bits, uint32_t* scratch, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint32_t passes = (bits + 7u) / 8u;
+    if (passes == 0) {
+        hipError_t e = hipMemcpyAsync(keys_out, keys, sizeof(K) * n, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        return hipMemcpyAsync(vals_out, vals, 4ull * n, hipMemcpyDeviceToDevice, s);
+    }
+    const uint32_t nt = tiles(n);
+    uint32_t* hist = scratch;
+    uint32_t* part = scratch + (uint64_t)nt * kDigits;
+    const K* ki = keys;
+    const uint32_t* vi = vals;
+    for (uint32_t p = 0; p < passes; ++p) {
+        const bool to_out = ((passes - 1u - p) & 1u) == 0u;
+        K* ko = to_out ? keys_out : keys_tmp;
+        uint32_t* vo = to_out ? vals_out : vals_tmp;
+        detail::k_hist<K><<<nt, kThreads, 0, s>>>(ki, n, 8u * p, nt, hist);
+        hipError_t e = exclusive_scan(hist, hist, (uint64_t)nt * kDigits, part, s);
+        if (e != hipSuccess) return e;
+        detail::k_scatter<K><<<nt, kThreads, 0, s>>>(ki, vi, n, 8u * p, nt, hist, ko, vo);
+        ki = ko;
+        vi = vo;
     }
     return hipGetLastError();
 }

@@ -13,8 +13,6 @@
 //      states, so the result does not depend on scheduling);
 //   5. the accepted seeds are compacted in seed order and cut at the target,
 //      which is where the reference's loop stops.
-#include <hipcub/hipcub.hpp>
-
 #include "ifx_kern.h"
 #include "ifx_surfel.h"
 
@@ -23,10 +21,15 @@ namespace {
 
 enum : uint8_t { S_OPEN = 0, S_KEEP = 1, S_DROP = 2 };
 
+// The placement kernels leave lanes at different points of their loops (a
+// seed's Newton steps stop when it converges; rejected seeds skip the
+// output), so they query the scene with the per-lane traversal (LaneStack):
+// the wavefront-shared query is for call sites where all lanes arrive
+// together.
 template <bool LDS>
 __global__ __launch_bounds__(kBlock) void k_seed_project(SceneView Sg, LightsView Lg, LdsPlan P, SurfelParams SP,
                                                          uint32_t b0, uint32_t nb, float4* pos, uint32_t* ok) {
-    IFX_SETUP(LDS, false, false);
+    IFX_SETUP_T(LDS, false, LaneStack);
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= nb) return;
     f3 p;
@@ -137,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_surfel_out(SceneView Sg, LightsView 
                                                        const float4* __restrict__ vpos, uint32_t target, float area,
                                                        float4* out_p, float4* out_n, float4* out_a,
                                                        uint32_t* out_m) {
-    IFX_SETUP(LDS, true, false);
+    IFX_SETUP_T(LDS, true, LaneStack);
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= nv || !flag[k]) return;
     const uint32_t j = at[k];
@@ -188,12 +191,8 @@ int Engine::generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std
                      d_pos.as<float4>(), d_ok.as<uint32_t>());
         });
     }
-    size_t tmp = 0;
-    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_ok.as<uint32_t>(), d_at.as<uint32_t>(), (int)(seeds + 1),
-                                        stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_ok.as<uint32_t>(), d_at.as<uint32_t>(), (int)(seeds + 1),
-                                        stream));
+    if (int rc_s = scan_u32(d_ok.as<uint32_t>(), d_at.as<uint32_t>(), (seeds + 1), err))
+        return rc_s;
     uint32_t nv = 0;
     CK(hipMemcpyAsync(&nv, d_at.as<uint32_t>() + seeds, 4, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
@@ -204,23 +203,16 @@ int Engine::generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std
                                                                    d_vcell.as<uint32_t>(), d_counts.as<uint32_t>());
     });
     // 2. bucket the survivors by cell (stable: seed order within a cell)
-    tmp = 0;
-    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_counts.as<uint32_t>(), d_starts.as<uint32_t>(),
-                                        (int)(ncells + 1), stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_counts.as<uint32_t>(), d_starts.as<uint32_t>(),
-                                        (int)(ncells + 1), stream));
+    if (int rc_s = scan_u32(d_counts.as<uint32_t>(), d_starts.as<uint32_t>(), (ncells + 1), err))
+        return rc_s;
     CK(d_keys2.ensure(4ull * nv));
     CK(d_items.ensure(4ull * nv));
     CK(d_items_in.ensure(4ull * nv));
     k_iota<<<blocks_for(nv, 256), 256, 0, stream>>>(d_items_in.as<uint32_t>(), nv);
     const int kb = (int)bits_for(ncells ? ncells - 1 : 1);
-    tmp = 0;
-    CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_vcell.as<uint32_t>(), d_keys2.as<uint32_t>(),
-                                          d_items_in.as<uint32_t>(), d_items.as<uint32_t>(), (int)nv, 0, kb, stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp, d_vcell.as<uint32_t>(), d_keys2.as<uint32_t>(),
-                                          d_items_in.as<uint32_t>(), d_items.as<uint32_t>(), (int)nv, 0, kb, stream));
+    if (int rc_s = sort_u32(d_vcell.as<uint32_t>(), d_items_in.as<uint32_t>(), d_keys2.as<uint32_t>(),
+                       d_items.as<uint32_t>(), nv, (uint32_t)kb, err))
+        return rc_s;
     // 3. the conflict graph (earlier neighbours of each survivor)
     CK(d_adjn.ensure(4ull * (nv + 1)));
     CK(d_adjs.ensure(4ull * (nv + 1)));
@@ -233,12 +225,8 @@ int Engine::generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std
             nv, d_cpos.as<float4>(), d_vcell.as<uint32_t>(), d_starts.as<uint32_t>(), d_items.as<uint32_t>(), SP,
             nullptr, d_adjn.as<uint32_t>());
     });
-    tmp = 0;
-    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_adjn.as<uint32_t>(), d_adjs.as<uint32_t>(), (int)(nv + 1),
-                                        stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_adjn.as<uint32_t>(), d_adjs.as<uint32_t>(), (int)(nv + 1),
-                                        stream));
+    if (int rc_s = scan_u32(d_adjn.as<uint32_t>(), d_adjs.as<uint32_t>(), (nv + 1), err))
+        return rc_s;
     uint32_t edges = 0;
     CK(hipMemcpyAsync(&edges, d_adjs.as<uint32_t>() + nv, 4, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
@@ -275,8 +263,10 @@ int Engine::generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std
         CK(hipStreamSynchronize(stream));
         rounds += 8;
         if (open[7] == 0) break;
-        if (rounds > 1000000u) {
-            err = "surfel acceptance did not settle";
+        // every round settles at least the lowest undecided seed (its earlier
+        // neighbours are all decided), so nv rounds always suffice
+        if (rounds > nv + 8u) {
+            err = "surfel acceptance did not settle within " + std::to_string(nv) + " rounds";
             return -4;
         }
     }
@@ -285,12 +275,8 @@ int Engine::generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std
     CK(d_at2.ensure(4ull * (nv + 1)));
     CK(hipMemsetAsync(d_flag.as<uint32_t>() + nv, 0, 4, stream));
     k_keep_flags<<<blocks_for(nv, 256), 256, 0, stream>>>(nv, st_in, d_flag.as<uint32_t>());
-    tmp = 0;
-    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d_flag.as<uint32_t>(), d_at2.as<uint32_t>(), (int)(nv + 1),
-                                        stream));
-    CK(d_tmp.ensure(tmp));
-    CK(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp, d_flag.as<uint32_t>(), d_at2.as<uint32_t>(), (int)(nv + 1),
-                                        stream));
+    if (int rc_s = scan_u32(d_flag.as<uint32_t>(), d_at2.as<uint32_t>(), (nv + 1), err))
+        return rc_s;
     uint32_t kept = 0;
     CK(hipMemcpyAsync(&kept, d_at2.as<uint32_t>() + nv, 4, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
