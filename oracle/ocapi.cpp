@@ -16,6 +16,7 @@ struct Ctx {
     bool have_tr = false;
     uint64_t ff_evals = 0;
     uint32_t spp = 512;
+    OpTally vis, direct, shade;  // operation tallies of the last transfer, solve and render
 };
 V3 ld(const float* p) { return v3(p[0], p[1], p[2]); }
 void st(float* p, V3 v) {
@@ -189,6 +190,7 @@ int orc_transfer(void* h, int32_t res, int threads, int64_t row_begin, int64_t r
     size_t rb = row_begin < 0 ? 0 : (size_t)row_begin;
     size_t re = row_end < 0 ? SIZE_MAX : (size_t)row_end;
     c->ff_evals = build_transfer(c->sc, c->surfels, std::move(cl), c->tr, threads, rb, re);
+    c->vis = c->tr.vis_tally;
     c->have_tr = true;
     stats[0] = (double)c->tr.nc;
     stats[1] = (double)c->tr.links;
@@ -227,7 +229,8 @@ void orc_get_clusters(void* h, float* p, float* n, float* area, float* radius, u
 int orc_solve(void* h, uint32_t bounces, int threads, double stats[3]) {
     Ctx* c = (Ctx*)h;
     if (!c->have_tr) return -1;
-    uint64_t ev = light_surfels(c->sc, c->surfels, threads);
+    c->direct = OpTally();
+    uint64_t ev = light_surfels(c->sc, c->surfels, threads, &c->direct);
     SolveStats s = solve(c->surfels, c->tr, bounces, threads);
     stats[0] = (double)ev;
     stats[1] = (double)s.iterations;
@@ -287,9 +290,21 @@ uint64_t orc_render(void* h, int mode, uint32_t w, uint32_t hh, const float* cam
         gip = &gi;
     }
     std::vector<V3> film;
-    uint64_t ev = render_frame(sc, cm, (Mode)mode, gip, tmax, w, hh, film, threads, y0, y1, c->spp);
+    c->shade = OpTally();
+    uint64_t ev = render_frame(sc, cm, (Mode)mode, gip, tmax, w, hh, film, threads, y0, y1, c->spp, &c->shade);
     for (size_t i = 0; i < film.size(); ++i) st(film_out + 3 * i, film[i]);
     return ev;
+}
+
+// Operation tallies (the roofline's counting rule) of the next transfers,
+// solves and renders: on = 1 starts tallying, 0 stops.  out = link shadow
+// marches, direct light, render: (ops, primitive evaluations) each.
+void orc_set_tally(int on) { g_tally_on = on != 0; }
+void orc_tallies(void* h, uint64_t out[6]) {
+    Ctx* c = (Ctx*)h;
+    out[0] = c->vis.ops, out[1] = c->vis.prims;
+    out[2] = c->direct.ops, out[3] = c->direct.prims;
+    out[4] = c->shade.ops, out[5] = c->shade.prims;
 }
 
 // Paths per pixel of the REFERENCE mode (--spp, src/main.rs:81).

@@ -219,6 +219,7 @@ uint64_t build_transfer(const Scene& sc, const std::vector<Surfel>& surfels, std
         std::vector<float> w;
         uint64_t evals = 0;
         float sum = 0, sky = 0;
+        OpTally vis;
     };
     std::vector<Row> rows(N);
     par_for(row_end > row_begin ? row_end - row_begin : 0, threads, [&](size_t rr) {
@@ -226,6 +227,7 @@ uint64_t build_transfer(const Scene& sc, const std::vector<Surfel>& surfels, std
         const Surfel& s = surfels[i];
         Row& row = rows[i];
         float sum = 0.0f;
+        TallyScope tally_vis(row.vis);
         for (size_t ci = 0; ci < nc; ++ci) {
             const Cluster& c = clusters[ci];
             V3 d = c.p - s.p;
@@ -248,6 +250,7 @@ uint64_t build_transfer(const Scene& sc, const std::vector<Surfel>& surfels, std
             row.w.push_back(v);
             sum += v;
         }
+        g_tally = tally_vis.prev;  // the closure is not part of the link tally
         uint32_t e = 0;
         float sky = hemisphere_closure(sc, s.p, s.n, diag * 2.0f, 16, e);
         row.evals += e;
@@ -264,7 +267,9 @@ uint64_t build_transfer(const Scene& sc, const std::vector<Surfel>& surfels, std
     size_t links = 0;
     uint64_t evals = 0;
     size_t interior = 0;
+    out.vis_tally = OpTally();
     for (const Row& r : rows) {
+        out.vis_tally.add(r.vis);
         links += r.idx.size();
         evals += r.evals;
         if (!r.idx.empty()) interior += 1;
@@ -296,10 +301,13 @@ uint64_t build_transfer(const Scene& sc, const std::vector<Surfel>& surfels, std
 }
 
 // ---------------------------------------------------------------- solve.rs:27-39
-uint64_t light_surfels(const Scene& sc, std::vector<Surfel>& surfels, int threads) {
+uint64_t light_surfels(const Scene& sc, std::vector<Surfel>& surfels, int threads, OpTally* tally) {
     std::vector<uint64_t> ev(surfels.size(), 0);
+    std::vector<OpTally> tl(tally ? surfels.size() : 0);
     par_for(surfels.size(), threads, [&](size_t i) {
         Surfel& s = surfels[i];
+        OpTally none;
+        TallyScope ts(tally ? tl[i] : none);
         uint32_t e = 0;
         V3 c = direct_diffuse(sc, s.p, s.n, s.albedo, e);
         s.direct = c;
@@ -309,6 +317,8 @@ uint64_t light_surfels(const Scene& sc, std::vector<Surfel>& surfels, int thread
     });
     uint64_t tot = 0;
     for (uint64_t e : ev) tot += e;
+    if (tally)
+        for (const OpTally& t : tl) tally->add(t);
     return tot;
 }
 
@@ -445,14 +455,18 @@ V3 ref_path(const Scene& sc, V3 ro, V3 rd, float tmax, uint32_t max_depth, Rng& 
 }
 
 uint64_t render_frame(const Scene& sc, const Cam& cam, Mode mode, const Gi* gi, float tmax, uint32_t w,
-                      uint32_t h, std::vector<V3>& film, int threads, uint32_t y0, uint32_t y1, uint32_t spp) {
+                      uint32_t h, std::vector<V3>& film, int threads, uint32_t y0, uint32_t y1, uint32_t spp,
+                      OpTally* tally) {
     film.assign((size_t)w * h, splat(0.0f));
     y1 = std::min(y1, h);
     std::vector<uint64_t> row_evals(h, 0);
+    std::vector<OpTally> row_tally(tally ? h : 0);
     bool tonemapped = mode == BEAUTY || mode == DIRECT || mode == INDIRECT || mode == REFERENCE;
     par_for(y1 > y0 ? y1 - y0 : 0, threads, [&](size_t yy) {
         uint32_t y = y0 + (uint32_t)yy;
         uint64_t local = 0;
+        OpTally none;
+        TallyScope ts(tally ? row_tally[y] : none);
         for (uint32_t x = 0; x < w; ++x) {
             V3& px = film[(size_t)y * w + x];
             V3 rd = cam.ray(x, y, w, h);
@@ -533,6 +547,8 @@ uint64_t render_frame(const Scene& sc, const Cam& cam, Mode mode, const Gi* gi, 
     });
     uint64_t tot = 0;
     for (uint64_t e : row_evals) tot += e;
+    if (tally)
+        for (const OpTally& t : row_tally) tally->add(t);
     return tot;
 }
 

@@ -54,6 +54,7 @@ struct Transfer {
     size_t nc = 0, links = 0;
     // Per-row closure sky fraction and final row sum (diagnostics only).
     std::vector<float> sky, row_sum;
+    OpTally vis_tally;  // the link shadow marches (when tallying is on)
 };
 
 int normal_bucket(V3 n);
@@ -64,7 +65,7 @@ uint64_t build_transfer(const Scene& sc, const std::vector<Surfel>& s, std::vect
                         Transfer& out, int threads, size_t row_begin = 0, size_t row_end = SIZE_MAX);
 
 // src/solve.rs:27-108
-uint64_t light_surfels(const Scene& sc, std::vector<Surfel>& s, int threads);
+uint64_t light_surfels(const Scene& sc, std::vector<Surfel>& s, int threads, OpTally* tally = nullptr);
 struct SolveStats {
     uint32_t iterations;
     float residual;
@@ -112,7 +113,7 @@ struct Gi {
 // sample is asked for.
 uint64_t render_frame(const Scene& sc, const Cam& cam, Mode mode, const Gi* gi, float tmax, uint32_t w,
                       uint32_t h, std::vector<V3>& film, int threads, uint32_t y0 = 0, uint32_t y1 = UINT32_MAX,
-                      uint32_t spp = 512);
+                      uint32_t spp = 512, OpTally* tally = nullptr);
 
 // src/film.rs:8-61
 uint8_t srgb8(float c);

@@ -245,12 +245,16 @@ void Bvh::build(const std::vector<Object>& objects) {
     build_recursive(nodes, order, boxes, 0, 0, objects.size());
 }
 
+thread_local OpTally* g_tally = nullptr;
+bool g_tally_on = false;
+
 // bvh.rs:97-150
 void Bvh::query(V3 p, const std::vector<Object>& objects, DistHit& best) const {
     if (empty) return;
     uint32_t stack[64];
     size_t sp = 0;
     stack[sp++] = 0;
+    if (g_tally) g_tally->ops += kTallyAabbOps;
     while (sp > 0) {
         sp -= 1;
         size_t ni = stack[sp];
@@ -269,6 +273,7 @@ void Bvh::query(V3 p, const std::vector<Object>& objects, DistHit& best) const {
             size_t l = ni + 1, r = n.right;
             float dl = aabb_dist(p, nodes[l].lo, nodes[l].hi);
             float dr = aabb_dist(p, nodes[r].lo, nodes[r].hi);
+            if (g_tally) g_tally->ops += 2u * kTallyAabbOps;
             if (dl < dr) {
                 stack[sp++] = (uint32_t)r;
                 stack[sp++] = (uint32_t)l;
@@ -320,6 +325,7 @@ void Bvh::query_blend(V3 p, const std::vector<Object>& objects, float k, DistHit
     uint32_t stack[64];
     size_t sp = 0;
     stack[sp++] = 0;
+    if (g_tally) g_tally->ops += kTallyAabbOps;
     while (sp > 0) {
         sp -= 1;
         size_t ni = stack[sp];
@@ -335,6 +341,7 @@ void Bvh::query_blend(V3 p, const std::vector<Object>& objects, float k, DistHit
             size_t l = ni + 1, r = n.right;
             float dl = aabb_dist(p, nodes[l].lo, nodes[l].hi);
             float dr = aabb_dist(p, nodes[r].lo, nodes[r].hi);
+            if (g_tally) g_tally->ops += 2u * kTallyAabbOps;
             if (dl < dr) {
                 stack[sp++] = (uint32_t)r;
                 stack[sp++] = (uint32_t)l;
@@ -349,6 +356,7 @@ void Bvh::query_blend(V3 p, const std::vector<Object>& objects, float k, DistHit
 // ---------------------------------------------------------------- scene.rs:191-236
 DistHit Scene::dist_mat(V3 p) const {
     DistHit best{FLT_MAX, 0};
+    if (g_tally) g_tally->ops += kTallyEvalOps;
     for (const Object& o : planes) {
         float d = o.dist(p);
         if (d < best.d) best = DistHit{d, o.mat};

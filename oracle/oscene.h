@@ -24,6 +24,29 @@ struct Prim {
     bool bounds(V3& lo, V3& hi) const;      // src/sdf.rs:127-148
 };
 
+// Operation tally of the roofline's counting rule (test instrumentation, no
+// counterpart in the reference): FP32 operations per primitive kind, +2 per
+// CSG subtraction, 18 per AABB distance (the root once, then both children
+// of every interior node expanded) and 15 per scene-distance sample, plus
+// the number of primitive evaluations.  Accumulated on the calling thread
+// while g_tally points at a tally (TallyScope sets it when tallying is on).
+struct OpTally {
+    uint64_t ops = 0, prims = 0;
+    void add(const OpTally& o) {
+        ops += o.ops;
+        prims += o.prims;
+    }
+};
+extern thread_local OpTally* g_tally;
+extern bool g_tally_on;
+static const uint32_t kTallyPrimOps[7] = {10u, 23u, 6u, 24u, 13u, 40u, 57u};
+static const uint32_t kTallyAabbOps = 18u, kTallyEvalOps = 15u;
+struct TallyScope {
+    OpTally* prev;
+    explicit TallyScope(OpTally& t) : prev(g_tally) { g_tally = g_tally_on ? &t : nullptr; }
+    ~TallyScope() { g_tally = prev; }
+};
+
 // src/sdf.rs:151-173
 struct Object {
     Prim prim;
@@ -32,7 +55,11 @@ struct Object {
     float blend = 0.0f;
     float dist(V3 p) const {
         float d = prim.dist(p);
-        for (const Prim& s : subtract) d = fmax_(d, -s.dist(p));
+        if (g_tally) g_tally->ops += kTallyPrimOps[prim.kind], g_tally->prims += 1;
+        for (const Prim& s : subtract) {
+            d = fmax_(d, -s.dist(p));
+            if (g_tally) g_tally->ops += kTallyPrimOps[s.kind] + 2u, g_tally->prims += 1;
+        }
         return d;
     }
 };

@@ -93,6 +93,8 @@ def lib():
     L.orc_radical_inverse.argtypes = [C.c_uint32, C.c_uint32]
     L.orc_frame.argtypes = [f32p, f32p, f32p]
     L.orc_set_spp.argtypes = [vp, C.c_uint32]
+    L.orc_set_tally.argtypes = [C.c_int]
+    L.orc_tallies.argtypes = [vp, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
     L.orc_fresnel.argtypes = [C.c_float, f32p, f32p]
     L.orc_d_ggx.restype = C.c_float
     L.orc_d_ggx.argtypes = [C.c_float, C.c_float]
@@ -301,6 +303,15 @@ class Scene:
                               1 if use_gi else 0, self.threads, y0, y1, film)
         return film.reshape(h, w, 3), int(ev)
 
+    def tallies(self):
+        """(ops, primitive evaluations) of the last transfer's link shadow
+        marches, solve's direct light and render, under the roofline's
+        counting rule, when tallying is on (set_tally)."""
+        out = np.zeros(6, np.uint64)
+        lib().orc_tallies(self.h, out)
+        return {"visibility": (int(out[0]), int(out[1])), "direct_e": (int(out[2]), int(out[3])),
+                "shade": (int(out[4]), int(out[5]))}
+
     def direct(self, p, n, v, albedo, roughness, metallic):
         """shade::direct (src/shade.rs:50-125) at one point: (radiance, shadow steps)."""
         out = np.zeros(3, np.float32)
@@ -343,6 +354,12 @@ def prim_bounds(kind, params):
 
 def quad_ff(verts, p, n):
     return float(lib().orc_quad_ff(_f(verts), _f(p), _f(n)))
+
+
+def set_tally(on):
+    """Tally FP32 operations and primitive evaluations (the roofline's
+    counting rule) in the following transfers, solves and renders."""
+    lib().orc_set_tally(1 if on else 0)
 
 
 def fresnel(cos_theta, f0):
