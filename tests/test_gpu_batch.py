@@ -14,6 +14,7 @@ import pytest
 import interflect_amd as ia
 from conftest import scene_text
 from interflect_amd import scenes
+from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
@@ -86,6 +87,34 @@ def test_batch_matches_single_scene_runs(batch):
         img = b.image(s)
         assert np.array_equal(bits(img["film"]), bits(sh["film"]))
         assert np.array_equal(img["rgb8"], sh["rgb8"])
+
+
+def test_batch_scenes_match_the_oracle(batch):
+    """Each scene of the batch against the oracle on the same surfels (the
+    oracle's own placement, which equals the library's): every transfer row
+    exact, iterations equal, radiosity within 1e-4, image within 1/255 on
+    >= 99.9 % of pixels."""
+    b, texts = batch
+    for s, text in enumerate(texts):
+        say("batch scene vs oracle", s)
+        sc = po.Scene(text)
+        sc.generate()
+        st = b.stats(s)
+        assert st["surfels"] == int(sc.surfels()[0].shape[0])
+        to = sc.transfer(8)
+        so = sc.solve()
+        assert (st["clusters"], st["links"], st["iterations"]) == (to["clusters"], to["links"], so["iterations"])
+        assert st["evals_transfer"] == to["evals"]
+        starts, idx, w = b.transfer_arrays(s)
+        s1, i1, w1, _, _ = sc.transfer_arrays()
+        assert np.array_equal(starts, s1) and np.array_equal(idx, i1) and np.array_equal(bits(w), bits(w1))
+        for x, y in zip(b.radiosity(s), sc.radiosity()):
+            assert np.linalg.norm(x - y) <= 1e-4 * max(np.linalg.norm(y), 1e-30)
+        film, ev = sc.render("beauty")
+        img = b.image(s)
+        diff = np.abs(img["rgb8"].astype(np.int32) - po.encode(film, sc.exposure, True).astype(np.int32)).max(axis=2)
+        assert np.mean(diff <= 1) >= 0.999, (s, np.mean(diff <= 1), diff.max())
+        assert abs(st["evals_shade"] - ev) <= 1e-4 * ev
 
 
 def test_graph_replays_reproduce_the_eager_frame(batch):

@@ -14,6 +14,7 @@ import pytest
 
 import interflect_amd as ia
 from conftest import scene_text
+from interflect_amd import scenes
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
@@ -41,7 +42,7 @@ def pair(name):
 
 
 # --------------------------------------------------------------------------- kernel family 1+2: SDF + BVH
-@pytest.mark.parametrize("name", ["product.rad", "stress.rad", "cornell_bench.rad", "dog.rad"])
+@pytest.mark.parametrize("name", ["product.rad", "stress.rad", "cornell_bench.rad", "dog.rad", "TEMPLATE.rad"])
 def test_dist_mat_and_normals_bitexact(name):
     say("dist_mat", name)
     ctx, sc = pair(name)
@@ -81,7 +82,7 @@ def camera_rays(sc, n, seed):
     return ro, rd
 
 
-@pytest.mark.parametrize("name", ["product.rad", "stress.rad"])
+@pytest.mark.parametrize("name", ["product.rad", "stress.rad", "TEMPLATE.rad"])
 def test_trace_bitexact(name):
     say("trace", name)
     ctx, sc = pair(name)
@@ -97,7 +98,7 @@ def test_trace_bitexact(name):
     assert np.array_equal(g[3][h], o[3][h])
 
 
-@pytest.mark.parametrize("name", ["product.rad", "stress.rad", "flush_ceiling_light.rad"])
+@pytest.mark.parametrize("name", ["product.rad", "stress.rad", "flush_ceiling_light.rad", "TEMPLATE.rad"])
 def test_soft_shadow_bitexact(name):
     say("soft_shadow", name)
     ctx, sc = pair(name)
@@ -112,6 +113,51 @@ def test_soft_shadow_bitexact(name):
     v, s = ctx.soft_shadows(ro, rd, 0.015, dist, k)
     v0, s0 = sc.soft_shadow(ro, rd, 0.015, dist, k)
     assert np.array_equal(bits(v), bits(v0)) and np.array_equal(s, s0)
+
+
+# TEMPLATE.rad's capped cones and cylinder (src/sdf.rs:79-123): the only
+# reference scene with them.  Points and rays concentrated around each.
+TEMPLATE_SOLIDS = [  # (a, b, radius) of the cylinder and the two cones
+    ((-1.6, 0.0, 1.4), (-1.6, 0.55, 1.4), 0.3),
+    ((0.6, 0.0, -1.5), (0.6, 1.1, -1.5), 0.4),
+    ((2.1, 0.0, 0.9), (2.1, 0.6, 0.9), 0.42),
+]
+
+
+def test_cone_and_cylinder_neighbourhoods_bitexact():
+    say("cones and cylinder")
+    ctx, sc = pair("TEMPLATE.rad")
+    rng = np.random.default_rng(31)
+    pts, ro, rd, tmx = [], [], [], []
+    for a, b, r in TEMPLATE_SOLIDS:
+        lo = np.minimum(a, b) - (r + 0.25)
+        hi = np.maximum(a, b) + (r + 0.25)
+        p = (lo + (hi - lo) * rng.random((30000, 3))).astype(np.float32)
+        pts.append(p)
+        # rays from around the solid through its axis
+        o = (lo - 1.0 + (hi - lo + 2.0) * rng.random((10000, 3))).astype(np.float32)
+        t = (np.array(a) + (np.array(b) - np.array(a)) * rng.random((10000, 1))).astype(np.float32)
+        d = t - o
+        dist = np.linalg.norm(d, axis=1).astype(np.float32)
+        ro.append(o)
+        rd.append((d / dist[:, None]).astype(np.float32))
+        tmx.append(dist * 2)
+    pts = np.concatenate(pts)
+    d, m, nrm = ctx.dist_mat(pts, normals=True)
+    d0, m0 = sc.dist_mat(pts)
+    assert np.array_equal(bits(d), bits(d0)) and np.array_equal(m, m0)
+    assert np.array_equal(bits(nrm), bits(sc.normal(pts)))
+    ro, rd, tmx = np.concatenate(ro), np.concatenate(rd), np.concatenate(tmx).astype(np.float32)
+    g = ctx.trace_rays(ro, rd, 1e-3, np.float32(sc.diag * 4))
+    o = sc.trace(ro, rd, 1e-3, np.float32(sc.diag * 4))
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[4], o[4])
+    h = o[0]
+    assert h.mean() > 0.5
+    assert np.array_equal(bits(g[1][h]), bits(o[1][h])) and np.array_equal(g[3][h], o[3][h])
+    k = rng.uniform(1.5, 96, len(ro)).astype(np.float32)
+    v, st = ctx.soft_shadows(ro, rd, 0.015, tmx, k)
+    v0, st0 = sc.soft_shadow(ro, rd, 0.015, tmx, k)
+    assert np.array_equal(bits(v), bits(v0)) and np.array_equal(st, st0)
 
 
 # --------------------------------------------------------------------------- kernel family 4: transfer
@@ -157,6 +203,134 @@ def test_transfer_links_exact(name, target):
     say("transfer", name)
     ctx, sc, tg, to = run_both(name, target)
     compare_transfer(ctx, sc, tg, to)
+
+
+def test_template_pipeline_matches_oracle():
+    """TEMPLATE.rad (cones, cylinder, rough and shiny dielectrics, a metallic
+    mirror): all transfer rows exact, radiosity within 1e-4, beauty (mirror
+    branch and GGX lobe) within 1/255 on >= 99.9 % of pixels."""
+    say("TEMPLATE pipeline")
+    ctx, sc, tg, to = run_both("TEMPLATE.rad", 6000)
+    compare_transfer(ctx, sc, tg, to)
+    g, o = ctx.solve(), sc.solve()
+    assert g["iterations"] == o["iterations"] and g["evals"] == o["evals"]
+    _, _, rg = ctx.radiosity()
+    _, _, ro_ = sc.radiosity()
+    assert np.linalg.norm(rg - ro_) <= 1e-4 * np.linalg.norm(ro_)
+    r = ctx.shade("beauty", 320, 240)
+    film, ev = sc.render("beauty", 320, 240)
+    frac, worst = image_close(r["rgb8"], po.encode(film, sc.exposure, True))
+    assert frac >= 0.999, (frac, worst)
+    assert abs(r["evals"] - ev) <= 1e-4 * ev
+
+
+def test_cornell_unit_full_size_matches_oracle():
+    """Configuration A at full size: the unit Cornell box, 20000 surfels
+    requested, 260 x 260, 12 bounces.  Every transfer row exact, iterations
+    and evaluation counts equal, radiosity within 1e-4, image within 1/255
+    on >= 99.9 % of pixels (the reference CI's gates apply to this scene
+    family: .github/workflows/ci.yml:129-149)."""
+    say("cornell unit full")
+    text = scenes.cornell_unit()
+    sc = po.Scene(text)
+    sc.generate()
+    p, n, a, ar = sc.surfels()
+    ctx = ia.Context(0)
+    ctx.load_scene(text)
+    assert ctx.generate_surfels() == len(p)
+    for x, y in zip(ctx.surfels(), (p, n, a, ar)):
+        assert np.array_equal(bits(x), bits(y))
+    tg, to = ctx.transfer(8), sc.transfer(8)
+    compare_transfer(ctx, sc, tg, to)
+    assert 0.7 < to["mean_row_sum"] <= 1.0
+    g, o = ctx.solve(), sc.solve()
+    assert g["iterations"] == o["iterations"] and g["evals"] == o["evals"]
+    for x, y in zip(ctx.radiosity(), sc.radiosity()):
+        assert np.linalg.norm(x - y) <= 1e-4 * np.linalg.norm(y)
+    r = ctx.shade("beauty")
+    film, ev = sc.render("beauty")
+    assert r["rgb8"].shape == (260, 260, 3)
+    frac, worst = image_close(r["rgb8"], po.encode(film, sc.exposure, True))
+    assert frac >= 0.999, (frac, worst)
+    assert abs(r["evals"] - ev) <= 1e-4 * ev
+
+
+def _bytes_of(ctx, solve_first=True):
+    return [np.asarray(x).view(np.uint8).copy() for x in ctx.transfer_arrays() + ctx.radiosity()]
+
+
+@pytest.mark.parametrize("side", [0, 1])
+def test_parameter_updates_and_repeated_transfers_with_the_side_stream(side):
+    """The closure and direct-light terms run on a second stream after the
+    transfer.  Updating a light's emission or a material's albedo between
+    transfer and solve, or running two transfers in a row, must give the
+    bytes and counts of the same calls on one stream."""
+    say("side stream updates", side)
+
+    def run(side_stream, seq):
+        ctx = ia.Context(0)
+        ctx.load_scene(scene_text("product.rad"))
+        ctx.generate_surfels(5000)
+        ctx.set_tuning("side_stream", side_stream)
+        out = []
+        for step in seq:
+            if step == "transfer":
+                out.append(ctx.transfer(8)["evals"])
+            elif step == "emit":
+                ctx.set_light_emit(0, np.array([3.0, 5.0, 7.0], np.float32))
+            elif step == "albedo":
+                ctx.set_material_albedo(1, np.array([0.3, 0.6, 0.2], np.float32))
+            elif step == "solve":
+                s = ctx.solve()
+                out.append((s["iterations"], s["evals"]))
+            elif step == "shade":
+                r = ctx.shade("beauty", 96, 72)
+                out.append(r["evals"])
+                out.append(r["rgb8"].copy())
+        return out, _bytes_of(ctx)
+
+    for seq in (["transfer", "emit", "solve", "shade"], ["transfer", "albedo", "solve", "shade"],
+                ["transfer", "transfer", "solve", "shade"]):
+        a, ab = run(side, seq)
+        b, bb = run(0, seq)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), seq
+        assert all(np.array_equal(x, y) for x, y in zip(ab, bb)), seq
+
+
+@pytest.mark.parametrize("name,target", [("closed_box.rad", 3000), ("product.rad", 5000), ("TEMPLATE.rad", 3000)])
+def test_counted_operations_equal_the_oracle_tally(name, target):
+    """The roofline numerator: the counting instantiation's FP32-operation and
+    primitive-evaluation tallies of the link visibility, direct light and
+    beauty shade equal, as integers, the oracle's tally of the reference
+    traversal under the same counting rule (so the kernels visit exactly the
+    reference's nodes and objects)."""
+    say("op tally", name)
+    src = scene_text(name)
+    sc = po.Scene(src)
+    sc.generate(target)
+    p, n, a, ar = sc.surfels()
+    ctx = ia.Context(0)
+    ctx.load_scene(src)
+    ctx.set_surfels(p, n, a, ar)
+    po.set_tally(True)
+    try:
+        sc.transfer(8)
+        sc.solve()
+        sc.render("beauty", 128, 96)
+    finally:
+        po.set_tally(False)
+    want = sc.tallies()
+    ctx.reset_kernel_times()
+    ctx.set_counting(True)
+    ctx.transfer(8)
+    ctx.solve()
+    ctx.shade("beauty", 128, 96)
+    ctx.set_counting(False)
+    for k in ("visibility", "direct_e", "shade"):
+        got = (ctx.kernel_ops(k), ctx.kernel_ops(k + ".prims"))
+        assert got == want[k], (k, got, want[k])
+        assert got[0] > 0
 
 
 def test_transfer_stress_rows_exact():
@@ -283,7 +457,6 @@ def test_repeat_runs_are_bit_identical():
 
 
 # --------------------------------------------------------------------------- extension scenes
-from interflect_amd import scenes  # noqa: E402
 
 
 def pair_text(text):
