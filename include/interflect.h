@@ -61,7 +61,8 @@ typedef struct {
     uint64_t links;
     uint64_t evals;      /* SDF evaluations under the reference's counting rule */
     float mean_row_sum;
-    float ms;            /* wall time of the stage, H2D of the surfels included */
+    float ms;            /* wall time of the stage, including the upload of surfels set or placed since the
+                            last transfer (later transfers on the same surfels upload nothing) */
 } ifx_transfer_stats;
 
 typedef struct {

@@ -261,6 +261,7 @@ class Engine {
     DevBuf d_ckey, d_ckey2, d_cval, d_cval2, d_cflag, d_cseg, d_cpos, d_cnrm, d_crange, d_cvalid, d_cidx;
     DevBuf d_craw_pos, d_craw_nrm;
     // work order (Morton rows, cluster-major candidates) and queues
+    DevBuf d_mpos, d_mnrm, d_mrank;  // surfel positions / normals in Morton order, each surfel's rank
     DevBuf d_morton, d_mval, d_mkey_s, d_rorder, d_vkey, d_vval, d_vkey_s, d_vorder, d_queue, d_open;
     // candidates and links
     DevBuf d_ccount, d_cstart, d_cand_ci, d_cand_row, d_cand_v, d_sky, d_lcount, d_lstart, d_scale, d_rowsum, d_idx, d_w;

@@ -274,6 +274,19 @@ __global__ void k_morton(const float4* __restrict__ spos, uint32_t N, f3 lo, f3 
     iota[i] = i;
 }
 
+// Surfel records in Morton order (rank r holds surfel rorder[r]) and each
+// surfel's rank: the tracing kernels visit surfels in this order, so their
+// record loads are coalesced.
+__global__ void k_morton_copy(uint32_t N, const uint32_t* __restrict__ rorder, const float4* __restrict__ spos,
+                              const float4* __restrict__ snrm, float4* mpos, float4* mnrm, uint32_t* mrank) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const uint32_t i = rorder[r];
+    mpos[r] = spos[i];
+    mnrm[r] = snrm[i];
+    mrank[i] = r;
+}
+
 // Cone-traced link visibility (src/formfactor.rs:194-214) as a persistent
 // kernel: every lane marches one candidate's shadow ray a sample at a time
 // and takes the next candidate from the queue as soon as its ray ends, so
@@ -336,12 +349,12 @@ __global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView 
 // side stream's closure and direct-light launches (measured, see transfer()).
 constexpr uint32_t kVisOverlapMax = 1u << 23;
 
-// Visiting order <-> row-major candidates: the row of each visited
-// candidate, and its result back to the candidate's row-major slot.
+// Visiting order <-> row-major candidates: the Morton rank of each visited
+// candidate's row, and its result back to the candidate's row-major slot.
 __global__ void k_vis_order(uint32_t M, const uint32_t* __restrict__ order, const uint32_t* __restrict__ cand_row,
-                            uint32_t* vrow) {
+                            const uint32_t* __restrict__ mrank, uint32_t* vrow) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < M) vrow[q] = cand_row[order[q]];
+    if (q < M) vrow[q] = mrank[cand_row[order[q]]];
 }
 
 __global__ void k_vis_scatter(uint32_t M, const uint32_t* __restrict__ order, const float* __restrict__ vis,
@@ -352,13 +365,14 @@ __global__ void k_vis_scatter(uint32_t M, const uint32_t* __restrict__ order, co
 
 // Hemisphere closure (src/trace.rs:165-186), persistent like k_visibility.
 // Item q = direction-major over surfels in Morton order, so a wavefront
-// traces one Halton direction from 64 neighbouring surfels; a missed ray
-// adds one to its surfel's integer open count.  dirs[k] = (r cos phi,
-// r sin phi, sqrt(1 - u1)).
+// traces one Halton direction from 64 neighbouring surfels (whose records
+// mpos / mnrm hold in that order, read coalesced); a missed ray adds one to
+// its surfel's integer open count.  dirs[k] = (r cos phi, r sin phi,
+// sqrt(1 - u1)).
 template <int MODE, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg, LdsPlan P,
-                                                    const float4* __restrict__ spos,
-                                                    const float4* __restrict__ snrm, uint32_t N,
+                                                    const float4* __restrict__ mpos,
+                                                    const float4* __restrict__ mnrm, uint32_t N,
                                                     const uint32_t* __restrict__ row_order,
                                                     const float4* __restrict__ dirs, float tmax, uint32_t* open,
                                                     uint32_t* queue, uint32_t chunk, unsigned long long* evals,
@@ -374,9 +388,9 @@ __global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg,
     for (;;) {
         uint32_t q;
         if (refill(live, wq, queue, total, q)) {
-            uint32_t k = q / N;
-            row = row_order[q - k * N];
-            f3 p = ld3(spos[row]), n = ld3(snrm[row]);
+            const uint32_t k = q / N, r = q - k * N;
+            row = row_order[r];
+            f3 p = ld3(mpos[r]), n = ld3(mnrm[r]);
             f3 t, b;
             frame(n, t, b);
             float4 D = dirs[k];
@@ -458,10 +472,12 @@ __global__ void k_compact(uint32_t N, const uint32_t* __restrict__ cstart, const
 // Direct light E, one lane per (light, surfel) pair: the cone shadow of each
 // light runs in parallel and k_direct_sum adds the kept terms per surfel in
 // the reference's light order (src/shade.rs:137-162, src/solve.rs:27-39).
+// Lanes take the surfels in Morton order (positions and normals from the
+// Morton-ordered copies mpos / mnrm); terms are stored by surfel index.
 template <bool COUNT, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_direct_terms(SceneView Sg, LightsView Lg, LdsPlan P,
-                                                         const float4* __restrict__ spos,
-                                                         const float4* __restrict__ snrm,
+                                                         const float4* __restrict__ mpos,
+                                                         const float4* __restrict__ mnrm,
                                                          const float4* __restrict__ salb, uint32_t N, uint32_t nl,
                                                          const uint32_t* __restrict__ row_order, float4* terms,
                                                          float* gterm, unsigned long long* evals,
@@ -470,8 +486,8 @@ __global__ __launch_bounds__(kBlock) void k_direct_terms(SceneView Sg, LightsVie
     uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     uint32_t steps = 0;
     if (t < N * nl) {
-        uint32_t li = t / N, i = row_order[t - li * N];
-        f3 p = ld3(spos[i]), n = ld3(snrm[i]);
+        const uint32_t li = t / N, r = t - li * N, i = row_order[r];
+        f3 p = ld3(mpos[r]), n = ld3(mnrm[r]);
         f3 tan, bit;
         frame(n, tan, bit);
         f3 term = spl(0.0f);
@@ -1119,6 +1135,14 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         if (int rc_s = sort_u32(d_morton.as<uint32_t>(), d_mval.as<uint32_t>(), d_mkey_s.as<uint32_t>(),
                            d_rorder.as<uint32_t>(), N, 15u, err))
             return rc_s;
+        CK(d_mpos.ensure(16ull * N));
+        CK(d_mnrm.ensure(16ull * N));
+        CK(d_mrank.ensure(4ull * N));
+        launch("morton", [&] {
+            k_morton_copy<<<blocks_for(N, 256), 256, 0, stream>>>(N, d_rorder.as<uint32_t>(), spos, snrm,
+                                                                  d_mpos.as<float4>(), d_mnrm.as<float4>(),
+                                                                  d_mrank.as<uint32_t>());
+        });
     }
     CK(d_queue.ensure(16));
     CK(hipMemsetAsync(d_queue.p, 0, 16, stream));
@@ -1159,7 +1183,8 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         const uint32_t grid = std::max(1u, std::min<uint32_t>(persist, blocks_for(16ull * N, kBlock)));
         launch_on(cs, "closure", [&] {
             hipStream_t stream = cs;  // the launch macro's stream
-            LAUNCH_CM(k_closure, tally_mode(), ld.staged, grid, ld.bytes, S, Lg, ld.plan, spos, snrm, N,
+            LAUNCH_CM(k_closure, tally_mode(), ld.staged, grid, ld.bytes, S, Lg, ld.plan, d_mpos.as<float4>(),
+                      d_mnrm.as<float4>(), N,
                       d_rorder.as<uint32_t>(), d_dirs.as<float4>(), diag * 2.0f, d_open.as<uint32_t>(), queue + 1,
                       wave_chunk(16ull * N, grid), ctr, tally() ? ctr + 2 : nullptr);
         });
@@ -1293,11 +1318,13 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         if (sort_cand) {
             launch("vis_order", [&] {
                 k_vis_order<<<blocks_for(M, 256), 256, 0, stream>>>(M, d_vorder.as<uint32_t>(),
-                                                                    d_cand_row.as<uint32_t>(), vrow);
+                                                                    d_cand_row.as<uint32_t>(),
+                                                                    d_mrank.as<uint32_t>(), vrow);
             });
         }
         launch("visibility", [&] {
-            LAUNCH_CM(k_visibility, tally_mode(), ld.staged, visb, ld.bytes, S, Lg, ld.plan, spos, snrm,
+            LAUNCH_CM(k_visibility, tally_mode(), ld.staged, visb, ld.bytes, S, Lg, ld.plan,
+                      sort_cand ? d_mpos.as<float4>() : spos, sort_cand ? d_mnrm.as<float4>() : snrm,
                       d_cpos.as<float4>(), d_cnrm.as<float4>(),
                       sort_cand ? d_vkey_s.as<uint32_t>() : d_cand_ci.as<uint32_t>(), sort_cand ? 15u : 0u,
                       sort_cand ? vrow : d_cand_row.as<uint32_t>(), M, diag * 5e-4f, diag * 1e-3f, vis, queue,
@@ -1372,7 +1399,7 @@ int Engine::launch_direct(hipStream_t s, unsigned long long* evals, unsigned lon
     hipStream_t stream = s;  // the launch macro's stream
     launch_on(stream, "direct_e", [&] {
         LAUNCH_CL(k_direct_terms, count_ops_, ld.staged, blocks_for((uint64_t)N * nl, kBlock), ld.bytes, S, Lv,
-                  ld.plan, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, nl,
+                  ld.plan, d_mpos.as<float4>(), d_mnrm.as<float4>(), d_salb.as<float4>(), N, nl,
                   d_rorder.as<uint32_t>(), d_terms.as<float4>(), d_gterm.as<float>(), evals, ops);
     });
     return 0;
@@ -1548,7 +1575,7 @@ int Engine::solve_noalloc(uint32_t bounces, float4* direct, float4* irr, float4*
         if (nl) {
             const Lds ld = plan_lds(true);
             LAUNCH_CL(k_direct_terms, false, ld.staged, blocks_for((uint64_t)N * nl, kBlock), ld.bytes, S, Lv,
-                      ld.plan, d_spos.as<float4>(), d_snrm.as<float4>(), d_salb.as<float4>(), N, nl,
+                      ld.plan, d_mpos.as<float4>(), d_mnrm.as<float4>(), d_salb.as<float4>(), N, nl,
                       d_rorder.as<uint32_t>(), terms, gterm, ctr, nullptr);
         }
         k_direct_sum<<<blocks_for(N, 256), 256, 0, stream>>>(N, nl, terms, direct, rad, irr);
