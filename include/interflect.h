@@ -86,6 +86,95 @@ const char* ifx_last_error(const ifx_ctx* ctx);
 
 /* scene::parse + Bvh::build + world_bounds (src/scene.rs:365-571, src/bvh.rs:53). */
 ifx_status ifx_load_scene(ifx_ctx* ctx, const char* rad_text);
+
+/* ---- A parsed scene from the caller (the reference's own Scene,
+ * src/scene.rs:163-178, after its parser and Bvh::build), for hosts that keep
+ * their .rad loader.  Plain arrays; nothing is retained after the call. */
+#define IFX_PRIM_SPHERE 0
+#define IFX_PRIM_BOX 1
+#define IFX_PRIM_PLANE 2
+#define IFX_PRIM_CAPSULE 3
+#define IFX_PRIM_TORUS 4
+#define IFX_PRIM_CYLINDER 5
+#define IFX_PRIM_CONE 6
+
+/* Prim (src/sdf.rs:9-48).  a: sphere / box / torus centre, plane normal,
+ * capsule / cylinder / cone endpoint a; b: box half extents (Prim::RoundBox b),
+ * endpoint b; r: sphere / box / capsule / cylinder radius (box: rounding),
+ * plane height, torus major radius, cone radius at a; r2: torus minor radius,
+ * cone radius at b. */
+typedef struct {
+    uint32_t kind;
+    float a[3], b[3];
+    float r, r2;
+} ifx_prim;
+
+/* Object (src/sdf.rs:151-163): base primitive, subtractions
+ * subtractions[sub_first .. sub_first + sub_count), material index (0 = the
+ * implicit default), smooth-union radius (loader extension; 0 = hard union). */
+typedef struct {
+    ifx_prim prim;
+    uint32_t sub_first, sub_count, mat;
+    float blend;
+} ifx_object;
+
+/* BVH node (src/bvh.rs:26-36): leaf = [start, start + count) of the order
+ * array; interior (count 0) = left child at the next index, right child at
+ * `right`. */
+typedef struct {
+    float lo[3], hi[3];
+    uint32_t start, count, right;
+} ifx_bvh_node;
+
+/* Material (src/scene.rs:11-28); QuadLight (src/scene.rs:32-56); sphere and
+ * point lights (loader extensions). */
+typedef struct {
+    float albedo[3], roughness, metallic, emissive[3];
+} ifx_material;
+typedef struct {
+    float verts[4][3], emit[3];
+} ifx_quad_light;
+typedef struct {
+    float center[3], radius, emit[3];
+} ifx_sphere_light;
+typedef struct {
+    float pos[3], emit[3];
+} ifx_point_light;
+
+typedef struct {
+    const ifx_object* objects; /* bounded objects, indexed by the BVH */
+    uint32_t n_objects;
+    const ifx_object* planes; /* unbounded objects (planes) */
+    uint32_t n_planes;
+    const ifx_prim* subtractions;
+    uint32_t n_subtractions;
+    const ifx_bvh_node* bvh_nodes; /* n_bvh_nodes = 0: the library builds the BVH (src/bvh.rs:53) */
+    uint32_t n_bvh_nodes;
+    const uint32_t* bvh_order; /* n_objects object indices in leaf order, with the nodes */
+    const ifx_material* materials; /* index 0 is the implicit default material */
+    uint32_t n_materials;
+    const ifx_quad_light* quad_lights;
+    uint32_t n_quad_lights;
+    const ifx_sphere_light* sphere_lights;
+    uint32_t n_sphere_lights;
+    const ifx_point_light* point_lights;
+    uint32_t n_point_lights;
+    float lo[3], hi[3]; /* Scene::bounds (src/scene.rs:573-601) */
+    ifx_camera camera;
+    uint32_t width, height, bounces, reserved;
+    uint64_t surfels;
+    float exposure;
+} ifx_scene_desc;
+
+/* Replaces ifx_load_scene for a caller with a parsed scene.  Validated like
+ * the loader's output (render limits, a light, material and subtraction
+ * indices, a BVH whose leaves cover every object once).  Equivalent scenes
+ * give byte-identical device records whichever way they arrive. */
+ifx_status ifx_set_scene(ifx_ctx* ctx, const ifx_scene_desc* desc);
+/* 64-bit digest of the scene's flattened device records, BVH, bounds,
+ * camera and render settings (no counterpart in the reference; lets a caller
+ * check that two routes gave the same scene). */
+ifx_status ifx_scene_digest(ifx_ctx* ctx, uint64_t* digest);
 ifx_status ifx_get_scene_info(ifx_ctx* ctx, ifx_scene_info* out);
 /* CLI overrides of render {} (src/main.rs:394-413); re-validated like the reference. */
 ifx_status ifx_set_render(ifx_ctx* ctx, uint32_t width, uint32_t height, uint64_t surfels, uint32_t bounces);
@@ -98,6 +187,11 @@ ifx_status ifx_generate_surfels(ifx_ctx* ctx, uint64_t target, uint64_t* placed)
 /* Supply surfels directly (the pipeline input "given identical surfels"). */
 ifx_status ifx_set_surfels(ifx_ctx* ctx, const float* p, const float* n, const float* albedo, const float* area,
                            uint64_t count);
+/* Per-surfel material index (count entries, the surfels of the last
+ * ifx_set_surfels).  Without it, ifx_set_surfels takes each surfel's material
+ * from the scene at its position (Scene::dist_mat, src/scene.rs:191-208), as
+ * surfel::generate does; ifx_backward needs the materials. */
+ifx_status ifx_set_surfel_materials(ifx_ctx* ctx, const uint32_t* mat, uint64_t count);
 ifx_status ifx_num_surfels(ifx_ctx* ctx, uint64_t* count);
 ifx_status ifx_get_surfels(ifx_ctx* ctx, float* p, float* n, float* albedo, float* area);
 
@@ -191,8 +285,8 @@ ifx_status ifx_kernel_ops(ifx_ctx* ctx, const char* name, uint64_t* ops);
  * material's albedo and each light's emission, geometry frozen: transfer
  * weights, light visibility and gather weights are held fixed and the solve
  * is differentiated at its fixed point (implicit adjoint).  Diffuse
- * materials only (metallic 0, roughness >= 0.45); surfels must carry their
- * material (ifx_generate_surfels).  Materials are indexed as the scene
+ * materials only (metallic 0, roughness >= 0.45); surfels carry their
+ * material (ifx_generate_surfels, ifx_set_surfels).  Materials are indexed as the scene
  * indexes them: 0 is the implicit default material (src/scene.rs:373), k the
  * k-th material block of the file, so there are n_materials + 1 of them.
  * Lights are indexed quads first, then sphere lights, then point lights. */

@@ -42,6 +42,7 @@ EXPORTS = [
     "ifx_batch_generate_surfels", "ifx_batch_set_graph", "ifx_batch_set_tuning", "ifx_batch_autotune",
     "ifx_batch_run", "ifx_batch_get_stats", "ifx_batch_get_image", "ifx_batch_get_radiosity",
     "ifx_batch_get_transfer", "ifx_shade_views", "ifx_turntable_cameras", "ifx_solve_noalloc",
+    "ifx_set_scene", "ifx_scene_digest", "ifx_set_surfel_materials",
 ]
 
 
@@ -63,6 +64,19 @@ class SceneInfo(C.Structure):
 
 class Camera(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("look", C.c_float * 3), ("up", C.c_float * 3), ("fov", C.c_float)]
+
+
+class SceneDesc(C.Structure):
+    """ifx_scene_desc: array pointers (arrays in the layout of ifx_object,
+    ifx_prim, ifx_bvh_node, ifx_material and the light structs) and counts."""
+    _fields_ = [("objects", C.c_void_p), ("n_objects", C.c_uint32), ("planes", C.c_void_p), ("n_planes", C.c_uint32),
+                ("subtractions", C.c_void_p), ("n_subtractions", C.c_uint32), ("bvh_nodes", C.c_void_p),
+                ("n_bvh_nodes", C.c_uint32), ("bvh_order", C.c_void_p), ("materials", C.c_void_p),
+                ("n_materials", C.c_uint32), ("quad_lights", C.c_void_p), ("n_quad_lights", C.c_uint32),
+                ("sphere_lights", C.c_void_p), ("n_sphere_lights", C.c_uint32), ("point_lights", C.c_void_p),
+                ("n_point_lights", C.c_uint32), ("lo", C.c_float * 3), ("hi", C.c_float * 3), ("camera", Camera),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("bounces", C.c_uint32), ("reserved", C.c_uint32),
+                ("surfels", C.c_uint64), ("exposure", C.c_float)]
 
 
 class TransferStats(C.Structure):
@@ -104,6 +118,9 @@ def lib():
         "ifx_create": [C.POINTER(vp), i32],
         "ifx_destroy": [vp],
         "ifx_load_scene": [vp, C.c_char_p],
+        "ifx_set_scene": [vp, C.POINTER(SceneDesc)],
+        "ifx_scene_digest": [vp, C.POINTER(u64)],
+        "ifx_set_surfel_materials": [vp, P, u64],
         "ifx_get_scene_info": [vp, C.POINTER(SceneInfo)],
         "ifx_set_render": [vp, u32, u32, u64, u32],
         "ifx_set_use_bvh": [vp, i32],
@@ -212,6 +229,48 @@ class Context:
     def load_scene(self, rad_text):
         self._check(lib().ifx_load_scene(self._h, rad_text.encode()))
         return self.info()
+
+    def set_scene(self, desc, with_bvh=True):
+        """ifx_set_scene from arrays in the descriptor's layout (keys objects,
+        planes, subtractions, bvh_nodes, bvh_order, materials, quad_lights,
+        sphere_lights, point_lights: 2-D float32 arrays, integers as their bit
+        patterns; bounds[6]; camera[10]; render {width, height, bounces,
+        surfels, exposure}).  with_bvh=False lets the library build the BVH."""
+        keep = {k: np.ascontiguousarray(v) for k, v in desc.items() if isinstance(v, np.ndarray)}
+
+        def arr(k):
+            a = keep[k]
+            return (a.ctypes.data if a.size else None), (a.shape[0] if a.ndim > 1 else a.size)
+
+        d = SceneDesc()
+        d.objects, d.n_objects = arr("objects")
+        d.planes, d.n_planes = arr("planes")
+        d.subtractions, d.n_subtractions = arr("subtractions")
+        if with_bvh:
+            d.bvh_nodes, d.n_bvh_nodes = arr("bvh_nodes")
+            d.bvh_order = arr("bvh_order")[0]
+        d.materials, d.n_materials = arr("materials")
+        d.quad_lights, d.n_quad_lights = arr("quad_lights")
+        d.sphere_lights, d.n_sphere_lights = arr("sphere_lights")
+        d.point_lights, d.n_point_lights = arr("point_lights")
+        b = keep["bounds"]
+        d.lo, d.hi = (C.c_float * 3)(*b[:3]), (C.c_float * 3)(*b[3:6])
+        c = keep["camera"]
+        d.camera = Camera((C.c_float * 3)(*c[0:3]), (C.c_float * 3)(*c[3:6]), (C.c_float * 3)(*c[6:9]), float(c[9]))
+        r = desc["render"]
+        d.width, d.height, d.bounces = int(r["width"]), int(r["height"]), int(r["bounces"])
+        d.surfels, d.exposure = int(r["surfels"]), float(r["exposure"])
+        self._check(lib().ifx_set_scene(self._h, C.byref(d)))
+        return self.info()
+
+    def scene_digest(self):
+        v = C.c_uint64()
+        self._check(lib().ifx_scene_digest(self._h, C.byref(v)))
+        return int(v.value)
+
+    def set_surfel_materials(self, mat):
+        m = np.ascontiguousarray(mat, np.uint32)
+        self._check(lib().ifx_set_surfel_materials(self._h, _ptr(m), m.size))
 
     def info(self):
         s = SceneInfo()

@@ -1,6 +1,7 @@
 // extern "C" boundary of libinterflect_hip.so (declared in include/interflect.h).
 // No C++ exception crosses it: every entry point catches and maps to a status.
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -9,6 +10,7 @@
 #include "../../include/interflect.h"
 #include "batch.h"
 #include "engine.h"
+#include "host_surfels.h"
 
 struct ifx_ctx {
     ifx::Engine eng;
@@ -114,6 +116,161 @@ ifx_status ifx_load_scene(ifx_ctx* ctx, const char* rad_text) {
     });
 }
 
+namespace {
+bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+// ifx_prim -> the loader's primitive (fields as Prim, src/sdf.rs:9-48).
+bool prim_from(const ifx_prim& p, ifx::HPrim& h, std::string& err) {
+    if (p.kind > IFX_PRIM_CONE) {
+        err = "unknown primitive kind " + std::to_string(p.kind);
+        return false;
+    }
+    if (!finite3(p.a) || !finite3(p.b) || !std::isfinite(p.r) || !std::isfinite(p.r2)) {
+        err = "primitive parameters must be finite";
+        return false;
+    }
+    const ifx::f3 a = ifx::mk(p.a[0], p.a[1], p.a[2]), b = ifx::mk(p.b[0], p.b[1], p.b[2]);
+    h.kind = p.kind;
+    switch (p.kind) {
+        case IFX_PRIM_SPHERE: h.c = a, h.r = p.r; break;
+        case IFX_PRIM_BOX: h.c = a, h.b = b, h.r = p.r; break;
+        case IFX_PRIM_PLANE: h.n = a, h.h = p.r; break;
+        case IFX_PRIM_CAPSULE:
+        case IFX_PRIM_CYLINDER: h.a = a, h.b = b, h.r = p.r; break;
+        case IFX_PRIM_TORUS: h.c = a, h.major = p.r, h.minor = p.r2; break;
+        default: h.a = a, h.b = b, h.ra = p.r, h.rb = p.r2;
+    }
+    return true;
+}
+
+bool object_from(const ifx_object& o, const ifx_scene_desc& d, bool plane, ifx::HObject& h, std::string& err) {
+    if (!prim_from(o.prim, h.prim, err)) return false;
+    if ((o.prim.kind == IFX_PRIM_PLANE) != plane) {
+        err = plane ? "the planes list holds planes only" : "planes belong in the planes list (they are unbounded)";
+        return false;
+    }
+    if ((uint64_t)o.sub_first + o.sub_count > d.n_subtractions || (o.sub_count && !d.subtractions)) {
+        err = "subtraction range beyond the subtractions array";
+        return false;
+    }
+    if (o.mat >= d.n_materials) {
+        err = "material index " + std::to_string(o.mat) + " beyond the materials array";
+        return false;
+    }
+    if (!(o.blend >= 0.0f) || !std::isfinite(o.blend)) {
+        err = "blend must be finite and >= 0";
+        return false;
+    }
+    h.subs.resize(o.sub_count);
+    for (uint32_t k = 0; k < o.sub_count; ++k)
+        if (!prim_from(d.subtractions[o.sub_first + k], h.subs[k], err)) return false;
+    h.mat = o.mat;
+    h.blend = o.blend;
+    return true;
+}
+
+// ifx_scene_desc -> a finalized host scene (the loader's output).
+ifx_status scene_from(const ifx_scene_desc& d, ifx::HostScene& s, std::string& err) {
+    if ((d.n_objects && !d.objects) || (d.n_planes && !d.planes) || (d.n_bvh_nodes && (!d.bvh_nodes || !d.bvh_order)) ||
+        (d.n_quad_lights && !d.quad_lights) || (d.n_sphere_lights && !d.sphere_lights) ||
+        (d.n_point_lights && !d.point_lights) || !d.materials || d.n_materials == 0) {
+        err = "scene descriptor: null array (materials must hold at least the default material)";
+        return IFX_E_INVALID;
+    }
+    if (d.n_quad_lights + d.n_sphere_lights + d.n_point_lights == 0) {
+        err = "scene has no `light` block";
+        return IFX_E_INVALID;
+    }
+    s.objects.resize(d.n_objects);
+    for (uint32_t i = 0; i < d.n_objects; ++i)
+        if (!object_from(d.objects[i], d, false, s.objects[i], err)) return IFX_E_INVALID;
+    s.planes.resize(d.n_planes);
+    for (uint32_t i = 0; i < d.n_planes; ++i)
+        if (!object_from(d.planes[i], d, true, s.planes[i], err)) return IFX_E_INVALID;
+    s.mats.resize(d.n_materials);
+    for (uint32_t i = 0; i < d.n_materials; ++i) {
+        const ifx_material& m = d.materials[i];
+        s.mats[i].albedo = ifx::mk(m.albedo[0], m.albedo[1], m.albedo[2]);
+        s.mats[i].roughness = m.roughness;
+        s.mats[i].metallic = m.metallic;
+        s.mats[i].emissive = ifx::mk(m.emissive[0], m.emissive[1], m.emissive[2]);
+    }
+    for (uint32_t i = 0; i < d.n_quad_lights; ++i) {
+        ifx::HQuad q;
+        for (int k = 0; k < 4; ++k) q.v[k] = ifx::mk(d.quad_lights[i].verts[k][0], d.quad_lights[i].verts[k][1],
+                                                    d.quad_lights[i].verts[k][2]);
+        q.emit = ifx::mk(d.quad_lights[i].emit[0], d.quad_lights[i].emit[1], d.quad_lights[i].emit[2]);
+        s.quads.push_back(q);
+    }
+    for (uint32_t i = 0; i < d.n_sphere_lights; ++i) {
+        const ifx_sphere_light& l = d.sphere_lights[i];
+        s.spheres.push_back(ifx::HSphereLight{ifx::mk(l.center[0], l.center[1], l.center[2]), l.radius,
+                                              ifx::mk(l.emit[0], l.emit[1], l.emit[2])});
+    }
+    for (uint32_t i = 0; i < d.n_point_lights; ++i) {
+        const ifx_point_light& l = d.point_lights[i];
+        s.points.push_back(ifx::HPointLight{ifx::mk(l.pos[0], l.pos[1], l.pos[2]), ifx::mk(l.emit[0], l.emit[1],
+                                                                                          l.emit[2])});
+    }
+    s.cam.pos = ifx::mk(d.camera.pos[0], d.camera.pos[1], d.camera.pos[2]);
+    s.cam.look = ifx::mk(d.camera.look[0], d.camera.look[1], d.camera.look[2]);
+    s.cam.up = ifx::mk(d.camera.up[0], d.camera.up[1], d.camera.up[2]);
+    s.cam.fov = d.camera.fov;
+    s.cfg.width = d.width;
+    s.cfg.height = d.height;
+    s.cfg.surfels = d.surfels;
+    s.cfg.bounces = d.bounces;
+    s.cfg.exposure = d.exposure;
+    if (!ifx::validate_cfg(s.cfg, err)) return IFX_E_INVALID;
+    if (!finite3(d.lo) || !finite3(d.hi) || !(d.lo[0] <= d.hi[0] && d.lo[1] <= d.hi[1] && d.lo[2] <= d.hi[2])) {
+        err = "scene bounds must be finite with lo <= hi";
+        return IFX_E_INVALID;
+    }
+    const ifx::f3 bounds[2] = {ifx::mk(d.lo[0], d.lo[1], d.lo[2]), ifx::mk(d.hi[0], d.hi[1], d.hi[2])};
+    try {
+        if (d.n_bvh_nodes) {
+            std::vector<ifx::HBvhNode> nodes(d.n_bvh_nodes);
+            for (uint32_t i = 0; i < d.n_bvh_nodes; ++i) {
+                const ifx_bvh_node& n = d.bvh_nodes[i];
+                nodes[i] = ifx::HBvhNode{ifx::mk(n.lo[0], n.lo[1], n.lo[2]), ifx::mk(n.hi[0], n.hi[1], n.hi[2]),
+                                         n.start, n.count, n.right};
+            }
+            std::vector<uint32_t> order(d.bvh_order, d.bvh_order + d.n_objects);
+            ifx::finalize_scene_given(s, nodes, order, bounds[0], bounds[1]);
+        } else {
+            ifx::finalize_scene(s, bounds);
+        }
+    } catch (const std::runtime_error& e) {
+        err = e.what();
+        return IFX_E_INVALID;
+    }
+    return IFX_OK;
+}
+}  // namespace
+
+ifx_status ifx_set_scene(ifx_ctx* ctx, const ifx_scene_desc* desc) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (!desc) {
+            ctx->err = "null scene descriptor";
+            return IFX_E_INVALID;
+        }
+        ifx::HostScene s;
+        if (ifx_status rc = scene_from(*desc, s, ctx->err)) return rc;
+        int rc = ctx->eng.install_scene(std::move(s), ctx->err);
+        ctx->have_scene = rc == 0;
+        return rc;
+    });
+}
+
+ifx_status ifx_scene_digest(ifx_ctx* ctx, uint64_t* digest) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        if (!digest) return IFX_E_INVALID;
+        *digest = ifx::scene_digest(ctx->eng.scene);
+        return IFX_OK;
+    });
+}
+
 ifx_status ifx_get_scene_info(ifx_ctx* ctx, ifx_scene_info* o) {
     return guarded(ctx, [&]() -> ifx_status {
         if (ifx_status s = need_scene(ctx)) return s;
@@ -193,6 +350,26 @@ ifx_status ifx_set_surfels(ifx_ctx* ctx, const float* p, const float* n, const f
             s[i].n = ifx::mk(n[3 * i], n[3 * i + 1], n[3 * i + 2]);
             s[i].albedo = ifx::mk(albedo[3 * i], albedo[3 * i + 1], albedo[3 * i + 2]);
             s[i].area = area[i];
+        }
+        ifx::surfel_materials(ctx->eng.scene, s, host_threads());
+        return ctx->eng.set_surfels(s, ctx->err);
+    });
+}
+
+ifx_status ifx_set_surfel_materials(ifx_ctx* ctx, const uint32_t* mat, uint64_t count) {
+    return guarded(ctx, [&]() -> ifx_status {
+        if (ifx_status s = need_scene(ctx)) return s;
+        std::vector<ifx::HSurfel> s = ctx->eng.surfels;
+        if (count != s.size() || (count && !mat)) {
+            ctx->err = "need one material per surfel (" + std::to_string(s.size()) + ")";
+            return IFX_E_INVALID;
+        }
+        for (uint64_t i = 0; i < count; ++i) {
+            if (mat[i] >= ctx->eng.scene.mats.size()) {
+                ctx->err = "material index " + std::to_string(mat[i]) + " beyond the scene's materials";
+                return IFX_E_INVALID;
+            }
+            s[i].mat = mat[i];
         }
         return ctx->eng.set_surfels(s, ctx->err);
     });

@@ -96,6 +96,8 @@ class Engine {
     int init(int device, std::string& err);
 
     int load_scene(const std::string& src, std::string& err);
+    // A finalized host scene (parsed here or given by the caller) to the device.
+    int install_scene(HostScene&& s, std::string& err);
     int set_surfels(const std::vector<HSurfel>& s, std::string& err);
     // surfel::generate on the device, equal to generate_surfels() on the host.
     int generate_surfels_gpu(uint64_t target, std::vector<HSurfel>& out, std::string& err);

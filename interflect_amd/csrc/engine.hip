@@ -1014,6 +1014,10 @@ int Engine::copy3_h2d(void* dev, const float* host, uint64_t n, std::string& err
 int Engine::load_scene(const std::string& src, std::string& err) {
     HostScene s;
     if (!load_rad(src, s, err)) return -2;
+    return install_scene(std::move(s), err);
+}
+
+int Engine::install_scene(HostScene&& s, std::string& err) {
     if (s.max_stack > (uint32_t)kStackCap) {
         err = "BVH needs a traversal stack of " + std::to_string(s.max_stack) + " entries; the kernels hold " +
               std::to_string(kStackCap);

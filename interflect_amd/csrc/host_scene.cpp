@@ -358,20 +358,11 @@ void prim_rec(const HPrim& p, float4* r) {
 
 }  // namespace
 
-void finalize_scene(HostScene& s) {
-    // Subtraction primitives, shared by objects and planes.
-    s.sub_rec.clear();
-    auto object_rec = [&](const HObject& o, float4* r) {
-        prim_rec(o.prim, r);
-        uint32_t sb = (uint32_t)(s.sub_rec.size() / 4);
-        for (const HPrim& q : o.subs) {
-            float4 t[4];
-            prim_rec(q, t);
-            s.sub_rec.insert(s.sub_rec.end(), t, t + 4);
-        }
-        r[3] = make_float4(bitsf(o.mat), bitsf(sb), bitsf((uint32_t)o.subs.size()), bitsf(o.blend > 0.0f ? 1u : 0u));
-    };
+namespace {
+void flatten_scene(HostScene& s, const std::vector<BuildNode>& nodes, const f3* bounds);
+}  // namespace
 
+void finalize_scene(HostScene& s, const f3* bounds) {
     // BVH over bounded objects, boxes grown by the blend radius (src/bvh.rs:62-72).
     std::vector<BuildBox> boxes;
     for (const HObject& o : s.objects) {
@@ -389,6 +380,65 @@ void finalize_scene(HostScene& s) {
         nodes.emplace_back();
         BvhBuilder b{boxes, s.order, nodes};
         b.node(0, 0, s.objects.size());
+    }
+    flatten_scene(s, nodes, bounds);
+}
+
+void finalize_scene_given(HostScene& s, const std::vector<HBvhNode>& given, const std::vector<uint32_t>& order,
+                          f3 lo, f3 hi) {
+    if (order.size() != s.objects.size()) throw std::runtime_error("BVH order must list every object once");
+    std::vector<uint32_t> seen(order.size(), 0u);
+    for (uint32_t o : order)
+        if (o >= order.size() || seen[o]++) throw std::runtime_error("BVH order must list every object once");
+    if (given.empty() != s.objects.empty()) throw std::runtime_error("a scene with objects needs BVH nodes");
+    std::vector<BuildNode> nodes(given.size());
+    for (size_t i = 0; i < given.size(); ++i) {
+        nodes[i].lo = given[i].lo;
+        nodes[i].hi = given[i].hi;
+        nodes[i].start = given[i].start;
+        nodes[i].count = given[i].count;
+        nodes[i].right = given[i].right;
+    }
+    s.order = order;
+    const f3 b[2] = {lo, hi};
+    flatten_scene(s, nodes, b);
+}
+
+namespace {
+// Device records from the objects, the BVH (nodes over s.order) and the
+// bounds (world_bounds when `bounds` is null).  Throws std::runtime_error
+// on a BVH the traversals cannot walk.
+void flatten_scene(HostScene& s, const std::vector<BuildNode>& nodes, const f3* bounds) {
+    // Subtraction primitives, shared by objects and planes.
+    s.sub_rec.clear();
+    auto object_rec = [&](const HObject& o, float4* r) {
+        prim_rec(o.prim, r);
+        uint32_t sb = (uint32_t)(s.sub_rec.size() / 4);
+        for (const HPrim& q : o.subs) {
+            float4 t[4];
+            prim_rec(q, t);
+            s.sub_rec.insert(s.sub_rec.end(), t, t + 4);
+        }
+        r[3] = make_float4(bitsf(o.mat), bitsf(sb), bitsf((uint32_t)o.subs.size()), bitsf(o.blend > 0.0f ? 1u : 0u));
+    };
+
+    // The device traversals rely on this layout: every slot lies in exactly
+    // one non-empty leaf, and each interior node's children (node + 1 and
+    // `right`) lie after it, so a traversal always ends.
+    {
+        const size_t n_obj = s.objects.size();
+        std::vector<uint32_t> cover(n_obj, 0u);
+        for (size_t ni = 0; ni < nodes.size(); ++ni) {
+            const BuildNode& n = nodes[ni];
+            if (n.count > 0) {
+                for (uint64_t k = n.start; k < (uint64_t)n.start + n.count; ++k)
+                    if (k >= n_obj || cover[k]++) throw std::runtime_error("BVH leaf ranges overlap or overrun");
+            } else if (!(ni + 1 < nodes.size() && n.right > ni + 1 && n.right < nodes.size())) {
+                throw std::runtime_error("BVH interior node has an out-of-order child");
+            }
+        }
+        for (size_t k = 0; k < n_obj; ++k)
+            if (cover[k] != 1u) throw std::runtime_error("BVH leaves do not cover every object");
     }
     s.node_rec.resize(2 * nodes.size());
     for (size_t i = 0; i < nodes.size(); ++i) {
@@ -424,23 +474,7 @@ void finalize_scene(HostScene& s) {
     for (size_t ni = 0; ni < nodes.size(); ++ni)
         for (uint32_t k = nodes[ni].start; nodes[ni].count > 0 && k < nodes[ni].start + nodes[ni].count; ++k)
             s.lin[n_obj + k] = (uint32_t)ni;
-    // The device traversals rely on this layout: every slot lies in exactly
-    // one non-empty leaf, and each interior node's children (node + 1 and
-    // `right`) lie after it, so a traversal always ends.
-    {
-        std::vector<uint32_t> cover(n_obj, 0u);
-        for (size_t ni = 0; ni < nodes.size(); ++ni) {
-            const BuildNode& n = nodes[ni];
-            if (n.count > 0) {
-                for (uint32_t k = n.start; k < n.start + n.count; ++k)
-                    if (k >= n_obj || cover[k]++) throw std::runtime_error("BVH leaf ranges overlap or overrun");
-            } else if (!(ni + 1 < nodes.size() && n.right > ni + 1 && n.right < nodes.size())) {
-                throw std::runtime_error("BVH interior node has an out-of-order child");
-            }
-        }
-        for (size_t k = 0; k < n_obj; ++k)
-            if (cover[k] != 1u) throw std::runtime_error("BVH leaves do not cover every object");
-    }
+
     s.plane_rec.assign(4 * s.planes.size(), make_float4(0, 0, 0, 0));
     for (size_t i = 0; i < s.planes.size(); ++i) object_rec(s.planes[i], &s.plane_rec[4 * i]);
     s.blend_k = 0.0f;
@@ -448,6 +482,9 @@ void finalize_scene(HostScene& s) {
 
     // World bounds (src/scene.rs:573-601; EXT emitters included).
     f3 lo = spl(kFltMax), hi = spl(-kFltMax);
+    if (bounds) {
+        s.lo = bounds[0], s.hi = bounds[1];
+    } else {
     for (const HObject& o : s.objects) {
         f3 a, b;
         if (prim_bounds(o.prim, a, b)) lo = min3(lo, a), hi = max3(hi, b);
@@ -462,6 +499,7 @@ void finalize_scene(HostScene& s) {
         float dg = len3(hi - lo);
         float pad = s.planes.empty() ? dg * 0.02f : dg * 0.25f;
         s.lo = lo - spl(pad), s.hi = hi + spl(pad);
+    }
     }
 
     // Emitters and materials with the reference's per-call derived terms.
@@ -488,6 +526,33 @@ void finalize_scene(HostScene& s) {
         material_records(m, r);
         s.mat_rec.insert(s.mat_rec.end(), r, r + 4);
     }
+}
+
+}  // namespace
+
+uint64_t scene_digest(const HostScene& s) {
+    uint64_t h = 1469598103934665603ull;  // FNV-1a, 64 bit
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* b = (const unsigned char*)p;
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    };
+    auto vec = [&](const auto& v) {
+        const uint64_t n = v.size();
+        mix(&n, sizeof(n));
+        if (n) mix(v.data(), n * sizeof(v[0]));
+    };
+    vec(s.obj_rec), vec(s.plane_rec), vec(s.sub_rec), vec(s.node_rec), vec(s.order), vec(s.lin);
+    vec(s.quad_rec), vec(s.sph_rec), vec(s.pnt_rec), vec(s.mat_rec);
+    const uint32_t counts[6] = {(uint32_t)s.objects.size(), (uint32_t)s.planes.size(), (uint32_t)s.quads.size(),
+                                (uint32_t)s.spheres.size(), (uint32_t)s.points.size(), s.max_stack};
+    mix(counts, sizeof(counts));
+    const float f[18] = {s.lo.x, s.lo.y, s.lo.z, s.hi.x, s.hi.y, s.hi.z, s.blend_k, s.cam.pos.x, s.cam.pos.y,
+                         s.cam.pos.z, s.cam.look.x, s.cam.look.y, s.cam.look.z, s.cam.up.x, s.cam.up.y, s.cam.up.z,
+                         s.cam.fov, s.cfg.exposure};
+    mix(f, sizeof(f));
+    const uint64_t c[4] = {s.cfg.width, s.cfg.height, s.cfg.surfels, s.cfg.bounces};
+    mix(c, sizeof(c));
+    return h;
 }
 
 void material_records(const HMaterial& m, float4 out[4]) {

@@ -87,8 +87,21 @@ struct HostScene {
 bool load_rad(const std::string& src, HostScene& out, std::string& err);
 // Render-settings limits (src/scene.rs:94-148).
 bool validate_cfg(const HCfg& c, std::string& err);
-// Rebuilds the BVH, bounds and device records from the object lists.
-void finalize_scene(HostScene& s);
+// Rebuilds the BVH, bounds (unless given: lo, hi) and device records from
+// the object lists.  Throws std::runtime_error when the BVH cannot be walked.
+void finalize_scene(HostScene& s, const f3* bounds = nullptr);
+// A caller's BVH node (src/bvh.rs:26-36).
+struct HBvhNode {
+    f3 lo, hi;
+    uint32_t start, count, right;
+};
+// Device records from the object lists with a given BVH (nodes over `order`,
+// object indices in leaf order) and bounds; throws std::runtime_error when
+// the BVH does not cover every object once or cannot be walked.
+void finalize_scene_given(HostScene& s, const std::vector<HBvhNode>& nodes, const std::vector<uint32_t>& order,
+                          f3 lo, f3 hi);
+// 64-bit digest of the flattened records, BVH, bounds, camera and settings.
+uint64_t scene_digest(const HostScene& s);
 // The four device records of one material (albedo+roughness, emissive+metallic, f0, kd).
 void material_records(const HMaterial& m, float4 out[4]);
 

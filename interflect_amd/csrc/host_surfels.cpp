@@ -83,4 +83,17 @@ std::vector<HSurfel> generate_surfels(const HostScene& sc, uint64_t target, int 
     return out;
 }
 
+void surfel_materials(const HostScene& sc, std::vector<HSurfel>& s, int threads) {
+    const SceneView S = sc.host_view();
+    const int nt = std::max(1, threads);
+    auto work = [&](int t) {
+        ArrStack stk;
+        for (size_t i = (size_t)t; i < s.size(); i += (size_t)nt) s[i].mat = dist_mat(S, s[i].p, stk).mat;
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto& th : pool) th.join();
+}
+
 }  // namespace ifx

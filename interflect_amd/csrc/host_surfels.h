@@ -13,5 +13,8 @@ struct HSurfel {
 };
 
 std::vector<HSurfel> generate_surfels(const HostScene& sc, uint64_t target, int threads);
+// Each surfel's material from the scene at its position (Scene::dist_mat,
+// src/scene.rs:191-208), as surfel placement assigns it.
+void surfel_materials(const HostScene& sc, std::vector<HSurfel>& s, int threads);
 
 }  // namespace ifx

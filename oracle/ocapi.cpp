@@ -333,6 +333,95 @@ void orc_encode(const float* film, int64_t npx, float exposure, int tonemap, uin
     std::memcpy(out, o.data(), o.size());
 }
 
+// ---- the parsed scene in the layout of the product library's ifx_scene_desc
+// arrays (a test fixture for its scene-descriptor entry point): object 13
+// words (kind, a[3], b[3], r, r2, sub_first, sub_count, mat, blend), primitive
+// 9, BVH node 9 (lo[3], hi[3], start, count, right), material 8, quad light
+// 15, sphere light 7, point light 6; integers as their 32-bit patterns.
+// counts: objects, planes, subtractions, BVH nodes, materials, quad, sphere
+// and point lights.
+namespace {
+float float_of(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+void prim_words(const Prim& p, float* w) {
+    V3 a = p.a, b = p.b;
+    float r = p.r, r2 = 0.0f;
+    switch (p.kind) {
+        case SPHERE: a = p.c; break;
+        case ROUNDBOX: a = p.c; break;
+        case PLANE: a = p.n, r = p.h; break;
+        case TORUS: a = p.c, r = p.major, r2 = p.minor; break;
+        case CONE: r = p.ra, r2 = p.rb; break;
+        default: break;
+    }
+    w[0] = float_of((uint32_t)p.kind);
+    st(w + 1, a);
+    st(w + 4, b);
+    w[7] = r;
+    w[8] = r2;
+}
+}  // namespace
+
+void orc_scene_export_counts(void* h, uint32_t counts[8]) {
+    const Scene& sc = ((Ctx*)h)->sc;
+    uint32_t subs = 0;
+    for (const Object& o : sc.objects) subs += (uint32_t)o.subtract.size();
+    for (const Object& o : sc.planes) subs += (uint32_t)o.subtract.size();
+    counts[0] = (uint32_t)sc.objects.size(), counts[1] = (uint32_t)sc.planes.size(), counts[2] = subs;
+    counts[3] = (uint32_t)sc.bvh.nodes.size(), counts[4] = (uint32_t)sc.materials.size();
+    counts[5] = (uint32_t)sc.lights.size(), counts[6] = (uint32_t)sc.sphere_lights.size();
+    counts[7] = (uint32_t)sc.point_lights.size();
+}
+
+void orc_scene_export(void* h, float* objects, float* planes, float* subs, float* nodes, uint32_t* order,
+                      float* materials, float* quads, float* spheres, float* points, float* bounds) {
+    const Scene& sc = ((Ctx*)h)->sc;
+    uint32_t ns = 0;
+    auto obj = [&](const Object& o, float* w) {
+        prim_words(o.prim, w);
+        w[9] = float_of(ns);
+        w[10] = float_of((uint32_t)o.subtract.size());
+        w[11] = float_of(o.mat);
+        w[12] = o.blend;
+        for (const Prim& p : o.subtract) prim_words(p, subs + 9 * ns++);
+    };
+    for (size_t i = 0; i < sc.objects.size(); ++i) obj(sc.objects[i], objects + 13 * i);
+    for (size_t i = 0; i < sc.planes.size(); ++i) obj(sc.planes[i], planes + 13 * i);
+    for (size_t i = 0; i < sc.bvh.nodes.size(); ++i) {
+        const BvhNode& n = sc.bvh.nodes[i];
+        float* w = nodes + 9 * i;
+        st(w, n.lo);
+        st(w + 3, n.hi);
+        w[6] = float_of(n.start), w[7] = float_of(n.count), w[8] = float_of(n.right);
+    }
+    for (size_t i = 0; i < sc.bvh.order.size(); ++i) order[i] = sc.bvh.order[i];
+    for (size_t i = 0; i < sc.materials.size(); ++i) {
+        const Material& m = sc.materials[i];
+        float* w = materials + 8 * i;
+        st(w, m.albedo);
+        w[3] = m.roughness, w[4] = m.metallic;
+        st(w + 5, m.emissive);
+    }
+    for (size_t i = 0; i < sc.lights.size(); ++i) {
+        for (int k = 0; k < 4; ++k) st(quads + 15 * i + 3 * k, sc.lights[i].verts[k]);
+        st(quads + 15 * i + 12, sc.lights[i].emit);
+    }
+    for (size_t i = 0; i < sc.sphere_lights.size(); ++i) {
+        st(spheres + 7 * i, sc.sphere_lights[i].c);
+        spheres[7 * i + 3] = sc.sphere_lights[i].r;
+        st(spheres + 7 * i + 4, sc.sphere_lights[i].emit);
+    }
+    for (size_t i = 0; i < sc.point_lights.size(); ++i) {
+        st(points + 6 * i, sc.point_lights[i].pos);
+        st(points + 6 * i + 3, sc.point_lights[i].emit);
+    }
+    st(bounds, sc.lo);
+    st(bounds + 3, sc.hi);
+}
+
 // ---- known-answer helpers
 // params: kind-specific, packed as c|a[3], b[3], r|h|major|ra, minor|rb, n[3]
 float orc_prim_dist(int kind, const float* q, const float* p) {

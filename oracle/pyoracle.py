@@ -93,6 +93,8 @@ def lib():
     L.orc_radical_inverse.argtypes = [C.c_uint32, C.c_uint32]
     L.orc_frame.argtypes = [f32p, f32p, f32p]
     L.orc_set_spp.argtypes = [vp, C.c_uint32]
+    L.orc_scene_export_counts.argtypes = [vp, u32p]
+    L.orc_scene_export.argtypes = [vp, f32p, f32p, f32p, f32p, u32p, f32p, f32p, f32p, f32p, f32p]
     L.orc_set_tally.argtypes = [C.c_int]
     L.orc_tallies.argtypes = [vp, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
     L.orc_fresnel.argtypes = [C.c_float, f32p, f32p]
@@ -302,6 +304,27 @@ class Scene:
         ev = lib().orc_render(self.h, MODES[mode] if isinstance(mode, str) else int(mode), w, h, cam,
                               1 if use_gi else 0, self.threads, y0, y1, film)
         return film.reshape(h, w, 3), int(ev)
+
+    def scene_desc(self):
+        """The parsed scene (objects, planes, subtractions, BVH nodes and order,
+        materials, lights, bounds, camera, render settings) as arrays in the
+        layout of the product library's scene descriptor."""
+        c = np.zeros(8, np.uint32)
+        lib().orc_scene_export_counts(self.h, c)
+        no, npl, ns, nn, nm, nq, nsl, npt = (int(x) for x in c)
+        a = {"objects": np.zeros((no, 13), np.float32), "planes": np.zeros((npl, 13), np.float32),
+             "subtractions": np.zeros((ns, 9), np.float32), "bvh_nodes": np.zeros((nn, 9), np.float32),
+             "bvh_order": np.zeros(no, np.uint32), "materials": np.zeros((nm, 8), np.float32),
+             "quad_lights": np.zeros((nq, 15), np.float32), "sphere_lights": np.zeros((nsl, 7), np.float32),
+             "point_lights": np.zeros((npt, 6), np.float32), "bounds": np.zeros(6, np.float32)}
+        lib().orc_scene_export(self.h, *(np.ascontiguousarray(a[k]) if a[k].size else np.zeros(16, a[k].dtype)
+                                         for k in ("objects", "planes", "subtractions", "bvh_nodes", "bvh_order",
+                                                   "materials", "quad_lights", "sphere_lights", "point_lights",
+                                                   "bounds")))
+        a["camera"] = np.asarray(self.camera, np.float32)
+        a["render"] = {"width": self.width, "height": self.height, "bounces": self.bounces,
+                       "surfels": self.surfels_requested, "exposure": float(self.exposure)}
+        return a
 
     def tallies(self):
         """(ops, primitive evaluations) of the last transfer's link shadow

@@ -222,3 +222,34 @@ def test_backward_state_and_material_checks():
     with pytest.raises(ia.IfxError) as e:
         p.backward(np.zeros((32, 32, 3), np.float32))
     assert e.value.status == -1
+
+
+def test_backward_from_a_caller_scene_and_surfels():
+    """A Rust host that keeps its loader: the scene as a descriptor (the
+    oracle's parse) and the surfels as arrays give the gradients of a context
+    that parsed the text and placed its own surfels, bit for bit."""
+    say("adjoint: descriptor scene + caller surfels")
+    from oracle import pyoracle as po
+
+    src = scenes.diff_cornell(8, surfels=1500, width=40, height=40)
+    a = ia.Context(0)
+    a.load_scene(src)
+    a.set_tuning("surfels_gpu", 0)
+    a.generate_surfels()
+    b = ia.Context(0)
+    b.set_scene(po.Scene(src).scene_desc())
+    assert b.scene_digest() == a.scene_digest()
+    b.set_surfels(*a.surfels())
+    assert np.array_equal(b.surfel_materials(), a.surfel_materials())
+    u = np.random.default_rng(3).normal(size=(40, 40, 3)).astype(np.float32)
+    out = []
+    for ctx in (a, b):
+        ctx.transfer(8)
+        ctx.solve(64)
+        f = ctx.shade("beauty", rgb8=False)["film"]
+        out.append((f, ctx.backward(u)))
+    (fa, ga), (fb, gb) = out
+    assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32))
+    assert np.array_equal(ga["albedo"].view(np.uint32), gb["albedo"].view(np.uint32))
+    assert np.array_equal(ga["emit"].view(np.uint32), gb["emit"].view(np.uint32))
+    assert np.abs(ga["albedo"]).max() > 0

@@ -106,3 +106,82 @@ def test_stage_order_is_enforced():
     with pytest.raises(ia.IfxError) as e:
         ctx.generate_surfels(10)
     assert e.value.status == -3
+
+
+# --------------------------------------------------------------------------- scene descriptor
+SCENE_FIXTURES = ["product.rad", "cornell_bench.rad", "stress.rad", "closed_box.rad", "alcove.rad", "dog.rad",
+                  "minimal.rad", "TEMPLATE.rad"]
+
+
+@pytest.mark.parametrize("name", SCENE_FIXTURES)
+def test_scene_descriptor_equals_the_loader(name):
+    """ifx_set_scene with the oracle's parse (the reference's Scene after its
+    parser and Bvh::build) gives the device records ifx_load_scene gives for
+    the text, with the caller's BVH and with the library's own."""
+    src = scene_text(name)
+    a = ia.Context(-1)
+    ia_info = a.load_scene(src)
+    desc = po.Scene(src).scene_desc()
+    for with_bvh in (True, False):
+        b = ia.Context(-1)
+        info = b.set_scene(desc, with_bvh=with_bvh)
+        assert b.scene_digest() == a.scene_digest(), with_bvh
+        assert info["n_bvh_nodes"] == ia_info["n_bvh_nodes"]
+
+
+def test_scene_descriptor_extension_scenes():
+    from interflect_amd import scenes
+
+    for src in (scenes.stress(prims=64), scenes.product_literal()):
+        a = ia.Context(-1)
+        a.load_scene(src)
+        b = ia.Context(-1)
+        b.set_scene(po.Scene(src).scene_desc())
+        assert b.scene_digest() == a.scene_digest()
+
+
+def test_scene_descriptor_rejects_malformed_input():
+    base = po.Scene(scene_text("product.rad")).scene_desc()
+
+    def bad(mutate, msg):
+        d = {k: (v.copy() if isinstance(v, np.ndarray) else dict(v) if isinstance(v, dict) else v)
+             for k, v in base.items()}
+        mutate(d)
+        c = ia.Context(-1)
+        with pytest.raises(ia.IfxError) as e:
+            c.set_scene(d)
+        assert e.value.status == -1 and msg in str(e.value), str(e.value)
+
+    def as_u32(a):
+        return a.view(np.uint32)
+
+    bad(lambda d: as_u32(d["objects"])[0].__setitem__(11, 99), "material index")
+    bad(lambda d: as_u32(d["objects"])[0].__setitem__(0, 2), "planes belong")
+    bad(lambda d: as_u32(d["objects"])[0].__setitem__(0, 9), "unknown primitive")
+    bad(lambda d: as_u32(d["objects"])[0].__setitem__(10, 5), "subtraction range")
+    bad(lambda d: d["bvh_order"].__setitem__(0, d["bvh_order"][1]), "BVH order")
+    bad(lambda d: as_u32(d["bvh_nodes"])[1].__setitem__(7, 7), "BVH leaf")
+    bad(lambda d: d.__setitem__("quad_lights", np.zeros((0, 15), np.float32)), "light")
+    bad(lambda d: d["render"].__setitem__("width", 0), "width")
+    bad(lambda d: d["bounds"].__setitem__(0, np.inf), "bounds")
+
+
+def test_set_surfels_assigns_placement_materials():
+    """ifx_set_surfels takes each surfel's material from the scene at its
+    position, which is what surfel placement assigns; ifx_set_surfel_materials
+    overrides it and checks the range."""
+    src = scene_text("product.rad")
+    a = ia.Context(-1)
+    a.load_scene(src)
+    a.generate_surfels(3000)
+    placed = a.surfel_materials()
+    b = ia.Context(-1)
+    b.load_scene(src)
+    b.set_surfels(*a.surfels())
+    assert np.array_equal(b.surfel_materials(), placed)
+    b.set_surfel_materials(np.zeros_like(placed))
+    assert not b.surfel_materials().any()
+    with pytest.raises(ia.IfxError):
+        b.set_surfel_materials(np.full_like(placed, 1000))
+    with pytest.raises(ia.IfxError):
+        b.set_surfel_materials(placed[:-1])
