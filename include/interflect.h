@@ -275,7 +275,18 @@ ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
  *   "surfels_gpu"   1 = place surfels on the device (default), 0 = on the host
  *   "side_stream"   1 = closure rays and direct-light terms on a second stream beside the
  *                   link visibility (default), 0 = everything on one stream
- *   "spp"           paths per pixel of IFX_MODE_REFERENCE (--spp, default 512) */
+ *   "spp"           paths per pixel of IFX_MODE_REFERENCE (--spp, default 512)
+ * and of the backward pass's memory (gradients are identical for every
+ * link_budget; adj_tile_rows and adj_seg fix its summation order):
+ *   "link_budget"   bytes the backward pass may use for its copy of the links
+ *                   (0 = no limit, default; up to 2^50).  Keeping them costs about
+ *                   32 bytes per link while they are transposed; beyond the budget
+ *                   every adjoint sweep recomputes the links tile by tile from the
+ *                   geometry instead.  A budget below one tile's working set (about
+ *                   16 B per row + 12 B per candidate + 40 B per link of the tile)
+ *                   makes ifx_backward return IFX_E_NOMEM.
+ *   "adj_tile_rows" surfel rows per link tile (0 = 4096)
+ *   "adj_seg"       transposed links per partial sum (0 = 1024) */
 ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value);
 ifx_status ifx_kernel_ops(ifx_ctx* ctx, const char* name, uint64_t* ops);
 
@@ -294,7 +305,7 @@ typedef struct {
     uint32_t iterations; /* adjoint sweeps */
     float residual;      /* max |delta mu| / max |mu| of the last sweep */
     float ms;            /* wall time of the call */
-    float reserved;
+    uint32_t link_tiles; /* row tiles recomputed every sweep (link_budget), 0 = links kept */
 } ifx_backward_stats;
 
 /* Parameter updates that keep the transfer (albedo and emission do not enter it). */

@@ -101,7 +101,7 @@ class BatchSceneStats(C.Structure):
 
 
 class BackwardStats(C.Structure):
-    _fields_ = [("iterations", C.c_uint32), ("residual", C.c_float), ("ms", C.c_float), ("reserved", C.c_float)]
+    _fields_ = [("iterations", C.c_uint32), ("residual", C.c_float), ("ms", C.c_float), ("link_tiles", C.c_uint32)]
 
 
 def lib():
@@ -495,7 +495,8 @@ class Context:
         da, de = self._grad_shapes()
         st = BackwardStats()
         self._check(lib().ifx_backward(self._h, _ptr(u), _ptr(da), _ptr(de), C.byref(st)))
-        return {"albedo": da, "emit": de, "iterations": st.iterations, "residual": st.residual, "ms": st.ms}
+        return {"albedo": da, "emit": de, "iterations": st.iterations, "residual": st.residual, "ms": st.ms,
+                "link_tiles": st.link_tiles}
 
     def backward_surfels(self, dL_dirr):
         """The solve's share for dL/dirr (N x 3): gradients and the adjoint state mu."""
@@ -505,7 +506,7 @@ class Context:
         st = BackwardStats()
         self._check(lib().ifx_backward_surfels(self._h, _ptr(u), _ptr(da), _ptr(de), _ptr(mu), C.byref(st)))
         return {"albedo": da, "emit": de, "mu": mu, "iterations": st.iterations, "residual": st.residual,
-                "ms": st.ms}
+                "ms": st.ms, "link_tiles": st.link_tiles}
 
     def surfel_materials(self):
         m = np.zeros(self.num_surfels(), np.uint32)

@@ -187,13 +187,30 @@ __global__ void k_fix_to_float(uint32_t N, const unsigned long long* __restrict_
 }
 
 // ---------------------------------------------------------------- transposed links
-__global__ void k_link_rows(uint32_t N, const uint32_t* __restrict__ lstart, uint32_t* lrow, uint32_t* iota) {
+// Rows row0 .. row0 + N of a CSR (starts lstart): each link's row and its
+// own position.
+__global__ void k_link_rows(uint32_t N, const uint32_t* __restrict__ lstart, uint32_t row0, uint32_t* lrow,
+                            uint32_t* iota) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     for (uint32_t k = lstart[i], e = lstart[i + 1]; k < e; ++k) {
-        lrow[k] = i;
+        lrow[k] = row0 + i;
         iota[k] = k;
     }
+}
+
+// Links per (cluster, row tile): cnt2[c * T + row / R].
+__global__ void k_tile_col_count(uint64_t L, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ lrow,
+                                 uint32_t R, uint32_t T, uint32_t* cnt2) {
+    uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < L) atomicAdd(&cnt2[(uint64_t)idx[k] * T + lrow[k] / R], 1u);
+}
+
+// One tile's column counts into column t of cnt2.
+__global__ void k_tile_col_put(uint32_t C, uint32_t T, uint32_t t, const uint32_t* __restrict__ colcnt,
+                               uint32_t* cnt2) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < C) cnt2[(uint64_t)c * T + t] = colcnt[c];
 }
 
 __global__ void k_col_count(uint64_t L, const uint32_t* __restrict__ idx, uint32_t* cnt) {
@@ -210,20 +227,35 @@ __global__ void k_csc_fill(uint64_t L, const uint32_t* __restrict__ perm, const 
     cw[k] = w[s];
 }
 
-__global__ void k_seg_count(uint32_t C, const uint32_t* __restrict__ colstart, uint32_t S, uint32_t* segcnt) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C) return;
-    uint32_t len = colstart[c + 1] - colstart[c];
-    segcnt[c] = (len + S - 1) / S;
+// Segments: every (cluster, row tile) run of the transposed links is cut
+// into pieces of S entries, so a segment never spans two tiles and the
+// summation order is the same whether the links stay resident or are
+// recomputed tile by tile.
+__global__ void k_seg_count(uint64_t n, const uint32_t* __restrict__ cnt2, uint32_t S, uint32_t* segn) {
+    uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) segn[q] = (cnt2[q] + S - 1) / S;
 }
 
-__global__ void k_seg_make(uint32_t C, const uint32_t* __restrict__ colstart, const uint32_t* __restrict__ segfirst,
-                           uint32_t S, uint2* seg) {
+// segfirst[c] = first segment of cluster c (c = C: the total).
+__global__ void k_seg_first(uint32_t C, uint32_t T, const uint32_t* __restrict__ segoff, uint32_t* segfirst) {
+    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c <= C) segfirst[c] = segoff[(uint64_t)c * T];
+}
+
+// Resident links: the ranges of every segment, cluster by cluster, tile by tile.
+__global__ void k_seg_make(uint32_t C, uint32_t T, const uint32_t* __restrict__ colstart,
+                           const uint32_t* __restrict__ cnt2, const uint32_t* __restrict__ segoff, uint32_t S,
+                           uint2* seg) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
-    const uint32_t b = colstart[c], e = colstart[c + 1];
-    uint32_t s = segfirst[c];
-    for (uint32_t k = b; k < e; k += S, ++s) seg[s] = make_uint2(k, e - k < S ? e : k + S);
+    uint32_t b = colstart[c];
+    for (uint32_t t = 0; t < T; ++t) {
+        const uint64_t q = (uint64_t)c * T + t;
+        const uint32_t n = cnt2[q];
+        uint32_t s = segoff[q];
+        for (uint32_t k = 0; k < n; k += S, ++s) seg[s] = make_uint2(b + k, n - k < S ? b + n : b + k + S);
+        b += n;
+    }
 }
 
 // Surfel -> cluster and the aggregation weight a_m / max(A_c, 1e-12).
@@ -302,22 +334,56 @@ __global__ __launch_bounds__(kBlock) void k_adj_surfels(AdjArgs A, uint32_t it) 
     }
 }
 
-// Sweep part 2: one wavefront per segment sums w * v[row] over its entries,
-// lane-strided, then a fixed xor tree.
+// Sum of w * v[row] over the transposed entries b .. e (at most S of them):
+// entry b + o goes to lane o % 64 in entry order, then a fixed xor tree.
+__device__ __forceinline__ f3 seg_sum(const float4* __restrict__ v, const uint32_t* __restrict__ crow,
+                                      const float* __restrict__ cw, uint32_t b, uint32_t e, uint32_t lane) {
+    f3 acc = spl(0.0f);
+    for (uint32_t k = b + lane; k < e; k += 64u) {
+        const f3 x = ld3(v[crow[k]]);
+        const float w = cw[k];
+        acc.x = fmaf(x.x, w, acc.x);
+        acc.y = fmaf(x.y, w, acc.y);
+        acc.z = fmaf(x.z, w, acc.z);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        acc.x += __shfl_xor(acc.x, o, 64);
+        acc.y += __shfl_xor(acc.y, o, 64);
+        acc.z += __shfl_xor(acc.z, o, 64);
+    }
+    return acc;
+}
+
+// Sweep part 2, resident links: one wavefront per segment.
 __global__ __launch_bounds__(kBlock) void k_adj_segments(AdjArgs A, uint32_t it) {
     if (adj_converged(A, it)) return;
     const uint32_t lane = lane_id();
     const uint32_t gw = (blockIdx.x * kBlock + threadIdx.x) >> 6, nw = (gridDim.x * kBlock) >> 6;
     for (uint32_t s = gw; s < A.nseg; s += nw) {
         const uint2 r = A.seg[s];
-        f3 acc = spl(0.0f);
-        for (uint32_t k = r.x + lane; k < r.y; k += 64u) acc = acc + ld3(A.v[A.crow[k]]) * A.cw[k];
-        for (int o = 32; o > 0; o >>= 1) {
-            acc.x += __shfl_xor(acc.x, o, 64);
-            acc.y += __shfl_xor(acc.y, o, 64);
-            acc.z += __shfl_xor(acc.z, o, 64);
-        }
+        const f3 acc = seg_sum(A.v, A.crow, A.cw, r.x, r.y, lane);
         if (lane == 0) A.segsum[s] = st4(acc);
+    }
+}
+
+// Sweep part 2, one recomputed tile t of T: one wavefront per cluster walks
+// the tile's transposed links of that cluster (tcolstart) in pieces of S,
+// the same segments as the resident layout, written at segoff[c * T + t].
+__global__ __launch_bounds__(kBlock) void k_adj_tile(AdjArgs A, uint32_t it, uint32_t S, uint32_t T, uint32_t t,
+                                                     const uint32_t* __restrict__ tcolstart,
+                                                     const uint32_t* __restrict__ tcrow,
+                                                     const float* __restrict__ tcw,
+                                                     const uint32_t* __restrict__ segoff) {
+    if (adj_converged(A, it)) return;
+    const uint32_t lane = lane_id();
+    const uint32_t gw = (blockIdx.x * kBlock + threadIdx.x) >> 6, nw = (gridDim.x * kBlock) >> 6;
+    for (uint32_t c = gw; c < A.C; c += nw) {
+        const uint32_t b = tcolstart[c], e = tcolstart[c + 1];
+        uint32_t s = segoff[(uint64_t)c * T + t];
+        for (uint32_t k = b; k < e; k += S, ++s) {
+            const f3 acc = seg_sum(A.v, tcrow, tcw, k, e - k < S ? e : k + S, lane);
+            if (lane == 0) A.segsum[s] = st4(acc);
+        }
     }
 }
 
@@ -462,25 +528,68 @@ int Engine::download_direct_geometry(float* g, std::string& err) {
     return 0;
 }
 
-// Transposed links (cluster-major, rows ascending within a cluster), their
-// fixed-size segments and the surfel -> cluster map; once per transfer.
+int Engine::tile_transpose(uint32_t r0, uint32_t nr, uint32_t L, std::string& err) {
+    const uint32_t C = n_clusters;
+    CK(d_tcolcnt.ensure(4ull * (C + 1)));
+    CK(d_tcolstart.ensure(4ull * (C + 1)));
+    CK(d_tlrow.ensure(4ull * L));
+    CK(d_tperm.ensure(4ull * L));
+    CK(d_tperm2.ensure(4ull * L));
+    CK(d_tkeys.ensure(4ull * L));
+    CK(d_tcrow.ensure(4ull * L));
+    CK(d_tcw.ensure(4ull * L));
+    CK(hipMemsetAsync(d_tcolcnt.p, 0, 4ull * (C + 1), stream));
+    if (L) {
+        k_link_rows<<<blocks_for(nr, 256), 256, 0, stream>>>(nr, d_tlstart.as<uint32_t>(), r0, d_tlrow.as<uint32_t>(),
+                                                             d_tperm.as<uint32_t>());
+        k_col_count<<<blocks_for(L, 256), 256, 0, stream>>>(L, d_tidx.as<uint32_t>(), d_tcolcnt.as<uint32_t>());
+        if (int rc = sort_u32(d_tidx.as<uint32_t>(), d_tperm.as<uint32_t>(), d_tkeys.as<uint32_t>(),
+                              d_tperm2.as<uint32_t>(), L, bits_for(C ? C - 1 : 1), err))
+            return rc;
+        k_csc_fill<<<blocks_for(L, 256), 256, 0, stream>>>(L, d_tperm2.as<uint32_t>(), d_tlrow.as<uint32_t>(),
+                                                           d_tw.as<float>(), d_tcrow.as<uint32_t>(),
+                                                           d_tcw.as<float>());
+    }
+    if (int rc = scan_u32(d_tcolcnt.as<uint32_t>(), d_tcolstart.as<uint32_t>(), (C + 1), err)) return rc;
+    CK(hipGetLastError());
+    return 0;
+}
+
+void Engine::release_link_tiles() {
+    for (DevBuf* b : {&d_tcount, &d_tstart, &d_tci, &d_trow, &d_tv, &d_tkeep, &d_tlstart, &d_tidx, &d_tw, &d_tlrow,
+                      &d_tperm, &d_tperm2, &d_tkeys, &d_tcrow, &d_tcw, &d_tcolcnt, &d_tcolstart, &d_tqueue})
+        b->release();
+    tile_m_.clear();
+    tile_l_.clear();
+}
+
+// The backward pass's link layout, once per transfer.  Resident: the links
+// transposed (cluster-major, rows ascending within a cluster), about 32 bytes
+// per link while they are built.  When link_budget is set and that exceeds
+// it, nothing is kept: the surfel rows are cut into tiles of adj_tile_rows
+// and every sweep recomputes each tile's links from the geometry with the
+// transfer's kernels (bit-identical to the transfer's links) and transposes
+// them; this pass records each tile's candidate and link counts.  Either way
+// the segments are the (cluster, row tile) runs cut into adj_seg entries, so
+// both layouts sum the same entries in the same order.
 int Engine::adjoint_prepare(std::string& err) {
     if (adj_gen_ == transfer_gen_) return 0;
     const uint32_t N = (uint32_t)surfels.size(), C = n_clusters;
     const uint64_t L = n_links;
     const uint32_t S = adj_seg ? adj_seg : 1024u;
+    const uint32_t R = adj_tile_rows ? adj_tile_rows : 4096u;
+    const uint32_t T = N ? (N + R - 1) / R : 1u;
+    const uint64_t CT = (uint64_t)C * T;
+    if (CT + 1 >= (1ull << 31)) {
+        err = "clusters x row tiles exceeds 2^31 (raise adj_tile_rows)";
+        return -6;
+    }
     CK(d_scl.ensure(4ull * N));
     CK(d_sscale.ensure(4ull * N));
-    CK(d_colcnt.ensure(4ull * (C + 1)));
-    CK(d_colstart.ensure(4ull * (C + 1)));
-    CK(d_segcnt.ensure(4ull * (C + 1)));
     CK(d_segfirst.ensure(4ull * (C + 1)));
-    CK(d_lrow.ensure(4ull * L));
-    CK(d_perm.ensure(4ull * L));
-    CK(d_perm2.ensure(4ull * L));
-    CK(d_ckeyt.ensure(4ull * L));
-    CK(d_crow.ensure(4ull * L));
-    CK(d_cw.ensure(4ull * L));
+    CK(d_cnt2.ensure(4ull * (CT + 1)));
+    CK(d_segn.ensure(4ull * (CT + 1)));
+    CK(d_segoff.ensure(4ull * (CT + 1)));
     if (N) {
         launch("adj_prep", [&] {
             k_fill_u32<<<blocks_for(N, 256), 256, 0, stream>>>(d_scl.as<uint32_t>(), N, C);
@@ -491,43 +600,91 @@ int Engine::adjoint_prepare(std::string& err) {
                     d_scl.as<uint32_t>(), d_sscale.as<float>());
         });
     }
-    CK(hipMemsetAsync(d_colcnt.p, 0, 4ull * (C + 1), stream));
-    if (L) {
-        launch("adj_prep", [&] {
-            k_link_rows<<<blocks_for(N, 256), 256, 0, stream>>>(N, d_lstart.as<uint32_t>(), d_lrow.as<uint32_t>(),
-                                                                d_perm.as<uint32_t>());
-            k_col_count<<<blocks_for(L, 256), 256, 0, stream>>>(L, d_idx.as<uint32_t>(), d_colcnt.as<uint32_t>());
-        });
-        const int kb = (int)bits_for(C ? C - 1 : 1);
-        if (int rc_s = sort_u32(d_idx.as<uint32_t>(), d_perm.as<uint32_t>(), d_ckeyt.as<uint32_t>(),
-                           d_perm2.as<uint32_t>(), L, (uint32_t)kb, err))
-            return rc_s;
-        launch("adj_prep", [&] {
-            k_csc_fill<<<blocks_for(L, 256), 256, 0, stream>>>(L, d_perm2.as<uint32_t>(), d_lrow.as<uint32_t>(),
-                                                               d_w.as<float>(), d_crow.as<uint32_t>(),
-                                                               d_cw.as<float>());
-        });
+    CK(hipMemsetAsync(d_cnt2.p, 0, 4ull * (CT + 1), stream));
+    const bool tiled = link_budget && L && 32ull * L > link_budget;
+    if (!tiled) {
+        release_link_tiles();
+        CK(d_colcnt.ensure(4ull * (C + 1)));
+        CK(d_colstart.ensure(4ull * (C + 1)));
+        CK(d_lrow.ensure(4ull * L));
+        CK(d_perm.ensure(4ull * L));
+        CK(d_perm2.ensure(4ull * L));
+        CK(d_ckeyt.ensure(4ull * L));
+        CK(d_crow.ensure(4ull * L));
+        CK(d_cw.ensure(4ull * L));
+        CK(hipMemsetAsync(d_colcnt.p, 0, 4ull * (C + 1), stream));
+        if (L) {
+            launch("adj_prep", [&] {
+                k_link_rows<<<blocks_for(N, 256), 256, 0, stream>>>(N, d_lstart.as<uint32_t>(), 0u,
+                                                                    d_lrow.as<uint32_t>(), d_perm.as<uint32_t>());
+                k_col_count<<<blocks_for(L, 256), 256, 0, stream>>>(L, d_idx.as<uint32_t>(),
+                                                                    d_colcnt.as<uint32_t>());
+                k_tile_col_count<<<blocks_for(L, 256), 256, 0, stream>>>(L, d_idx.as<uint32_t>(),
+                                                                         d_lrow.as<uint32_t>(), R, T,
+                                                                         d_cnt2.as<uint32_t>());
+            });
+            const int kb = (int)bits_for(C ? C - 1 : 1);
+            if (int rc_s = sort_u32(d_idx.as<uint32_t>(), d_perm.as<uint32_t>(), d_ckeyt.as<uint32_t>(),
+                                    d_perm2.as<uint32_t>(), L, (uint32_t)kb, err))
+                return rc_s;
+            launch("adj_prep", [&] {
+                k_csc_fill<<<blocks_for(L, 256), 256, 0, stream>>>(L, d_perm2.as<uint32_t>(), d_lrow.as<uint32_t>(),
+                                                                   d_w.as<float>(), d_crow.as<uint32_t>(),
+                                                                   d_cw.as<float>());
+            });
+        }
+        if (int rc_s = scan_u32(d_colcnt.as<uint32_t>(), d_colstart.as<uint32_t>(), (C + 1), err)) return rc_s;
+    } else {
+        for (DevBuf* b : {&d_lrow, &d_perm, &d_perm2, &d_ckeyt, &d_crow, &d_cw, &d_seg}) b->release();
+        tile_m_.assign(T, 0u);
+        tile_l_.assign(T, 0u);
+        uint64_t ws_max = 0;
+        for (uint32_t t = 0; t < T; ++t) {
+            const uint32_t r0 = t * R, nr = std::min(R, N - r0);
+            uint32_t M = 0, Lt = 0;
+            if (int rc = tile_links(r0, nr, M, Lt, false, err)) return rc;
+            // one tile's working set: row arrays, candidates, links, transpose and sort
+            const uint64_t ws = 16ull * (R + 1) + 12ull * M + 40ull * Lt + 8ull * (C + 1);
+            ws_max = std::max(ws_max, ws);
+            if (ws_max > link_budget) {
+                release_link_tiles();
+                err = "link_budget " + std::to_string(link_budget) + " bytes is below one link tile's " +
+                      std::to_string(ws_max) + " bytes (lower adj_tile_rows)";
+                return -5;
+            }
+            tile_m_[t] = M;
+            tile_l_[t] = Lt;
+            if (int rc = tile_transpose(r0, nr, Lt, err)) return rc;
+            if (C)
+                k_tile_col_put<<<blocks_for(C, 256), 256, 0, stream>>>(C, T, t, d_tcolcnt.as<uint32_t>(),
+                                                                       d_cnt2.as<uint32_t>());
+        }
     }
-    if (int rc_s = scan_u32(d_colcnt.as<uint32_t>(), d_colstart.as<uint32_t>(), (C + 1), err))
-        return rc_s;
-    CK(hipMemsetAsync(d_segcnt.p, 0, 4ull * (C + 1), stream));
-    if (C)
-        launch("adj_prep", [&] {
-            k_seg_count<<<blocks_for(C, 256), 256, 0, stream>>>(C, d_colstart.as<uint32_t>(), S,
-                                                                d_segcnt.as<uint32_t>());
-        });
-    if (int rc_s = scan_u32(d_segcnt.as<uint32_t>(), d_segfirst.as<uint32_t>(), (C + 1), err))
-        return rc_s;
+    launch("adj_prep", [&] {
+        k_seg_count<<<blocks_for(CT + 1, 256), 256, 0, stream>>>(CT + 1, d_cnt2.as<uint32_t>(), S,
+                                                                 d_segn.as<uint32_t>());
+    });
+    if (int rc_s = scan_u32(d_segn.as<uint32_t>(), d_segoff.as<uint32_t>(), CT + 1, err)) return rc_s;
+    launch("adj_prep", [&] {
+        k_seg_first<<<blocks_for(C + 1, 256), 256, 0, stream>>>(C, T, d_segoff.as<uint32_t>(),
+                                                                d_segfirst.as<uint32_t>());
+    });
     uint32_t nseg = 0;
-    CK(hipMemcpyAsync(&nseg, d_segfirst.as<uint32_t>() + C, 4, hipMemcpyDeviceToHost, stream));
+    CK(hipMemcpyAsync(&nseg, d_segoff.as<uint32_t>() + CT, 4, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
-    CK(d_seg.ensure(8ull * (nseg + 1)));
     CK(d_segsum.ensure(16ull * (nseg + 1)));
-    if (C && nseg)
-        launch("adj_prep", [&] {
-            k_seg_make<<<blocks_for(C, 256), 256, 0, stream>>>(C, d_colstart.as<uint32_t>(),
-                                                               d_segfirst.as<uint32_t>(), S, d_seg.as<uint2>());
-        });
+    if (!tiled) {
+        CK(d_seg.ensure(8ull * (nseg + 1)));
+        if (C && nseg)
+            launch("adj_prep", [&] {
+                k_seg_make<<<blocks_for(C, 256), 256, 0, stream>>>(C, T, d_colstart.as<uint32_t>(),
+                                                                   d_cnt2.as<uint32_t>(), d_segoff.as<uint32_t>(),
+                                                                   S, d_seg.as<uint2>());
+            });
+    }
+    CK(hipGetLastError());
+    adj_tiled_ = tiled;
+    adj_tiles_ = T;
     adj_nseg_ = nseg;
     adj_gen_ = transfer_gen_;
     return 0;
@@ -565,8 +722,45 @@ int Engine::adjoint_solve(BackwardOut& out, std::string& err) {
     // Sweeps are enqueued in groups; between groups the host reads the
     // maxima and stops enqueuing once the adjoint has converged.
     std::vector<uint32_t> dm(2ull * iters, 0u);
-    const uint32_t group = 8;
     out.iterations = iters;
+    out.link_tiles = adj_tiled_ ? adj_tiles_ : 0u;
+    if (adj_tiled_) {
+        // Recomputed links: the host checks convergence after every sweep, so
+        // no tile is rebuilt for a sweep that will not run.
+        const uint32_t S = adj_seg ? adj_seg : 1024u;
+        const uint32_t R = adj_tile_rows ? adj_tile_rows : 4096u;
+        const uint32_t T = adj_tiles_;
+        const uint32_t g_c = jacobi_blocks ? jacobi_blocks : std::max<uint32_t>(1u, blocks_for(64ull * C, kBlock));
+        for (uint32_t it = 0; it < iters; ++it) {
+            launch("adjoint", [&] { k_adj_surfels<<<g_s, kBlock, 0, stream>>>(A, it); });
+            CK(hipMemcpyAsync(dm.data() + 2ull * it, A.dmax + 2ull * it, 8, hipMemcpyDeviceToHost, stream));
+            CK(hipStreamSynchronize(stream));
+            if (it >= 1 && bitsf(dm[2 * it]) <= adj_tol * bitsf(dm[2 * it + 1])) {
+                out.iterations = it;
+                break;
+            }
+            if (it + 1 == iters) break;
+            for (uint32_t t = 0; t < T; ++t) {
+                const uint32_t r0 = t * R, nr = std::min(R, N - r0);
+                uint32_t M = tile_m_[t], Lt = tile_l_[t];
+                int rc = 0;
+                launch("adj_tiles", [&] {
+                    rc = tile_links(r0, nr, M, Lt, true, err);
+                    if (!rc) rc = tile_transpose(r0, nr, Lt, err);
+                    if (!rc)
+                        k_adj_tile<<<g_c, kBlock, 0, stream>>>(A, it, S, T, t, d_tcolstart.as<uint32_t>(),
+                                                               d_tcrow.as<uint32_t>(), d_tcw.as<float>(),
+                                                               d_segoff.as<uint32_t>());
+                });
+                if (rc) return rc;
+            }
+        }
+        const uint32_t last = out.iterations < iters ? out.iterations : iters - 1;
+        const float mmax = bitsf(dm[2 * last + 1]);
+        out.residual = mmax > 0.0f ? bitsf(dm[2 * last]) / mmax : 0.0f;
+        return 0;
+    }
+    const uint32_t group = 8;
     for (uint32_t it0 = 0; it0 < iters; it0 += group) {
         const uint32_t it1 = std::min(iters, it0 + group);
         launch("adjoint", [&] {

@@ -612,11 +612,18 @@ ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value) {
     return guarded(ctx, [&]() -> ifx_status {
         if (!key) return IFX_E_INVALID;
         std::string k(key);
-        if (value < 0 || value > (int64_t)1 << 30) {
+        const int64_t hi = k == "link_budget" ? (int64_t)1 << 50 : (int64_t)1 << 30;
+        if (value < 0 || value > hi) {
             ctx->err = "tuning value out of range";
             return IFX_E_INVALID;
         }
         if (k == "lds") ctx->eng.use_lds = value != 0;
+        else if (k == "link_budget" || k == "adj_tile_rows" || k == "adj_seg") {
+            if (k == "link_budget") ctx->eng.link_budget = (uint64_t)value;
+            else if (k == "adj_tile_rows") ctx->eng.adj_tile_rows = (uint32_t)value;
+            else ctx->eng.adj_seg = (uint32_t)value;
+            ctx->eng.invalidate_adjoint();
+        }
         else if (k == "vis_blocks") ctx->eng.vis_blocks = (uint32_t)value;
         else if (k == "ff_blocks") ctx->eng.ff_blocks = (uint32_t)value;
         else if (k == "jacobi_blocks") ctx->eng.jacobi_blocks = (uint32_t)value;
@@ -689,6 +696,7 @@ ifx_status ifx_backward(ifx_ctx* ctx, const float* dL_dfilm, float* dL_dalbedo, 
             o->iterations = r.iterations;
             o->residual = r.residual;
             o->ms = (float)ms_since(t0);
+            o->link_tiles = r.link_tiles;
         }
         return IFX_OK;
     });
@@ -707,6 +715,7 @@ ifx_status ifx_backward_surfels(ifx_ctx* ctx, const float* dL_dirr, float* dL_da
             o->iterations = r.iterations;
             o->residual = r.residual;
             o->ms = (float)ms_since(t0);
+            o->link_tiles = r.link_tiles;
         }
         return IFX_OK;
     });

@@ -58,6 +58,7 @@ struct SolveOut {
 struct BackwardOut {
     uint32_t iterations = 0;  // adjoint Jacobi sweeps
     float residual = 0;       // last max |delta mu| / max |mu|
+    uint32_t link_tiles = 0;  // row tiles recomputed every sweep, 0 = links kept
 };
 
 struct CamV {
@@ -181,6 +182,14 @@ class Engine {
     uint32_t adj_max_iters = 1000;  // adjoint sweeps at most
     float adj_tol = 1e-6f;          // adjoint stops once max |delta mu| <= adj_tol * max |mu|
     uint32_t adj_seg = 1024;        // transposed-link entries per partial sum (fixes the summation order)
+    uint32_t adj_tile_rows = 4096;  // surfel rows per link tile (segments never span two tiles)
+    // Bytes the backward pass may hold for its own copy of the links (0 = no
+    // limit).  Beyond it, every adjoint sweep recomputes the links tile by
+    // tile from the geometry instead of keeping a transposed copy.
+    uint64_t link_budget = 0;
+    // The backward pass's link layout depends on adj_seg, adj_tile_rows and
+    // link_budget: call after changing them.
+    void invalidate_adjoint() { adj_gen_ = ~0ull; }
 
    private:
     struct Lds {
@@ -194,6 +203,15 @@ class Engine {
     int shade_setup(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, ShadeArgs& A, std::string& err);
     CamV cam_basis(const HCamera& cam, uint32_t W, uint32_t H) const;
     int adjoint_prepare(std::string& err);
+    // Links of rows r0 .. r0 + nr recomputed from the geometry with the
+    // transfer's kernels: the tile-local CSR d_tlstart / d_tidx / d_tw.
+    // sized: the tile buffers already hold this tile (M candidates, L links);
+    // else M and L are read back and the buffers grown.
+    int tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool sized, std::string& err);
+    // The tile's links transposed (cluster-major, rows ascending), with
+    // column starts d_tcolstart.
+    int tile_transpose(uint32_t r0, uint32_t nr, uint32_t L, std::string& err);
+    void release_link_tiles();
     int build_sliced(std::string& err);
     int adjoint_solve(BackwardOut& out, std::string& err);
     int adjoint_params(bool pixels, uint64_t npx, float* d_albedo, float* d_emit, std::string& err);
@@ -278,6 +296,12 @@ class Engine {
         d_segcnt, d_segfirst, d_seg, d_segsum, d_ubar, d_uacc, d_umax, d_mu, d_v, d_dmax, d_pmat, d_pdr, d_pde, d_sdr,
         d_sde, d_partial;
     DevBuf d_ufilm, d_stage, d_sbase, d_sidx, d_sw;
+    // link tiles of the budgeted backward pass
+    DevBuf d_cnt2, d_segn, d_segoff, d_tcount, d_tstart, d_tci, d_trow, d_tv, d_tkeep, d_tlstart, d_tidx, d_tw,
+        d_tlrow, d_tperm, d_tperm2, d_tkeys, d_tcrow, d_tcw, d_tcolcnt, d_tcolstart, d_tqueue;
+    bool adj_tiled_ = false;
+    uint32_t adj_tiles_ = 0;
+    std::vector<uint32_t> tile_m_, tile_l_;
     uint64_t sliced_gen_ = ~0ull, sliced_cells_ = 0;
     uint64_t transfer_gen_ = 0, adj_gen_ = ~0ull;
     uint32_t adj_nseg_ = 0;

@@ -221,12 +221,13 @@ __global__ void k_cluster_compact(uint32_t nseg, const uint32_t* valid, const ui
 }
 
 // ====================================================================== transfer: links
-// One wavefront per surfel row; lanes sweep the clusters 64 at a time and a
-// ballot keeps survivors in ascending cluster order.  EMIT=false counts,
-// EMIT=true writes candidates at the row's scanned offset.
+// One wavefront per surfel row (rows row0 .. row0 + n; count and start are
+// indexed from row0); lanes sweep the clusters 64 at a time and a ballot
+// keeps survivors in ascending cluster order.  EMIT=false counts, EMIT=true
+// writes candidates at the row's scanned offset.
 template <bool EMIT>
 __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, const float4* __restrict__ snrm,
-                                               uint32_t N, const float4* __restrict__ cpos,
+                                               uint32_t row0, uint32_t n, const float4* __restrict__ cpos,
                                                const float4* __restrict__ cnrm, uint32_t C, uint32_t* count,
                                                const uint32_t* __restrict__ start, uint32_t* cand_ci,
                                                uint32_t* cand_row, const uint32_t* __restrict__ morton,
@@ -234,9 +235,10 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
     uint32_t lane = lane_id();
     uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     uint32_t nwaves = (gridDim.x * kBlock) >> 6;
-    for (uint32_t i = wave; i < N; i += nwaves) {
+    for (uint32_t j = wave; j < n; j += nwaves) {
+        const uint32_t i = row0 + j;
         f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
-        uint32_t base = EMIT ? start[i] : 0u;
+        uint32_t base = EMIT ? start[j] : 0u;
         uint32_t mc = (EMIT && key) ? morton[i] : 0u;
         uint32_t run = 0;
         for (uint32_t c0 = 0; c0 < C; c0 += 64) {
@@ -259,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
             }
             run += (uint32_t)__popcll(m);
         }
-        if (!EMIT && lane == 0) count[i] = run;
+        if (!EMIT && lane == 0) count[j] = run;
     }
 }
 
@@ -466,6 +468,16 @@ __global__ void k_compact(uint32_t N, const uint32_t* __restrict__ cstart, const
         }
         pos += (uint32_t)__popcll(m);
     }
+}
+
+// Kept candidates (v > 0) of each row, the count k_row_sum reports.
+__global__ void k_keep_count(uint32_t n, const uint32_t* __restrict__ cstart, const float* __restrict__ cand_v,
+                             uint32_t* keep) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t k = 0;
+    for (uint32_t q = cstart[j], e = cstart[j + 1]; q < e; ++q) k += cand_v[q] > 0.0f ? 1u : 0u;
+    keep[j] = k;
 }
 
 // ====================================================================== solve
@@ -1270,7 +1282,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     CK(hipMemsetAsync(d_ccount.as<uint32_t>() + N, 0, 4, stream));
     const uint32_t ffb = ff_blocks ? ff_blocks : std::min<uint32_t>(blocks_for(N, 4), 8192);
     launch("ff_count", [&] {
-        k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
+        k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, 0u, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
                                                  d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr,
                                                  nullptr);
     });
@@ -1297,7 +1309,8 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         CK(d_vorder.ensure(4ull * M));
     }
     launch("ff_emit", [&] {
-        k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C, nullptr,
+        k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, 0u, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
+                                                nullptr,
                                                 d_cstart.as<uint32_t>(), d_cand_ci.as<uint32_t>(),
                                                 d_cand_row.as<uint32_t>(), d_morton.as<uint32_t>(),
                                                 sort_cand ? d_vkey.as<uint32_t>() : nullptr,
@@ -1649,6 +1662,64 @@ int Engine::solve_noalloc(uint32_t bounces, float4* direct, float4* irr, float4*
         }
         out.residual = bitsf(res[out.iterations - 1]);
     }
+    return 0;
+}
+
+int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool sized, std::string& err) {
+    const uint32_t C = n_clusters;
+    const float diag = scene.diag();
+    if (!sized) {
+        CK(d_tcount.ensure(4ull * (nr + 1)));
+        CK(d_tstart.ensure(4ull * (nr + 1)));
+        CK(d_tkeep.ensure(4ull * (nr + 1)));
+        CK(d_tlstart.ensure(4ull * (nr + 1)));
+        CK(d_tqueue.ensure(16));
+    }
+    const float4* spos = d_spos.as<float4>();
+    const float4* snrm = d_snrm.as<float4>();
+    const uint32_t ffb = ff_blocks ? ff_blocks : std::min<uint32_t>(blocks_for(nr, 4), 8192);
+    CK(hipMemsetAsync(d_tcount.as<uint32_t>() + nr, 0, 4, stream));
+    k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, r0, nr, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
+                                             d_tcount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                                             nullptr);
+    if (int rc = scan_u32(d_tcount.as<uint32_t>(), d_tstart.as<uint32_t>(), nr + 1, err)) return rc;
+    if (!sized) {
+        CK(hipMemcpyAsync(&M, d_tstart.as<uint32_t>() + nr, 4, hipMemcpyDeviceToHost, stream));
+        CK(hipStreamSynchronize(stream));
+        CK(d_tci.ensure(4ull * M));
+        CK(d_trow.ensure(4ull * M));
+        CK(d_tv.ensure(4ull * M));
+    }
+    k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, r0, nr, d_cpos.as<float4>(), d_cnrm.as<float4>(), C, nullptr,
+                                            d_tstart.as<uint32_t>(), d_tci.as<uint32_t>(), d_trow.as<uint32_t>(),
+                                            nullptr, nullptr, nullptr);
+    if (M) {
+        const uint32_t persist = (uint32_t)n_cu_ * 8u;
+        const uint32_t visb = vis_blocks ? vis_blocks : std::max(1u, std::min<uint32_t>(persist, blocks_for(M, kBlock)));
+        const Lds ld = plan_lds(false, false);
+        const LightsView Lg{};
+        const SceneView S = dev_view();
+        unsigned long long* ctr = d_counter.as<unsigned long long>();
+        CK(hipMemsetAsync(d_tqueue.p, 0, 4, stream));
+        LAUNCH_CM(k_visibility, 0, ld.staged, visb, ld.bytes, S, Lg, ld.plan, spos, snrm, d_cpos.as<float4>(),
+                  d_cnrm.as<float4>(), d_tci.as<uint32_t>(), 0u, d_trow.as<uint32_t>(), M, diag * 5e-4f,
+                  diag * 1e-3f, d_tv.as<float>(), d_tqueue.as<uint32_t>(), wave_chunk(M, visb), ctr + 20, nullptr);
+    }
+    CK(hipMemsetAsync(d_tkeep.as<uint32_t>() + nr, 0, 4, stream));
+    k_keep_count<<<blocks_for(nr, 256), 256, 0, stream>>>(nr, d_tstart.as<uint32_t>(), d_tv.as<float>(),
+                                                          d_tkeep.as<uint32_t>());
+    if (int rc = scan_u32(d_tkeep.as<uint32_t>(), d_tlstart.as<uint32_t>(), nr + 1, err)) return rc;
+    if (!sized) {
+        CK(hipMemcpyAsync(&L, d_tlstart.as<uint32_t>() + nr, 4, hipMemcpyDeviceToHost, stream));
+        CK(hipStreamSynchronize(stream));
+        CK(d_tidx.ensure(4ull * L));
+        CK(d_tw.ensure(4ull * L));
+    }
+    k_compact<<<blocks_for(64ull * nr, 256), 256, 0, stream>>>(nr, d_tstart.as<uint32_t>(), d_tv.as<float>(),
+                                                               d_tci.as<uint32_t>(), d_tlstart.as<uint32_t>(),
+                                                               d_scale.as<float>() + r0, d_tidx.as<uint32_t>(),
+                                                               d_tw.as<float>());
+    CK(hipGetLastError());
     return 0;
 }
 

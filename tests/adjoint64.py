@@ -57,9 +57,41 @@ def adjoint(W, P, rho, D, irr, ubar, mats, g, nm):
     for c in range(3):
         r = np.asarray(rho[:, c], np.float64)
         mu[:, c] = spl.spsolve(I - MT @ sp.diags(r), MT @ np.asarray(ubar[:, c], np.float64))
+    return (mu,) + params(mu, rho, D, irr, mats, g, nm)
+
+
+def params(mu, rho, D, irr, mats, g, nm):
+    """dL/drho summed per material and dL/de from the adjoint state mu."""
     drho = mu * (D + irr)
     d_alb = np.zeros((nm, 3))
     np.add.at(d_alb, np.asarray(mats, np.int64), drho)
     g = np.asarray(g, np.float64)
     d_emit = g @ (mu * np.asarray(rho, np.float64))
-    return mu, d_alb, d_emit
+    return d_alb, d_emit
+
+
+def _fixed_point(step, n, tol, max_iter):
+    x = np.zeros((n, 3))
+    for _ in range(max_iter):
+        y = step(x)
+        if np.abs(y - x).max() <= tol * max(np.abs(y).max(), 1e-300):
+            return y
+        x = y
+    raise RuntimeError("float64 fixed point did not converge in %d iterations" % max_iter)
+
+
+def forward_iterative(W, P, rho, D, tol=1e-12, max_iter=20000):
+    """forward() by float64 Jacobi on the factors W and P (never forming
+    W P, which fills in at tens of thousands of surfels)."""
+    rho = np.asarray(rho, np.float64)
+    E = rho * D
+    return _fixed_point(lambda x: W @ (P @ (E + rho * x)), W.shape[0], tol, max_iter)
+
+
+def adjoint_iterative(W, P, rho, D, irr, ubar, mats, g, nm, tol=1e-12, max_iter=20000):
+    """adjoint() by float64 Jacobi on the factors: mu = P^T W^T (ubar + rho (.) mu)."""
+    rho = np.asarray(rho, np.float64)
+    ubar = np.asarray(ubar, np.float64)
+    WT, PT = W.T.tocsr(), P.T.tocsr()
+    mu = _fixed_point(lambda x: PT @ (WT @ (ubar + rho * x)), W.shape[0], tol, max_iter)
+    return (mu,) + params(mu, rho, D, irr, mats, g, nm)

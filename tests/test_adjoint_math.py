@@ -113,3 +113,30 @@ def test_planted_mistake_is_caught():
 def test_sparse_shapes():
     W, P, *_ = random_instance(5)
     assert sp.issparse(W) and W.shape == (60, 9) and P.shape == (9, 60)
+
+
+def test_iterative_solves_match_the_direct_ones():
+    """The factored float64 Jacobi forms used at sizes where W P cannot be
+    formed agree with the direct sparse solves."""
+    for seed in (6, 7):
+        W, P, mats, alb, g, emit = random_instance(seed, n=200, nc=17)
+        rho = alb[mats]
+        D = a64.direct_factor(g, emit)
+        irr = a64.forward(W, P, rho, D)
+        irr_it = a64.forward_iterative(W, P, rho, D, tol=1e-14)
+        assert np.allclose(irr_it, irr, rtol=1e-11, atol=1e-11 * np.abs(irr).max())
+        ubar = np.random.default_rng(seed).normal(size=(200, 3))
+        mu, da, de = a64.adjoint(W, P, rho, D, irr, ubar, mats, g, alb.shape[0])
+        mu2, da2, de2 = a64.adjoint_iterative(W, P, rho, D, irr, ubar, mats, g, alb.shape[0], tol=1e-14)
+        assert np.allclose(mu2, mu, rtol=1e-11, atol=1e-11 * np.abs(mu).max())
+        assert np.allclose(da2, da, rtol=1e-10, atol=1e-10 * np.abs(da).max())
+        assert np.allclose(de2, de, rtol=1e-10, atol=1e-10 * np.abs(de).max())
+    # transposing on the wrong side (W P instead of its transpose) is caught
+    W, P, mats, alb, g, emit = random_instance(8, n=200, nc=17)
+    rho = alb[mats]
+    ubar = np.random.default_rng(8).normal(size=(200, 3))
+    D = a64.direct_factor(g, emit)
+    irr = a64.forward(W, P, rho, D)
+    mu = a64.adjoint(W, P, rho, D, irr, ubar, mats, g, alb.shape[0])[0]
+    wrong = a64._fixed_point(lambda x: W @ (P @ (ubar + rho * x)), 200, 1e-14, 20000)
+    assert not np.allclose(wrong, mu, rtol=1e-3, atol=1e-3 * np.abs(mu).max())

@@ -108,6 +108,18 @@ def test_stage_order_is_enforced():
     assert e.value.status == -3
 
 
+def test_tuning_keys_and_ranges():
+    ctx = ia.Context(-1)
+    for key, v in (("link_budget", 1 << 40), ("link_budget", 0), ("adj_tile_rows", 512), ("adj_seg", 256),
+                   ("side_stream", 0), ("jacobi_rows", 3)):
+        ctx.set_tuning(key, v)
+    for key, v in (("link_budget", -1), ("link_budget", (1 << 50) + 1), ("adj_tile_rows", (1 << 30) + 1),
+                   ("spp", 0), ("no_such_key", 1)):
+        with pytest.raises(ia.IfxError) as e:
+            ctx.set_tuning(key, v)
+        assert e.value.status == -1
+
+
 # --------------------------------------------------------------------------- scene descriptor
 SCENE_FIXTURES = ["product.rad", "cornell_bench.rad", "stress.rad", "closed_box.rad", "alcove.rad", "dog.rad",
                   "minimal.rad", "TEMPLATE.rad"]
