@@ -260,13 +260,20 @@ ifx_status ifx_reset_kernel_times(ifx_ctx* ctx);
  *      tallies algorithmic FP32 operations (ifx_kernel_ops "name") and
  *      primitive evaluations ("name.prims");
  *   2: the timed kernels tally the primitive evaluations they execute, every
- *      lane of a shared traversal counted ("name.exec_prims");
+ *      lane of a shared traversal counted ("name.exec_prims"), and the
+ *      persistent ones (visibility, closure) the lane slots of their march
+ *      steps, 64 per wavefront step ("name.lane_slots"; the step count of the
+ *      last transfer over it is their lane utilisation);
  *   0: off. */
 ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
 /* Launch-configuration knobs (no counterpart in the reference; results are
  * identical for every setting, only speed changes):
  *   "lds"           1 = stage scene records in LDS when they fit (default), 0 = global
  *   "vis_blocks"    grid of the link-visibility kernel (0 = one lane per candidate)
+ *   "vis_chunk"     candidates a link-visibility wavefront takes per queue access (0 = automatic)
+ *   "vis_min_idle"  1..64: idle link-visibility lanes take new candidates once this many
+ *                   lanes of their wavefront are idle (default 64)
+ *   "closure_min_idle" 1..64: the same for the closure rays
  *   "ff_blocks"     grid of the candidate-cull kernels (0 = automatic)
  *   "jacobi_blocks" grid of the Jacobi and adjoint sweep kernels (0 = automatic)
  *   "jacobi_rows"   0 = automatic (sliced rows when rows average >= 64 links, else 1),

@@ -16,7 +16,8 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libinterflect_hip.so")
 ARCH = os.environ.get("IFX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["engine.hip", "adjoint.hip", "batch.hip", "surfels.hip", "host_scene.cpp", "host_surfels.cpp", "capi.cpp"]
+SOURCES = ["engine.hip", "shade_wave_lds.hip", "shade_wave_glb.hip", "shade_lane.hip", "shade_count.hip", "probe.hip",
+           "adjoint.hip", "batch.hip", "surfels.hip", "host_scene.cpp", "host_surfels.cpp", "capi.cpp"]
 
 # Bit-parity with the reference needs IEEE f32: no contraction to FMA,
 # correctly rounded division and sqrt, denormals kept.  Packed-FP32 VALU

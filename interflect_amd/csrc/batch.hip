@@ -244,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void kb_visibility(
         const float off = T.diag * 5e-4f, tmin = T.diag * 1e-3f;
         uint32_t acc = 0;
         bool live = false;
-        WaveQueue wq{0u, 0u, T.chunk_v, false};
+        WaveQueue wq{0u, 0u, T.chunk_v, false, kVisMinIdle};
         uint32_t slot = 0;
         float ff = 0.0f;
         ShadowMarch m;

@@ -625,6 +625,20 @@ ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value) {
             ctx->eng.invalidate_adjoint();
         }
         else if (k == "vis_blocks") ctx->eng.vis_blocks = (uint32_t)value;
+        else if (k == "vis_chunk") ctx->eng.vis_chunk = (uint32_t)value;
+        else if (k == "vis_min_idle") {
+            if (value < 1 || value > 64) {
+                ctx->err = "vis_min_idle must be in 1..64";
+                return IFX_E_INVALID;
+            }
+            ctx->eng.vis_min_idle = (uint32_t)value;
+        } else if (k == "closure_min_idle") {
+            if (value < 1 || value > 64) {
+                ctx->err = "closure_min_idle must be in 1..64";
+                return IFX_E_INVALID;
+            }
+            ctx->eng.closure_min_idle = (uint32_t)value;
+        }
         else if (k == "ff_blocks") ctx->eng.ff_blocks = (uint32_t)value;
         else if (k == "jacobi_blocks") ctx->eng.jacobi_blocks = (uint32_t)value;
         else if (k == "jacobi_rows") ctx->eng.jacobi_rows = (uint32_t)value;

@@ -12,6 +12,10 @@
 
 namespace ifx {
 
+// Idle link-visibility lanes of a wavefront take new candidates only when
+// this many are idle: 64 refills the whole wavefront at once.
+constexpr uint32_t kVisMinIdle = 64;
+
 struct LdsPlan {
     uint32_t scene_f4, light_f4;  // float4s of scene / light records staged in LDS (0 = global)
     uint32_t stack_depth;         // traversal-stack entries per lane
@@ -171,6 +175,9 @@ class Engine {
 
     bool use_lds = true;  // stage scene records in LDS when they fit
     uint32_t vis_blocks = 0, ff_blocks = 0, jacobi_blocks = 0;  // 0 = automatic
+    uint32_t vis_chunk = 0;     // link-visibility queue chunk per wavefront (0 = automatic)
+    uint32_t vis_min_idle = kVisMinIdle;  // link visibility: idle lanes refill once this many are idle
+    uint32_t closure_min_idle = 1;  // closure rays: the same for k_closure
     uint32_t jacobi_rows = 0;  // 0 automatic, 1 lane per row on the CSR, 2 wavefront per row, 3 sliced rows
     float solve_tol = 1e-4f;   // Jacobi stops once max |delta| < solve_tol (src/solve.rs:98)
     bool place_on_gpu = true;  // ifx_generate_surfels on the device when there is one

@@ -18,6 +18,7 @@
 #include <mutex>
 
 #include "ifx_kern.h"
+#include "ifx_launch.h"
 #include "ifx_sort.h"
 
 namespace ifx {
@@ -91,53 +92,6 @@ void PinBuf::release() {
     if (p) (void)hipHostFree(p);
     p = nullptr;
     cap = 0;
-}
-
-// ====================================================================== probes
-template <bool LDS>
-__global__ __launch_bounds__(kBlock) void k_probe_dist(SceneView Sg, LightsView Lg, LdsPlan P, const float* pts,
-                                                      uint32_t n, float* d, uint32_t* mat, float* nrm) {
-    IFX_SETUP(LDS, false, false);
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    f3 p = mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
-    DistHit h = dist_mat(S, p, stk);
-    d[i] = h.d;
-    mat[i] = h.mat;
-    if (nrm) {
-        f3 q = scene_normal(S, p, stk);
-        nrm[3 * i] = q.x, nrm[3 * i + 1] = q.y, nrm[3 * i + 2] = q.z;
-    }
-}
-
-template <bool LDS>
-__global__ __launch_bounds__(kBlock) void k_probe_trace(SceneView Sg, LightsView Lg, LdsPlan P, const float* ro,
-                                                       const float* rd, uint32_t n, float tmin, float tmax, float* t,
-                                                       float* nrm, uint32_t* mat, int32_t* hit, uint32_t* steps) {
-    IFX_SETUP(LDS, false, false);
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint32_t s = 0;
-    Hit h = trace<true>(S, mk(ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]), mk(rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]),
-                        tmin, tmax, s, stk);
-    t[i] = h.t;
-    nrm[3 * i] = h.n.x, nrm[3 * i + 1] = h.n.y, nrm[3 * i + 2] = h.n.z;
-    mat[i] = h.mat;
-    hit[i] = h.hit ? 1 : 0;
-    steps[i] = s;
-}
-
-template <bool LDS>
-__global__ __launch_bounds__(kBlock) void k_probe_shadow(SceneView Sg, LightsView Lg, LdsPlan P, const float* ro,
-                                                        const float* rd, uint32_t n, float tmin, const float* tmax,
-                                                        const float* k, float* vis, uint32_t* steps) {
-    IFX_SETUP(LDS, false, false);
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint32_t s = 0;
-    vis[i] = soft_shadow(S, mk(ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]), mk(rd[3 * i], rd[3 * i + 1], rd[3 * i + 2]),
-                         tmin, tmax[i], k[i], s, stk);
-    steps[i] = s;
 }
 
 // ====================================================================== transfer: clusters
@@ -231,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
                                                const float4* __restrict__ cnrm, uint32_t C, uint32_t* count,
                                                const uint32_t* __restrict__ start, uint32_t* cand_ci,
                                                uint32_t* cand_row, const uint32_t* __restrict__ morton,
-                                               uint32_t* key, uint32_t* val) {
+                                               uint32_t mbits, uint32_t* key, uint32_t* val) {
     uint32_t lane = lane_id();
     uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     uint32_t nwaves = (gridDim.x * kBlock) >> 6;
@@ -255,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
                 cand_ci[o] = c;
                 cand_row[o] = i;
                 if (key) {
-                    key[o] = (c << 15) | mc;
+                    key[o] = (c << mbits) | (mc >> (kMortonBits - mbits));
                     val[o] = o;
                 }
             }
@@ -270,9 +224,9 @@ __global__ void k_morton(const float4* __restrict__ spos, uint32_t N, f3 lo, f3 
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     f3 q = (ld3(spos[i]) - lo) * scale;
-    uint32_t x = (uint32_t)clampi(sat_i32(q.x), 0, 31), y = (uint32_t)clampi(sat_i32(q.y), 0, 31),
-             z = (uint32_t)clampi(sat_i32(q.z), 0, 31);
-    code[i] = spread5(x) | (spread5(y) << 1) | (spread5(z) << 2);
+    uint32_t x = (uint32_t)clampi(sat_i32(q.x), 0, 1023), y = (uint32_t)clampi(sat_i32(q.y), 0, 1023),
+             z = (uint32_t)clampi(sat_i32(q.z), 0, 1023);
+    code[i] = spread10(x) | (spread10(y) << 1) | (spread10(z) << 2);
     iota[i] = i;
 }
 
@@ -312,12 +266,13 @@ __global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView 
                                                        const uint32_t* __restrict__ ckey, uint32_t shift,
                                                        const uint32_t* __restrict__ crow, uint32_t M, float off,
                                                        float tmin, float* vis_out, uint32_t* queue, uint32_t chunk,
-                                                       unsigned long long* evals, unsigned long long* ops) {
+                                                       uint32_t min_idle, unsigned long long* evals,
+                                                       unsigned long long* ops) {
     IFX_SETUP(LDS, false, MODE);
     uint32_t acc = 0;
     bool live = false;
-    WaveQueue wq{0u, 0u, chunk, false};
-    uint32_t slot = 0;
+    WaveQueue wq{0u, 0u, chunk, false, min_idle};
+    uint32_t slot = 0, wsteps = 0;
     float ff = 0.0f;
     ShadowMarch m;
     m.begin(spl(0.0f), spl(0.0f), 0.0f, 0.0f, 0.0f);
@@ -336,6 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView 
             live = true;
         }
         if (__ballot(live) == 0) break;
+        if constexpr (MODE == 2) ++wsteps;
         float vis = 0.0f;
         if (shadow_step(S, m, live, vis, stk)) {
             vis_out[slot] = vis <= 1e-3f ? 0.0f : ff * vis;
@@ -345,6 +301,8 @@ __global__ __launch_bounds__(kBlock) void k_visibility(SceneView Sg, LightsView 
     }
     count_steps(evals, acc);
     stk_flush(stk, ops);
+    if constexpr (MODE == 2)
+        if (lane_id() == 0 && wsteps) atomicAdd(ops + 12, 64ull * wsteps);  // lane slots of the marches
 }
 
 // Candidates up to which the link visibility may share the GPU with the
@@ -377,12 +335,12 @@ __global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg,
                                                     const float4* __restrict__ mnrm, uint32_t N,
                                                     const uint32_t* __restrict__ row_order,
                                                     const float4* __restrict__ dirs, float tmax, uint32_t* open,
-                                                    uint32_t* queue, uint32_t chunk, unsigned long long* evals,
-                                                    unsigned long long* ops) {
+                                                    uint32_t* queue, uint32_t chunk, uint32_t min_idle,
+                                                    unsigned long long* evals, unsigned long long* ops) {
     IFX_SETUP(LDS, false, MODE);
-    uint32_t acc = 0;
+    uint32_t acc = 0, wsteps = 0;
     bool live = false;
-    WaveQueue wq{0u, 0u, chunk, false};
+    WaveQueue wq{0u, 0u, chunk, false, min_idle};
     uint32_t row = 0;
     TraceMarch m;
     m.begin(spl(0.0f), spl(0.0f), 0.0f, 0.0f);
@@ -401,6 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg,
             live = true;
         }
         if (__ballot(live) == 0) break;
+        if constexpr (MODE == 2) ++wsteps;
         const int st = miss_step(S, m, live, stk);
         if (st) {
             if (st == 2) atomicAdd(&open[row], 1u);
@@ -410,6 +369,8 @@ __global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg,
     }
     count_steps(evals, acc);
     stk_flush(stk, ops);
+    if constexpr (MODE == 2)
+        if (lane_id() == 0 && wsteps) atomicAdd(ops + 12, 64ull * wsteps);
 }
 
 // Serial per-row link sum in cluster order plus the closure calibration
@@ -714,44 +675,6 @@ __global__ void k_grid_keys(const float4* __restrict__ spos, uint32_t N, f3 lo, 
     atomicAdd(&counts[c], 1u);
 }
 
-// 16x16 pixel blocks, one 8x8 tile per wavefront.
-template <bool COUNT, bool LDS, bool WAVE>
-__global__ __launch_bounds__(kBlock) void k_shade(SceneView Sg, LightsView Lg, LdsPlan P, ShadeArgs A, float4* film,
-                                                  unsigned long long* evals, unsigned long long* ops) {
-    IFX_SETUP_W(LDS, true, COUNT, WAVE);
-    uint32_t wv = threadIdx.x >> 6, l = threadIdx.x & 63u;
-    uint32_t x = blockIdx.x * 16u + (wv & 1u) * 8u + (l & 7u);
-    uint32_t y = blockIdx.y * 16u + (wv >> 1) * 8u + (l >> 3);
-    uint32_t local = 0;
-    if (x < A.W && y < A.H) {
-        f3 c = shade_pixel(S, L, A, x, y, local, stk);
-        film[(size_t)y * A.W + x] = st4(c);
-    }
-    count_steps(evals, local);
-    stk_flush(stk, ops);
-}
-
-// Many cameras over one solve (the turntable, src/main.rs:492-520): grid z
-// is the view; each view is k_shade with its own camera.
-template <bool COUNT, bool LDS, bool WAVE>
-__global__ __launch_bounds__(kBlock) void k_shade_views(SceneView Sg, LightsView Lg, LdsPlan P, ShadeArgs A0,
-                                                        const CamV* __restrict__ cams, float4* film,
-                                                        unsigned long long* evals, unsigned long long* ops) {
-    IFX_SETUP_W(LDS, true, COUNT, WAVE);
-    ShadeArgs A = A0;
-    A.cam = cams[blockIdx.z];
-    uint32_t wv = threadIdx.x >> 6, l = threadIdx.x & 63u;
-    uint32_t x = blockIdx.x * 16u + (wv & 1u) * 8u + (l & 7u);
-    uint32_t y = blockIdx.y * 16u + (wv >> 1) * 8u + (l >> 3);
-    uint32_t local = 0;
-    if (x < A.W && y < A.H) {
-        f3 c = shade_pixel(S, L, A, x, y, local, stk);
-        film[(size_t)blockIdx.z * A.W * A.H + (size_t)y * A.W + x] = st4(c);
-    }
-    count_steps(evals, local);
-    stk_flush(stk, ops);
-}
-
 __global__ void k_encode(const float4* __restrict__ film, uint32_t npx, float exposure, int32_t tonemap,
                          const float* __restrict__ thr, uint8_t* out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -956,6 +879,7 @@ void Engine::tally_ops(const char* name, const unsigned long long* cv, int slot)
         kops_[k + ".prims"] += cv[slot + 4];
     } else if (count_exec_) {
         kops_[k + ".exec_prims"] += cv[slot + 8];
+        kops_[k + ".lane_slots"] += cv[slot + 12];
     }
 }
 
@@ -1143,13 +1067,13 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     CK(d_rorder.ensure(4ull * N));
     {
         const f3 ext = scene.hi - scene.lo;
-        const f3 sc = mk(32.0f / fmx(ext.x, 1e-6f), 32.0f / fmx(ext.y, 1e-6f), 32.0f / fmx(ext.z, 1e-6f));
+        const f3 sc = mk(1024.0f / fmx(ext.x, 1e-6f), 1024.0f / fmx(ext.y, 1e-6f), 1024.0f / fmx(ext.z, 1e-6f));
         launch("morton", [&] {
             k_morton<<<blocks_for(N, 256), 256, 0, stream>>>(spos, N, scene.lo, sc, d_morton.as<uint32_t>(),
                                                              d_mval.as<uint32_t>());
         });
         if (int rc_s = sort_u32(d_morton.as<uint32_t>(), d_mval.as<uint32_t>(), d_mkey_s.as<uint32_t>(),
-                           d_rorder.as<uint32_t>(), N, 15u, err))
+                           d_rorder.as<uint32_t>(), N, kMortonBits, err))
             return rc_s;
         CK(d_mpos.ensure(16ull * N));
         CK(d_mnrm.ensure(16ull * N));
@@ -1202,7 +1126,8 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
             LAUNCH_CM(k_closure, tally_mode(), ld.staged, grid, ld.bytes, S, Lg, ld.plan, d_mpos.as<float4>(),
                       d_mnrm.as<float4>(), N,
                       d_rorder.as<uint32_t>(), d_dirs.as<float4>(), diag * 2.0f, d_open.as<uint32_t>(), queue + 1,
-                      wave_chunk(16ull * N, grid), ctr, tally() ? ctr + 2 : nullptr);
+                      wave_chunk(16ull * N, grid), std::min(64u, std::max(1u, closure_min_idle)), ctr,
+                      tally() ? ctr + 2 : nullptr);
         });
     }
     if (side) {
@@ -1283,8 +1208,8 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     const uint32_t ffb = ff_blocks ? ff_blocks : std::min<uint32_t>(blocks_for(N, 4), 8192);
     launch("ff_count", [&] {
         k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, 0u, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
-                                                 d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                 nullptr);
+                                                 d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, 0u,
+                                                 nullptr, nullptr);
     });
     if (int rc_s = scan_u32(d_ccount.as<uint32_t>(), d_cstart.as<uint32_t>(), (N + 1), err))
         return rc_s;
@@ -1296,9 +1221,12 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         err = "candidate count exceeds 2^31";
         return -6;
     }
-    // Cluster-major, Morton-minor visiting order of the candidates.
-    const uint32_t vkey_bits = bits_for(C ? C - 1 : 0) + 15u;
-    const bool sort_cand = vkey_bits <= 32u && M > 0;
+    // Cluster-major, Morton-minor visiting order of the candidates: the
+    // cluster above as many Morton bits of the surfel's cell as fit in 32.
+    const uint32_t cbits = bits_for(C ? C - 1 : 0);
+    const uint32_t mbits = cbits < 32u ? std::min(kMortonBits, 32u - cbits) : 0u;
+    const uint32_t vkey_bits = cbits + mbits;
+    const bool sort_cand = cbits <= 17u && M > 0;
     CK(d_cand_ci.ensure(4ull * M));
     CK(d_cand_row.ensure(4ull * M));
     CK(d_cand_v.ensure(4ull * M));
@@ -1312,7 +1240,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, 0u, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
                                                 nullptr,
                                                 d_cstart.as<uint32_t>(), d_cand_ci.as<uint32_t>(),
-                                                d_cand_row.as<uint32_t>(), d_morton.as<uint32_t>(),
+                                                d_cand_row.as<uint32_t>(), d_morton.as<uint32_t>(), mbits,
                                                 sort_cand ? d_vkey.as<uint32_t>() : nullptr,
                                                 sort_cand ? d_vval.as<uint32_t>() : nullptr);
     });
@@ -1343,9 +1271,10 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
             LAUNCH_CM(k_visibility, tally_mode(), ld.staged, visb, ld.bytes, S, Lg, ld.plan,
                       sort_cand ? d_mpos.as<float4>() : spos, sort_cand ? d_mnrm.as<float4>() : snrm,
                       d_cpos.as<float4>(), d_cnrm.as<float4>(),
-                      sort_cand ? d_vkey_s.as<uint32_t>() : d_cand_ci.as<uint32_t>(), sort_cand ? 15u : 0u,
+                      sort_cand ? d_vkey_s.as<uint32_t>() : d_cand_ci.as<uint32_t>(), sort_cand ? mbits : 0u,
                       sort_cand ? vrow : d_cand_row.as<uint32_t>(), M, diag * 5e-4f, diag * 1e-3f, vis, queue,
-                      wave_chunk(M, visb), ctr, tally() ? ctr + 1 : nullptr);
+                      vis_chunk ? vis_chunk : wave_chunk(M, visb), std::min(64u, std::max(1u, vis_min_idle)), ctr,
+                      tally() ? ctr + 1 : nullptr);
         });
         if (sort_cand) {
             launch("vis_scatter", [&] {
@@ -1680,7 +1609,7 @@ int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool 
     const uint32_t ffb = ff_blocks ? ff_blocks : std::min<uint32_t>(blocks_for(nr, 4), 8192);
     CK(hipMemsetAsync(d_tcount.as<uint32_t>() + nr, 0, 4, stream));
     k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, r0, nr, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
-                                             d_tcount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                                             d_tcount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, 0u, nullptr,
                                              nullptr);
     if (int rc = scan_u32(d_tcount.as<uint32_t>(), d_tstart.as<uint32_t>(), nr + 1, err)) return rc;
     if (!sized) {
@@ -1692,7 +1621,7 @@ int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool 
     }
     k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, r0, nr, d_cpos.as<float4>(), d_cnrm.as<float4>(), C, nullptr,
                                             d_tstart.as<uint32_t>(), d_tci.as<uint32_t>(), d_trow.as<uint32_t>(),
-                                            nullptr, nullptr, nullptr);
+                                            nullptr, 0u, nullptr, nullptr);
     if (M) {
         const uint32_t persist = (uint32_t)n_cu_ * 8u;
         const uint32_t visb = vis_blocks ? vis_blocks : std::max(1u, std::min<uint32_t>(persist, blocks_for(M, kBlock)));
@@ -1703,7 +1632,8 @@ int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool 
         CK(hipMemsetAsync(d_tqueue.p, 0, 4, stream));
         LAUNCH_CM(k_visibility, 0, ld.staged, visb, ld.bytes, S, Lg, ld.plan, spos, snrm, d_cpos.as<float4>(),
                   d_cnrm.as<float4>(), d_tci.as<uint32_t>(), 0u, d_trow.as<uint32_t>(), M, diag * 5e-4f,
-                  diag * 1e-3f, d_tv.as<float>(), d_tqueue.as<uint32_t>(), wave_chunk(M, visb), ctr + 20, nullptr);
+                  diag * 1e-3f, d_tv.as<float>(), d_tqueue.as<uint32_t>(), wave_chunk(M, visb),
+                  std::min(64u, std::max(1u, vis_min_idle)), ctr + 20, nullptr);
     }
     CK(hipMemsetAsync(d_tkeep.as<uint32_t>() + nr, 0, 4, stream));
     k_keep_count<<<blocks_for(nr, 256), 256, 0, stream>>>(nr, d_tstart.as<uint32_t>(), d_tv.as<float>(),
@@ -1829,6 +1759,17 @@ int Engine::shade_setup(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode
     return 0;
 }
 
+// One shading launch: counting runs keep the per-lane traversal; timed runs
+// share each query across the wavefront, except for camera rays of a scene
+// small enough for the uniform scan when one launch holds kWaveShadeMaxPixels
+// or more pixels.
+static void launch_shade_group(const ShadeLaunch& a, uint64_t npx, bool count) {
+    if (count) launch_shade_count(a);
+    else if (a.S.n_obj <= kWaveScanObjs && npx >= kWaveShadeMaxPixels) launch_shade_lane(a);
+    else if (a.staged) launch_shade_wave_lds(a);
+    else launch_shade_wave_glb(a);
+}
+
 int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, float* film, uint8_t* rgb8,
                   uint64_t& evals, std::string& err) {
     ShadeArgs A;
@@ -1844,9 +1785,9 @@ int Engine::shade(const HCamera& cam, uint32_t W, uint32_t H, int32_t mode, floa
     dim3 grid((W + 15) / 16, (H + 15) / 16);
     const Lds ld = plan_lds(true);
     launch("shade", [&] {
-        const bool wave = S.n_obj > kWaveScanObjs || npx < kWaveShadeMaxPixels;
-        LAUNCH_CLW(k_shade, count_ops_, ld.staged, wave, grid, ld.bytes, S, Lv, ld.plan, A, d_film.as<float4>(), ctr,
-                   tally() ? ctr + 4 : nullptr);
+        launch_shade_group(ShadeLaunch{S, Lv, ld.plan, A, nullptr, d_film.as<float4>(), ctr,
+                                       tally() ? ctr + 4 : nullptr, grid, ld.bytes, ld.staged, stream},
+                           npx, count_ops_);
     });
     const bool tonemapped = mode_tonemapped(mode);
     if (rgb8) {
@@ -1912,9 +1853,9 @@ int Engine::shade_views(const std::vector<HCamera>& cams, uint32_t W, uint32_t H
     dim3 grid((W + 15) / 16, (H + 15) / 16, nv);
     const Lds ld = plan_lds(true);
     launch("shade_views", [&] {
-        const bool wave = S.n_obj > kWaveScanObjs || npx < kWaveShadeMaxPixels;
-        LAUNCH_CLW(k_shade_views, count_ops_, ld.staged, wave, grid, ld.bytes, S, Lv, ld.plan, A, d_cams.as<CamV>(),
-                   d_film.as<float4>(), ctr, tally() ? ctr + 4 : nullptr);
+        launch_shade_group(ShadeLaunch{S, Lv, ld.plan, A, d_cams.as<CamV>(), d_film.as<float4>(), ctr,
+                                       tally() ? ctr + 4 : nullptr, grid, ld.bytes, ld.staged, stream},
+                           npx, count_ops_);
     });
     if (rgb8) {
         launch("encode", [&] {
@@ -1962,10 +1903,9 @@ int Engine::probe_dist(const float* pts, uint64_t n, float* d, uint32_t* mat, fl
     float* on = nrm ? od + 2 * n : nullptr;
     const SceneView S = dev_view();
     const Lds ld = plan_lds(false);
-    const LightsView Lg{};
     launch("probe_dist", [&] {
-        LAUNCH_L(k_probe_dist, ld.staged, blocks_for(n, kBlock), ld.bytes, S, Lg, ld.plan, d_probe_in.as<float>(),
-                 (uint32_t)n, od, om, on);
+        launch_probe_dist(ProbeLaunch{S, ld.plan, blocks_for(n, kBlock), ld.bytes, ld.staged, stream},
+                          d_probe_in.as<float>(), (uint32_t)n, od, om, on);
     });
     CK(hipMemcpyAsync(d, od, 4ull * n, hipMemcpyDeviceToHost, stream));
     CK(hipMemcpyAsync(mat, om, 4ull * n, hipMemcpyDeviceToHost, stream));
@@ -1993,10 +1933,9 @@ int Engine::probe_trace(const float* ro, const float* rd, uint64_t n, float tmin
     uint32_t* os = (uint32_t*)(o + 6 * n);
     const SceneView S = dev_view();
     const Lds ld = plan_lds(false);
-    const LightsView Lg{};
     launch("probe_trace", [&] {
-        LAUNCH_L(k_probe_trace, ld.staged, blocks_for(n, kBlock), ld.bytes, S, Lg, ld.plan, in, in + 3 * n, (uint32_t)n,
-                 tmin, tmax, ot, on, om, oh, os);
+        launch_probe_trace(ProbeLaunch{S, ld.plan, blocks_for(n, kBlock), ld.bytes, ld.staged, stream}, in,
+                           in + 3 * n, (uint32_t)n, tmin, tmax, ot, on, om, oh, os);
     });
     CK(hipMemcpyAsync(t, ot, 4ull * n, hipMemcpyDeviceToHost, stream));
     CK(hipMemcpyAsync(nrm, on, 12ull * n, hipMemcpyDeviceToHost, stream));
@@ -2024,10 +1963,9 @@ int Engine::probe_shadow(const float* ro, const float* rd, uint64_t n, float tmi
     uint32_t* os = (uint32_t*)(ov + n);
     const SceneView S = dev_view();
     const Lds ld = plan_lds(false);
-    const LightsView Lg{};
     launch("probe_shadow", [&] {
-        LAUNCH_L(k_probe_shadow, ld.staged, blocks_for(n, kBlock), ld.bytes, S, Lg, ld.plan, in, in + 3 * n,
-                 (uint32_t)n, tmin, in + 6 * n, in + 7 * n, ov, os);
+        launch_probe_shadow(ProbeLaunch{S, ld.plan, blocks_for(n, kBlock), ld.bytes, ld.staged, stream}, in,
+                            in + 3 * n, (uint32_t)n, tmin, in + 6 * n, in + 7 * n, ov, os);
     });
     CK(hipMemcpyAsync(vis, ov, 4ull * n, hipMemcpyDeviceToHost, stream));
     CK(hipMemcpyAsync(steps, os, 4ull * n, hipMemcpyDeviceToHost, stream));

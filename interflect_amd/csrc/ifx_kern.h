@@ -511,17 +511,32 @@ __device__ __forceinline__ uint32_t spread5(uint32_t v) {
     return r;
 }
 
+// 30-bit Morton code (10 bits per axis); its high bits are the coarser codes.
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+    uint32_t r = 0;
+    for (int b = 0; b < 10; ++b) r |= ((v >> b) & 1u) << (3 * b);
+    return r;
+}
+constexpr uint32_t kMortonBits = 30;
+
 // Wavefront-private slice of a global work queue.  A wave takes kChunk
 // consecutive items per atomic and hands them to its idle lanes in order;
 // all fields are wave-uniform.
+// Idle lanes wait until at least min_idle (1..64) of them are idle, so the
+// lanes refilled together start their rays together.  The link visibility
+// refills a whole wavefront at once (kVisMinIdle): 64 consecutive candidates
+// (one cluster, Morton-adjacent surfels) march side by side, so the shared
+// query's union of their nodes stays near one lane's (stress scene: 311 ->
+// 222 ms, although 27 % of the lane slots then idle at the batch tails).
 struct WaveQueue {
     uint32_t next, end, chunk;
     bool drained;
+    uint32_t min_idle = 1;
 };
 
 __device__ __forceinline__ bool refill(bool live, WaveQueue& wq, uint32_t* queue, uint32_t total, uint32_t& item) {
     uint64_t idle = __ballot(!live);
-    if (!idle) return false;
+    if (!idle || (uint32_t)__popcll(idle) < wq.min_idle) return false;
     if (wq.next >= wq.end) {
         if (wq.drained) return false;
         uint32_t base = 0;
@@ -713,28 +728,6 @@ void Engine::launch_on(hipStream_t stream, const char* name, F&& f) {
                 K<false, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);              \
             else                                                                         \
                 K<false, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);             \
-        }                                                                                \
-    } while (0)
-
-// As LAUNCH_CL with the WAVE choice of the shading kernels (counting runs
-// keep the per-lane traversal).
-#define LAUNCH_CLW(K, COUNT, LDS, WAVE, GRID, DYN, ...)                                   \
-    do {                                                                                 \
-        if (COUNT) {                                                                     \
-            if (LDS)                                                                     \
-                K<true, true, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);        \
-            else                                                                         \
-                K<true, false, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);       \
-        } else if (WAVE) {                                                               \
-            if (LDS)                                                                     \
-                K<false, true, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);        \
-            else                                                                         \
-                K<false, false, true><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);       \
-        } else {                                                                         \
-            if (LDS)                                                                     \
-                K<false, true, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);       \
-            else                                                                         \
-                K<false, false, false><<<GRID, kBlock, DYN, stream>>>(__VA_ARGS__);      \
         }                                                                                \
     } while (0)
 
