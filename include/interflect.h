@@ -274,6 +274,8 @@ ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
  *   "vis_min_idle"  1..64: idle link-visibility lanes take new candidates once this many
  *                   lanes of their wavefront are idle (default 64)
  *   "closure_min_idle" 1..64: the same for the closure rays
+ *   "flat_query"    1 = a wavefront's scene queries along marches test every object's box
+ *                   against the wavefront's points (default), 0 = they walk the BVH
  *   "ff_blocks"     grid of the candidate-cull kernels (0 = automatic)
  *   "jacobi_blocks" grid of the Jacobi and adjoint sweep kernels (0 = automatic)
  *   "jacobi_rows"   0 = automatic (sliced rows when rows average >= 64 links, else 1),

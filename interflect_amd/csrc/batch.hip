@@ -686,6 +686,7 @@ int Batch::build_slots(std::string& err) {
         B.S.lin = (const uint32_t*)(rec + r);
         r += (sc.lin.size() + 3) / 4;
         B.S.keep_global();
+        B.S.g_obox = nullptr;  // the batch's shared queries walk the BVH
         B.L.quad = rec + r;
         r += sc.quad_rec.size();
         B.L.sph = rec + r;

@@ -644,6 +644,7 @@ ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value) {
         else if (k == "jacobi_rows") ctx->eng.jacobi_rows = (uint32_t)value;
         else if (k == "surfels_gpu") ctx->eng.place_on_gpu = value != 0;
         else if (k == "side_stream") ctx->eng.overlap_direct = value != 0;
+        else if (k == "flat_query") ctx->eng.flat_query = value != 0;
         else if (k == "spp") {
             if (value < 1) {
                 ctx->err = "spp must be >= 1";

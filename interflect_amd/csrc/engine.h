@@ -178,6 +178,9 @@ class Engine {
     uint32_t vis_chunk = 0;     // link-visibility queue chunk per wavefront (0 = automatic)
     uint32_t vis_min_idle = kVisMinIdle;  // link visibility: idle lanes refill once this many are idle
     uint32_t closure_min_idle = 1;  // closure rays: the same for k_closure
+    // Wavefront-shared queries of marches with a distance bound test each
+    // object's box against the wavefront's points instead of walking the BVH.
+    bool flat_query = true;
     uint32_t jacobi_rows = 0;  // 0 automatic, 1 lane per row on the CSR, 2 wavefront per row, 3 sliced rows
     float solve_tol = 1e-4f;   // Jacobi stops once max |delta| < solve_tol (src/solve.rs:98)
     bool place_on_gpu = true;  // ifx_generate_surfels on the device when there is one
@@ -280,7 +283,7 @@ class Engine {
     std::vector<hipEvent_t> event_pool_;
 
     // scene
-    DevBuf d_obj, d_planes, d_subs, d_nodes, d_lin, d_quad, d_sph, d_pnt, d_mat, d_dirs, d_srgb;
+    DevBuf d_obj, d_planes, d_subs, d_nodes, d_obox, d_lin, d_quad, d_sph, d_pnt, d_mat, d_dirs, d_srgb;
     // surfels (float4: pos.xyz+area, nrm, albedo)
     DevBuf d_spos, d_snrm, d_salb;
     bool surfels_dirty = true;

@@ -898,6 +898,7 @@ SceneView Engine::dev_view() const {
     v.lin = d_lin.as<uint32_t>();
     v.use_bvh = use_bvh ? 1u : 0u;
     v.keep_global();
+    v.g_obox = flat_query && !scene.obox_rec.empty() ? d_obox.as<float4>() : nullptr;
     return v;
 }
 
@@ -983,6 +984,7 @@ int Engine::install_scene(HostScene&& s, std::string& err) {
     if ((rc = up(d_planes, s.plane_rec.data(), s.plane_rec.size() * 16))) return rc;
     if ((rc = up(d_subs, s.sub_rec.data(), s.sub_rec.size() * 16))) return rc;
     if ((rc = up(d_nodes, s.node_rec.data(), s.node_rec.size() * 16))) return rc;
+    if ((rc = up(d_obox, s.obox_rec.data(), s.obox_rec.size() * 16))) return rc;
     {
         std::vector<uint32_t> lin_pad((s.lin.size() + 3) & ~(size_t)3, 0u);
         std::copy(s.lin.begin(), s.lin.end(), lin_pad.begin());

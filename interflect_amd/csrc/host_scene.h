@@ -74,6 +74,7 @@ struct HostScene {
     std::vector<float4> plane_rec;     // 4 per plane
     std::vector<float4> sub_rec;       // 4 per subtraction primitive
     std::vector<float4> node_rec;      // 2 per BVH node
+    std::vector<float4> obox_rec;      // 2 per object (lo, hi of its BVH box), BVH leaf order
     std::vector<uint32_t> order;       // BVH leaf slot -> object index
     std::vector<uint32_t> lin;         // object index -> leaf slot, then slot -> BVH leaf node
     std::vector<float4> quad_rec, sph_rec, pnt_rec, mat_rec;

@@ -22,6 +22,16 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
+__device__ __forceinline__ float wave_minf(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_maxf(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 // Adds a per-lane step count to a 64-bit counter, one atomic per wave.
 __device__ __forceinline__ void count_steps(unsigned long long* ctr, uint32_t steps) {
     uint32_t t = wave_sum(steps);
@@ -198,6 +208,13 @@ __device__ __forceinline__ void place_stack(Stk& stk, float4* lds, uint32_t f4, 
 //    traversal, so every lane returns dist() bit for bit.
 //  Scenes of at most kWaveScanObjs objects skip the interior nodes: their
 //  leaves are tested in slot order, each under the same opening test.
+//  Flat list (S.g_obox set, all 64 lanes active, every lane that needs a
+//  value has a bound ub): instead of walking the BVH, each lane tests the boxes of objects lane,
+//  lane + 64, ... against the box of the wavefront's points, grown by the
+//  largest opening bound; the objects that pass are taken in slot order, each
+//  after its own box passes some lane's opening test.  An object left out is
+//  then beyond every lane's bound, as under an unopened node, so the same
+//  checks apply (a leaf's objects keep their slot order).
 //  Records are read from their global copies (S.g_*) through
 //  constant-address-space pointers: with the node and object indices uniform,
 //  those reads are scalar loads.  The per-lane fallback reads S as given
@@ -295,7 +312,37 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
             h2 = fmn(h2, d);
         }
     };
-    if (S.n_obj <= kWaveScanObjs) {
+    // The flat list needs the whole wavefront: lane l tests objects l, l + 64, ...
+    // and the reductions read every lane.
+    const bool flat = S.g_obox != nullptr && S.n_obj > kWaveScanObjs && __ballot(true) == ~0ull &&
+                      __ballot(need && !(ub < kFltMax)) == 0ull;
+    if (flat) {
+        const ConstF4 obox = as_const(S.g_obox);
+        f3 wlo = need ? p : spl(kFltMax), whi = need ? p : spl(-kFltMax);
+        float wb = need ? blin : 0.0f;
+        wlo = mk(wave_minf(wlo.x), wave_minf(wlo.y), wave_minf(wlo.z));
+        whi = mk(wave_maxf(whi.x), wave_maxf(whi.y), wave_maxf(whi.z));
+        wb = wave_maxf(wb);
+        const float wb2 = wb * wb;
+        const uint32_t lane = lane_id();
+        for (uint32_t g = 0; g < S.n_obj; g += 64u) {
+            bool pass = false;
+            if (g + lane < S.n_obj) {
+                const float4 lo = S.g_obox[2 * (g + lane)], hi = S.g_obox[2 * (g + lane) + 1];
+                const f3 e = max3(max3(ld3(lo) - whi, wlo - ld3(hi)), spl(0.0f));
+                pass = dot3(e, e) < wb2;
+            }
+            uint64_t m = __ballot(pass);
+            while (m) {
+                const uint32_t k = g + (uint32_t)__builtin_ctzll(m);
+                m &= m - 1ull;
+                const f3 e = max3(max3(ld3(obox[2 * k]) - p, p - ld3(obox[2 * k + 1])), spl(0.0f));
+                if (__ballot(need && dot3(e, e) < open2) == 0ull) continue;
+                take(k, __builtin_amdgcn_readfirstlane(S.lin[S.n_obj + k]));
+                refresh();
+            }
+        }
+    } else if (S.n_obj <= kWaveScanObjs) {
         // few objects: the leaves in slot order (no interior nodes), each
         // opened under the same test
         for (uint32_t k = 0; k < S.n_obj;) {

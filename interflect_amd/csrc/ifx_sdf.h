@@ -60,6 +60,10 @@ struct SceneView {
     // The same records in global memory (the pointers above may be moved to
     // an LDS copy); the wavefront-shared traversal reads these.
     const float4 *g_obj, *g_planes, *g_subs, *g_nodes;
+    // Each object's box (lo, hi; 2 x float4 per slot) in global memory, for
+    // the flat candidate list of the wavefront-shared query; null: that
+    // query always walks the BVH.
+    const float4* g_obox;
     IFX_HD void keep_global() {
         g_obj = obj;
         g_planes = planes;
