@@ -273,7 +273,7 @@ ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
  *   "vis_chunk"     candidates a link-visibility wavefront takes per queue access (0 = automatic)
  *   "vis_min_idle"  1..64: idle link-visibility lanes take new candidates once this many
  *                   lanes of their wavefront are idle (default 64)
- *   "closure_min_idle" 1..64: the same for the closure rays
+ *   "closure_min_idle" 1..64: the same for the closure rays (default 64)
  *   "flat_query"    1 = a wavefront's scene queries along marches test every object's box
  *                   against the wavefront's points (default), 0 = they walk the BVH
  *   "ff_blocks"     grid of the candidate-cull kernels (0 = automatic)

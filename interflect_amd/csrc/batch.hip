@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void kb_closure(const BSlot* __restrict__ s
         const float tmax = T.diag * 2.0f;
         uint32_t acc = 0;
         bool live = false;
-        WaveQueue wq{0u, 0u, T.chunk_c, false};
+        WaveQueue wq{0u, 0u, T.chunk_c, false, kMarchMinIdle};
         uint32_t row = 0;
         TraceMarch m;
         m.begin(spl(0.0f), spl(0.0f), 0.0f, 0.0f);
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void kb_visibility(
         const float off = T.diag * 5e-4f, tmin = T.diag * 1e-3f;
         uint32_t acc = 0;
         bool live = false;
-        WaveQueue wq{0u, 0u, T.chunk_v, false, kVisMinIdle};
+        WaveQueue wq{0u, 0u, T.chunk_v, false, kMarchMinIdle};
         uint32_t slot = 0;
         float ff = 0.0f;
         ShadowMarch m;
@@ -286,15 +286,17 @@ __global__ __launch_bounds__(kBlock) void kb_direct_terms(const BSlot* __restric
     IFX_BSETUP(true);
     const uint32_t N = T.N, nl = T.nl;
     const uint32_t t = bid * kBlock + threadIdx.x;
+    const bool in = t < N * nl;  // every lane calls diffuse_term (tail lanes follow along)
     uint32_t steps = 0;
-    if (t < N * nl) {
-        uint32_t li = t / N, i = rorder[T.s0 + (t - li * N)];
-        f3 p = ld3(spos[i]), n = ld3(snrm[i]);
-        f3 tan, bit;
-        frame(n, tan, bit);
-        f3 term = spl(0.0f);
-        float g = 0.0f;
-        bool on = diffuse_term(S, L, li, p, n, tan, bit, ld3(salb[i]), term, g, steps, stk);
+    const uint32_t tc = in ? t : 0u;
+    const uint32_t li = tc / N, i = rorder[T.s0 + (tc - li * N)];
+    f3 p = ld3(spos[i]), n = ld3(snrm[i]);
+    f3 tan, bit;
+    frame(n, tan, bit);
+    f3 term = spl(0.0f);
+    float g = 0.0f;
+    const bool on = diffuse_term(S, L, li, p, n, tan, bit, ld3(salb[i]), term, g, steps, stk, in);
+    if (in) {
         const size_t o = (size_t)T.term0 + (size_t)li * N + (i - T.s0);
         terms[o] = on ? st4(term, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         gterm[o] = on ? g : 0.0f;
@@ -597,7 +599,7 @@ int Batch::upload_static(std::string& err) {
         cells += dx * dy * dz;
         rec += sc.obj_rec.size() + sc.plane_rec.size() + sc.sub_rec.size() + sc.node_rec.size() +
                (sc.lin.size() + 3) / 4 + sc.quad_rec.size() + sc.sph_rec.size() + sc.pnt_rec.size() +
-               sc.mat_rec.size();
+               sc.mat_rec.size() + sc.obox_rec.size();
         Bmax_ = std::max(Bmax_, sc.cfg.bounces);
     }
     if (N >= (1ull << 31) || P >= (1ull << 31) || T >= (1ull << 32) || cells >= (1ull << 31)) {
@@ -640,6 +642,7 @@ int Batch::upload_static(std::string& err) {
         put(sc.sph_rec);
         put(sc.pnt_rec);
         put(sc.mat_rec);
+        put(sc.obox_rec);
     }
     if (!hrec.empty()) BK(hipMemcpy(d_rec.p, hrec.data(), 16ull * hrec.size(), hipMemcpyHostToDevice));
     BK(d_spos.ensure(16ull * N));
@@ -686,7 +689,6 @@ int Batch::build_slots(std::string& err) {
         B.S.lin = (const uint32_t*)(rec + r);
         r += (sc.lin.size() + 3) / 4;
         B.S.keep_global();
-        B.S.g_obox = nullptr;  // the batch's shared queries walk the BVH
         B.L.quad = rec + r;
         r += sc.quad_rec.size();
         B.L.sph = rec + r;
@@ -695,6 +697,8 @@ int Batch::build_slots(std::string& err) {
         r += sc.pnt_rec.size();
         B.L.mat = rec + r;
         r += sc.mat_rec.size();
+        B.S.g_obox = sc.obox_rec.empty() ? nullptr : rec + r;
+        r += sc.obox_rec.size();
         B.S.use_bvh = 1;
         // LDS plans (as Engine::plan_lds)
         const uint32_t n_obj = (uint32_t)sc.objects.size();

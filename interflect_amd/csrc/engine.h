@@ -12,9 +12,10 @@
 
 namespace ifx {
 
-// Idle link-visibility lanes of a wavefront take new candidates only when
-// this many are idle: 64 refills the whole wavefront at once.
-constexpr uint32_t kVisMinIdle = 64;
+// Idle lanes of a persistent march kernel (link visibility, closure rays)
+// take new work only when this many lanes of their wavefront are idle: 64
+// refills the whole wavefront at once.
+constexpr uint32_t kMarchMinIdle = 64;
 
 struct LdsPlan {
     uint32_t scene_f4, light_f4;  // float4s of scene / light records staged in LDS (0 = global)
@@ -176,8 +177,8 @@ class Engine {
     bool use_lds = true;  // stage scene records in LDS when they fit
     uint32_t vis_blocks = 0, ff_blocks = 0, jacobi_blocks = 0;  // 0 = automatic
     uint32_t vis_chunk = 0;     // link-visibility queue chunk per wavefront (0 = automatic)
-    uint32_t vis_min_idle = kVisMinIdle;  // link visibility: idle lanes refill once this many are idle
-    uint32_t closure_min_idle = 1;  // closure rays: the same for k_closure
+    uint32_t vis_min_idle = kMarchMinIdle;  // link visibility: idle lanes refill once this many are idle
+    uint32_t closure_min_idle = kMarchMinIdle;  // closure rays: the same for k_closure
     // Wavefront-shared queries of marches with a distance bound test each
     // object's box against the wavefront's points instead of walking the BVH.
     bool flat_query = true;

@@ -456,16 +456,19 @@ __global__ __launch_bounds__(kBlock) void k_direct_terms(SceneView Sg, LightsVie
                                                          float* gterm, unsigned long long* evals,
                                                          unsigned long long* ops) {
     IFX_SETUP(LDS, true, COUNT);
-    uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    // every lane of the grid calls diffuse_term (tail lanes with in = false)
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const bool in = t < N * nl;
     uint32_t steps = 0;
-    if (t < N * nl) {
-        const uint32_t li = t / N, r = t - li * N, i = row_order[r];
-        f3 p = ld3(mpos[r]), n = ld3(mnrm[r]);
-        f3 tan, bit;
-        frame(n, tan, bit);
-        f3 term = spl(0.0f);
-        float g = 0.0f;
-        bool on = diffuse_term(S, L, li, p, n, tan, bit, ld3(salb[i]), term, g, steps, stk);
+    const uint32_t tc = in ? t : 0u;
+    const uint32_t li = tc / N, r = tc - li * N, i = row_order[r];
+    f3 p = ld3(mpos[r]), n = ld3(mnrm[r]);
+    f3 tan, bit;
+    frame(n, tan, bit);
+    f3 term = spl(0.0f);
+    float g = 0.0f;
+    const bool on = diffuse_term(S, L, li, p, n, tan, bit, ld3(salb[i]), term, g, steps, stk, in);
+    if (in) {
         terms[(size_t)li * N + i] = on ? st4(term, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         gterm[(size_t)li * N + i] = on ? g : 0.0f;
     }

@@ -571,10 +571,12 @@ constexpr uint32_t kMortonBits = 30;
 // all fields are wave-uniform.
 // Idle lanes wait until at least min_idle (1..64) of them are idle, so the
 // lanes refilled together start their rays together.  The link visibility
-// refills a whole wavefront at once (kVisMinIdle): 64 consecutive candidates
-// (one cluster, Morton-adjacent surfels) march side by side, so the shared
-// query's union of their nodes stays near one lane's (stress scene: 311 ->
-// 222 ms, although 27 % of the lane slots then idle at the batch tails).
+// and closure kernels refill a whole wavefront at once (kMarchMinIdle): 64
+// consecutive items (one cluster and Morton-adjacent surfels, or one
+// direction from Morton-adjacent surfels) march side by side, so the shared
+// query's union of nodes or listed objects stays near one lane's (stress
+// scene link visibility: 311 -> 222 ms before the flat list, although 27 %
+// of the lane slots then idle at the batch tails).
 struct WaveQueue {
     uint32_t next, end, chunk;
     bool drained;

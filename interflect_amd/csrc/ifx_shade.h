@@ -219,14 +219,16 @@ IFX_HD bool shadow_step(const SceneView& S, ShadowMarch& m, bool live, float& vi
     return live && m.step(S, vis, stk);
 }
 
+// `go` false: the lane only follows along (shared stacks: its wavefront's
+// other lanes can then keep whole-wavefront queries) and gets vis 0, steps 0.
 template <class Stk>
 IFX_HD float soft_shadow(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, float k, uint32_t& steps_out,
-                         Stk& stk) {
+                         Stk& stk, bool go = true) {
     ShadowMarch m;
     m.begin(ro, rd, tmin, tmax, k);
     float vis = 0.0f;
     if constexpr (Stk::kShared) {
-        bool live = true;
+        bool live = go;
         while (any_lane(live)) {
             float v = 0.0f;
             if (shadow_step(S, m, live, v, stk)) {
@@ -234,7 +236,7 @@ IFX_HD float soft_shadow(const SceneView& S, f3 ro, f3 rd, float tmin, float tma
                 live = false;
             }
         }
-    } else {
+    } else if (go) {
         while (!m.step(S, vis, stk)) {
         }
     }
@@ -425,43 +427,58 @@ IFX_HD float ext_shadow(const SceneView& S, f3 p, f3 n, const ExtGeo& g, uint32_
 
 // Light `li` (quad lights first, then EXT sphere and point lights) of the
 // diffuse direct term (src/shade.rs:137-162).  Returns false when the
-// reference's loop would `continue` past this light without adding.
+// reference's loop would `continue` past this light without adding.  The
+// culls come first and the shadow march after them is entered by every
+// calling lane (culled lanes, and lanes with `on` false, only follow along),
+// so a wavefront that calls together keeps whole-wavefront queries.
 template <class Stk>
 IFX_HD bool diffuse_term(const SceneView& S, const LightsView& Lv, uint32_t li, f3 p, f3 n, f3 tan, f3 bit, f3 albedo,
-                         f3& term, float& g, uint32_t& steps, Stk& stk) {
+                         f3& term, float& g, uint32_t& steps, Stk& stk, bool on = true) {
+    bool go = on;
+    f3 ldir = spl(0.0f), emit = spl(0.0f);
+    float ff = 0.0f, k = 1.5f, end = 0.05f;
     if (li < Lv.n_quad) {
         const float4* L = Lv.quad + 7 * li;
-        f3 lc = ld3(L[5]);
-        f3 to_light = lc - p;
-        if (dot3(ld3(L[6]), -nrm3(to_light)) <= 0.0f) return false;
-        float ff = quad_ff(L, p, tan, bit, n);
-        if (ff <= 1e-6f) return false;
-        float dist = len3(to_light);
-        float radius = L[5].w;
-        float k = clmp(dist / radius, 1.5f, 96.0f);
-        float end = fmx(dist - radius * 1.25f, 0.05f);
-        uint32_t s = 0;
-        float vis = soft_shadow(S, p + n * 3e-3f, to_light / dist, 0.015f, end, k, s, stk);
-        steps += s;
-        if (vis <= 1e-4f) return false;
-        g = ff * vis;
-        term = (albedo * ld3(L[4])) * g;
-        return true;
+        if (go) {
+            f3 lc = ld3(L[5]);
+            f3 to_light = lc - p;
+            go = !(dot3(ld3(L[6]), -nrm3(to_light)) <= 0.0f);
+            if (go) {
+                ff = quad_ff(L, p, tan, bit, n);
+                go = !(ff <= 1e-6f);
+            }
+            if (go) {
+                float dist = len3(to_light);
+                float radius = L[5].w;
+                k = clmp(dist / radius, 1.5f, 96.0f);
+                end = fmx(dist - radius * 1.25f, 0.05f);
+                ldir = to_light / dist;
+                emit = ld3(L[4]);
+            }
+        }
+    } else {
+        const uint32_t le = li - Lv.n_quad;
+        const bool sph = le < Lv.n_sph;  // EXT
+        const float4* L = sph ? Lv.sph + 2 * le : Lv.pnt + 2 * (le - Lv.n_sph);
+        if (go) {
+            ExtGeo geo;
+            go = sph ? sphere_geo(L, p, n, geo) : point_geo(L, p, n, geo);
+            if (go) {
+                k = clmp(geo.dist / geo.radius, 1.5f, 96.0f);
+                end = fmx(geo.dist - geo.radius * 1.25f, 0.05f);
+                ldir = geo.ldir;
+                ff = geo.ff;
+                emit = ld3(L[1]);
+            }
+        }
     }
-    li -= Lv.n_quad;
-    ExtGeo geo;
-    const float4* L;
-    if (li < Lv.n_sph) {  // EXT
-        L = Lv.sph + 2 * li;
-        if (!sphere_geo(L, p, n, geo)) return false;
-    } else {  // EXT
-        L = Lv.pnt + 2 * (li - Lv.n_sph);
-        if (!point_geo(L, p, n, geo)) return false;
-    }
-    float vis = ext_shadow(S, p, n, geo, steps, stk);
+    uint32_t s = 0;
+    float vis = soft_shadow(S, p + n * 3e-3f, ldir, 0.015f, end, k, s, stk, go);
+    if (!go) return false;
+    steps += s;
     if (vis <= 1e-4f) return false;
-    g = geo.ff * vis;
-    term = (albedo * ld3(L[1])) * g;
+    g = ff * vis;
+    term = (albedo * emit) * g;
     return true;
 }
 
