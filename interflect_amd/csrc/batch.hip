@@ -315,10 +315,9 @@ __global__ __launch_bounds__(kBlock) void kb_shade(const BSlot* __restrict__ slo
     const uint32_t x = bx * 16u + (wv & 1u) * 8u + (l & 7u);
     const uint32_t y = by * 16u + (wv >> 1) * 8u + (l >> 3);
     uint32_t local = 0;
-    if (x < A.W && y < A.H) {
-        f3 c = shade_pixel(S, L, A, x, y, local, stk);
-        film[(size_t)T.px0 + (size_t)y * A.W + x] = st4(c);
-    }
+    const bool on = x < A.W && y < A.H;  // every lane calls (pixels off the film follow along)
+    f3 c = shade_pixel(S, L, A, on ? x : 0u, on ? y : 0u, local, stk, on);
+    if (on) film[(size_t)T.px0 + (size_t)y * A.W + x] = st4(c);
     count_steps(evals + 4 * sid + 2, local);
 }
 

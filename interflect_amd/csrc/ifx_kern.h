@@ -22,15 +22,29 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
-__device__ __forceinline__ float wave_minf(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
+// Minimum / maximum of a finite float over all 64 lanes, which must all be
+// active: on order-preserving integer keys, DPP steps within each row of 16
+// lanes (xor 1, xor 2, half-row mirror, row mirror), then the four rows'
+// values through readlane.  The result is wave-uniform.
+__device__ __forceinline__ int32_t f2key(float x) {
+    const int32_t b = __float_as_int(x);
+    return b ^ ((b >> 31) & 0x7fffffff);
 }
+__device__ __forceinline__ float key2f(int32_t k) { return __int_as_float(k ^ ((k >> 31) & 0x7fffffff)); }
 
-__device__ __forceinline__ float wave_maxf(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
+template <bool MAX>
+__device__ __forceinline__ int32_t wave_reduce_full(int32_t v) {
+    auto op = [](int32_t a, int32_t b) { return MAX ? (a > b ? a : b) : (a < b ? a : b); };
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));   // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));   // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));  // row_half_mirror
+    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));  // row_mirror
+    const int32_t a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+    const int32_t c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return op(op(a, b), op(c, d));
 }
+__device__ __forceinline__ float wave_min_full(float v) { return key2f(wave_reduce_full<false>(f2key(v))); }
+__device__ __forceinline__ float wave_max_full(float v) { return key2f(wave_reduce_full<true>(f2key(v))); }
 
 // Adds a per-lane step count to a 64-bit counter, one atomic per wave.
 __device__ __forceinline__ void count_steps(unsigned long long* ctr, uint32_t steps) {
@@ -320,9 +334,9 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
         const ConstF4 obox = as_const(S.g_obox);
         f3 wlo = need ? p : spl(kFltMax), whi = need ? p : spl(-kFltMax);
         float wb = need ? blin : 0.0f;
-        wlo = mk(wave_minf(wlo.x), wave_minf(wlo.y), wave_minf(wlo.z));
-        whi = mk(wave_maxf(whi.x), wave_maxf(whi.y), wave_maxf(whi.z));
-        wb = wave_maxf(wb);
+        wlo = mk(wave_min_full(wlo.x), wave_min_full(wlo.y), wave_min_full(wlo.z));
+        whi = mk(wave_max_full(whi.x), wave_max_full(whi.y), wave_max_full(whi.z));
+        wb = wave_max_full(wb);
         const float wb2 = wb * wb;
         const uint32_t lane = lane_id();
         for (uint32_t g = 0; g < S.n_obj; g += 64u) {
@@ -474,11 +488,13 @@ __device__ __forceinline__ int trace_step_shared(const SceneView& S, TraceMarch&
     return st;
 }
 
-__device__ __forceinline__ float query_dist(const SceneView& S, f3 p, LdsStack& stk, float ub = kFltMax) {
-    return wave_dist(S, p, true, stk, ub);
+__device__ __forceinline__ float query_dist(const SceneView& S, f3 p, LdsStack& stk, float ub = kFltMax,
+                                            bool need = true) {
+    return wave_dist(S, p, need, stk, ub);
 }
-__device__ __forceinline__ float query_dist(const SceneView& S, f3 p, ExecStack& stk, float ub = kFltMax) {
-    return wave_dist(S, p, true, stk, ub);
+__device__ __forceinline__ float query_dist(const SceneView& S, f3 p, ExecStack& stk, float ub = kFltMax,
+                                            bool need = true) {
+    return wave_dist(S, p, need, stk, ub);
 }
 __device__ __forceinline__ bool shadow_step(const SceneView& S, ShadowMarch& m, bool live, float& vis,
                                             LdsStack& stk) {
@@ -607,75 +623,99 @@ __device__ __forceinline__ bool refill(bool live, WaveQueue& wq, uint32_t* queue
     return !live && rank < take;
 }
 
-// One pixel of render_frame (src/main.rs:247-361).
+// One pixel of render_frame (src/main.rs:247-361).  The reference's early
+// returns become a per-lane outcome: every march (camera ray, its normal
+// taps, shadows, mirror ray) is entered by all calling lanes, lanes done or
+// off (`on` false: no pixel; the result is then meaningless) only following
+// along, so a wavefront that calls together keeps whole-wavefront queries.
 template <class Stk>
-__device__ __forceinline__ f3 shade_pixel(const SceneView& S, const LightsView& L, const ShadeArgs& A, uint32_t x, uint32_t y,
-                          uint32_t& local, Stk& stk) {
+__device__ __forceinline__ f3 shade_pixel(const SceneView& S, const LightsView& L, const ShadeArgs& A, uint32_t x,
+                                          uint32_t y, uint32_t& local, Stk& stk, bool on = true) {
     const f3 bg = mk(0.02f, 0.03f, 0.05f);
     const CamV& cam = A.cam;
     float px = (2.0f * ((float)x + 0.5f) / (float)A.W - 1.0f) * cam.aspect * cam.tan_half;
     float py = (1.0f - 2.0f * ((float)y + 0.5f) / (float)A.H) * cam.tan_half;
     f3 rd = nrm3((cam.fwd + cam.right * px) + cam.up * py);
     if (A.mode == M_REFERENCE) {  // src/main.rs:249-258: spp paths, seeded per pixel
+        if (!on) return bg;
         PathRng rng(((uint64_t)y << 32) | (uint64_t)x);
         f3 acc = spl(0.0f);
         for (uint32_t k = 0; k < A.spp; ++k) acc = acc + ref_path(S, L, cam.origin, rd, A.tmax, 12u, rng, stk);
         return acc / (float)A.spp;
     }
     uint32_t steps = 0;
-    Hit hh = trace<true>(S, cam.origin, rd, 1e-3f, A.tmax, steps, stk);
+    Hit hh = trace<true>(S, cam.origin, rd, 1e-3f, A.tmax, steps, stk, on);
     local += steps;
-    float surf_t = hh.hit ? hh.t : kFltMax;
-    bool tonemapped = A.mode <= M_INDIRECT;
-    f3 e;
-    if (emitter_hit(L, cam.origin, rd, fmn(surf_t, A.tmax), e)) return tonemapped ? e : spl(0.9f);
-    if (!hh.hit) return bg;
-    if (A.mode == M_NORMALS) return hh.n * 0.5f + spl(0.5f);
-    if (A.mode == M_DEPTH) {
-        float d = 1.0f - clmp(hh.t / A.tmax, 0.0f, 1.0f);
-        return spl(d * d);
+    bool alive = on;
+    f3 res = bg;
+    if (alive) {
+        float surf_t = hh.hit ? hh.t : kFltMax;
+        bool tonemapped = A.mode <= M_INDIRECT;
+        f3 e;
+        if (emitter_hit(L, cam.origin, rd, fmn(surf_t, A.tmax), e)) {
+            res = tonemapped ? e : spl(0.9f);
+            alive = false;
+        } else if (!hh.hit) {
+            res = bg;
+            alive = false;
+        } else if (A.mode == M_NORMALS) {
+            res = hh.n * 0.5f + spl(0.5f);
+            alive = false;
+        } else if (A.mode == M_DEPTH) {
+            float d = 1.0f - clmp(hh.t / A.tmax, 0.0f, 1.0f);
+            res = spl(d * d);
+            alive = false;
+        } else if (A.mode == M_STEPS) {
+            float f = (float)steps / (float)kMaxSteps;
+            res = mk(f, 1.0f - f, 0.2f);
+            alive = false;
+        }
     }
-    if (A.mode == M_STEPS) {
-        float f = (float)steps / (float)kMaxSteps;
-        return mk(f, 1.0f - f, 0.2f);
-    }
-    MatV m = load_mat(L, hh.mat);
+    MatV m = load_mat(L, alive ? hh.mat : 0u);
     f3 c = spl(0.0f);
     if (A.mode != M_INDIRECT) {
         uint32_t s = 0;
-        f3 d = direct(S, L, hh.p, hh.n, -rd, m, s, stk);
-        local += s;
-        c = c + (d + m.emissive);
+        f3 d = direct(S, L, hh.p, hh.n, -rd, m, s, stk, alive);
+        if (alive) {
+            local += s;
+            c = c + (d + m.emissive);
+        }
     }
     if (A.use_gi) {
-        f3 irr = gather(A.grid, A.spos, A.snrm, A.irr, hh.p, hh.n, A.radius);
-        c = c + m.kd * irr;
-        if (m.roughness < 0.45f || m.metallic > 0.05f) {
+        if (alive) {
+            f3 irr = gather(A.grid, A.spos, A.snrm, A.irr, hh.p, hh.n, A.radius);
+            c = c + m.kd * irr;
+        }
+        const bool glossy = alive && (m.roughness < 0.45f || m.metallic > 0.05f);
+        if (any_lane(glossy)) {
             f3 refl = rd - hh.n * (2.0f * dot3(hh.n, rd));
             uint32_t s = 0;
-            Hit rh = trace<true>(S, hh.p + hh.n * 1e-3f, refl, 1e-3f, A.tmax, s, stk);
-            local += s;
-            float rt = rh.hit ? rh.t : kFltMax;
-            f3 refl_col, e2;
-            if (emitter_hit(L, hh.p, refl, fmn(rt, A.tmax), e2)) {
-                refl_col = e2;
-            } else if (rh.hit) {
-                MatV m2 = load_mat(L, rh.mat);
-                uint32_t s2 = 0;
-                f3 d2 = direct(S, L, rh.p, rh.n, -refl, m2, s2, stk);
+            Hit rh = trace<true>(S, hh.p + hh.n * 1e-3f, refl, 1e-3f, A.tmax, s, stk, glossy);
+            f3 refl_col = bg, e2;
+            bool rlit = false;  // the mirror ray hit a surface: its direct light and gather
+            if (glossy) {
+                local += s;
+                float rt = rh.hit ? rh.t : kFltMax;
+                if (emitter_hit(L, hh.p, refl, fmn(rt, A.tmax), e2)) refl_col = e2;
+                else if (rh.hit) rlit = true;
+            }
+            MatV m2 = load_mat(L, rlit ? rh.mat : 0u);
+            uint32_t s2 = 0;
+            f3 d2 = direct(S, L, rh.p, rh.n, -refl, m2, s2, stk, rlit);
+            if (rlit) {
                 local += s2;
                 f3 i2 = gather(A.grid, A.spos, A.snrm, A.irr, rh.p, rh.n, A.radius);
                 refl_col = (d2 + m2.kd * i2) + m2.emissive;
-            } else {
-                refl_col = bg;
             }
-            f3 f = fresnel_schlick(fmx(dot3(hh.n, -rd), 0.0f), m.f0);
-            float gloss = clmp(1.0f - m.roughness / 0.45f, 0.0f, 1.0f);
-            gloss = gloss * gloss;
-            c = c + (f * refl_col) * gloss;
+            if (glossy) {
+                f3 f = fresnel_schlick(fmx(dot3(hh.n, -rd), 0.0f), m.f0);
+                float gloss = clmp(1.0f - m.roughness / 0.45f, 0.0f, 1.0f);
+                gloss = gloss * gloss;
+                c = c + (f * refl_col) * gloss;
+            }
         }
     }
-    return c;
+    return alive ? c : res;
 }
 
 

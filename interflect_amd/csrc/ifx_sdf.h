@@ -413,11 +413,12 @@ IFX_HD float dist(const SceneView& S, f3 p, Stk& stk) {
 // the loops are instantiated) that answer all active lanes with one
 // traversal and return the same values.
 // (ub: an upper bound on the distance at p when the caller has one; the
-// per-lane version does not need it)
+// per-lane version does not need it.  need false: the lane wants no value,
+// it only follows its wavefront's shared query, and gets 0.)
 template <class Stk>
-IFX_HD float query_dist(const SceneView& S, f3 p, Stk& stk, float ub = kFltMax) {
+IFX_HD float query_dist(const SceneView& S, f3 p, Stk& stk, float ub = kFltMax, bool need = true) {
     (void)ub;
-    return dist(S, p, stk);
+    return need ? dist(S, p, stk) : 0.0f;
 }
 
 // The material at a point (a trace's hit when its step did not report it).
@@ -436,15 +437,16 @@ IFX_HD bool any_lane(bool v) {
 
 // src/scene.rs:218-236: tetrahedral 4-tap gradient.
 template <class Stk>
-IFX_HD f3 scene_normal(const SceneView& S, f3 p, Stk& stk, float d0 = kFltMax) {
+IFX_HD f3 scene_normal(const SceneView& S, f3 p, Stk& stk, float d0 = kFltMax, bool need = true) {
     const float H = 5e-4f;
     // d0: the distance at p, when known; each tap lies H sqrt(3) < 8.7e-4 away
+    // (need false: the lane only follows along, its result is meaningless)
     const float ub = d0 < kFltMax ? d0 + 8.7e-4f : kFltMax;
     f3 n = spl(0.0f);
-    n = n + mk(1.0f, -1.0f, -1.0f) * query_dist(S, p + mk(1.0f, -1.0f, -1.0f) * H, stk, ub);
-    n = n + mk(-1.0f, -1.0f, 1.0f) * query_dist(S, p + mk(-1.0f, -1.0f, 1.0f) * H, stk, ub);
-    n = n + mk(-1.0f, 1.0f, -1.0f) * query_dist(S, p + mk(-1.0f, 1.0f, -1.0f) * H, stk, ub);
-    n = n + mk(1.0f, 1.0f, 1.0f) * query_dist(S, p + mk(1.0f, 1.0f, 1.0f) * H, stk, ub);
+    n = n + mk(1.0f, -1.0f, -1.0f) * query_dist(S, p + mk(1.0f, -1.0f, -1.0f) * H, stk, ub, need);
+    n = n + mk(-1.0f, -1.0f, 1.0f) * query_dist(S, p + mk(-1.0f, -1.0f, 1.0f) * H, stk, ub, need);
+    n = n + mk(-1.0f, 1.0f, -1.0f) * query_dist(S, p + mk(-1.0f, 1.0f, -1.0f) * H, stk, ub, need);
+    n = n + mk(1.0f, 1.0f, 1.0f) * query_dist(S, p + mk(1.0f, 1.0f, 1.0f) * H, stk, ub, need);
     float l = len3(n);
     if (l > 1e-12f) return n / l;
     return mk(0.0f, 1.0f, 0.0f);

@@ -101,15 +101,19 @@ IFX_HD int trace_step(const SceneView& S, TraceMarch& m, bool live, uint32_t& ma
     return st;
 }
 
+// go false: the lane marches nothing (a miss with 0 steps) but follows its
+// wavefront's shared queries, the hit normals' included, so lanes that call
+// together keep whole-wavefront queries.
 template <bool NORMAL, class Stk>
-IFX_HD Hit trace(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, uint32_t& steps_out, Stk& stk) {
+IFX_HD Hit trace(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, uint32_t& steps_out, Stk& stk,
+                 bool go = true) {
     TraceMarch m;
     m.begin(ro, rd, tmin, tmax);
     int st = 0;
     bool have_mat = false;
     uint32_t mat = 0;
     if constexpr (Stk::kShared) {
-        bool live = true;
+        bool live = go;
         while (any_lane(live)) {
             uint32_t mm = 0;
             bool hm = false;
@@ -121,7 +125,7 @@ IFX_HD Hit trace(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, uint3
                 live = false;
             }
         }
-    } else {
+    } else if (go) {
         DistHit h;
         while ((st = m.step(S, h, stk)) == 0) {
         }
@@ -130,12 +134,15 @@ IFX_HD Hit trace(const SceneView& S, f3 ro, f3 rd, float tmin, float tmax, uint3
     }
     steps_out = m.steps;
     Hit r;
+    const f3 hp = ro + rd * m.t;
+    f3 hn = spl(0.0f);
+    if constexpr (NORMAL) hn = scene_normal(S, hp, stk, m.ld, st == 1);  // every calling lane enters
     if (st == 1) {
         r.t = m.t;
-        r.p = ro + rd * m.t;
+        r.p = hp;
         // (the sample's material: the same query as the step's)
         if (!have_mat) mat = query_mat(S, r.p, stk);
-        r.n = NORMAL ? scene_normal(S, r.p, stk, m.ld) : spl(0.0f);
+        r.n = hn;
         r.mat = mat;
         r.hit = true;
     } else {
@@ -415,12 +422,14 @@ IFX_HD bool point_geo(const float4* L, f3 p, f3 n, ExtGeo& g) {
     return true;
 }
 
+// go false: the lane only follows along (vis 0, no steps; g is not read).
 template <class Stk>
-IFX_HD float ext_shadow(const SceneView& S, f3 p, f3 n, const ExtGeo& g, uint32_t& steps, Stk& stk) {
-    float k = clmp(g.dist / g.radius, 1.5f, 96.0f);
-    float end = fmx(g.dist - g.radius * 1.25f, 0.05f);
+IFX_HD float ext_shadow(const SceneView& S, f3 p, f3 n, const ExtGeo& g, uint32_t& steps, Stk& stk,
+                        bool go = true) {
+    float k = go ? clmp(g.dist / g.radius, 1.5f, 96.0f) : 1.5f;
+    float end = go ? fmx(g.dist - g.radius * 1.25f, 0.05f) : 0.05f;
     uint32_t s = 0;
-    float vis = soft_shadow(S, p + n * 3e-3f, g.ldir, 0.015f, end, k, s, stk);
+    float vis = soft_shadow(S, p + n * 3e-3f, go ? g.ldir : spl(0.0f), 0.015f, end, k, s, stk, go);
     steps += s;
     return vis;
 }
@@ -528,9 +537,11 @@ IFX_HD f3 ggx_lobe(f3 n, f3 v, f3 l, float n_dot_v, float n_dot_l, float wide, f
 }
 
 // src/shade.rs:50-125: diffuse + representative-point GGX specular.
+// on false: the lane only follows its wavefront's shadow marches (the culls
+// of each light come first, then every calling lane enters the march).
 template <class Stk>
 IFX_HD f3 direct(const SceneView& S, const LightsView& Lv, f3 p, f3 n, f3 v, const MatV& m, uint32_t& steps_out,
-                 Stk& stk) {
+                 Stk& stk, bool on = true) {
     f3 out = spl(0.0f);
     uint32_t steps = 0;
     f3 tan, bit;
@@ -542,16 +553,23 @@ IFX_HD f3 direct(const SceneView& S, const LightsView& Lv, f3 p, f3 n, f3 v, con
         f3 lc = ld3(L[5]);
         f3 to_light = lc - p;
         f3 plane_n = ld3(L[6]);
-        if (dot3(plane_n, -nrm3(to_light)) <= 0.0f) continue;
-        float ff = quad_ff(L, p, tan, bit, n);
-        if (ff <= 1e-6f) continue;
-        float dist = len3(to_light);
-        float radius = L[5].w;
-        float k = clmp(dist / radius, 1.5f, 96.0f);
-        f3 ldir = to_light / dist;
-        float end = fmx(dist - radius * 1.25f, 0.05f);
+        bool go = on && !(dot3(plane_n, -nrm3(to_light)) <= 0.0f);
+        float ff = 0.0f;
+        if (go) {
+            ff = quad_ff(L, p, tan, bit, n);
+            go = !(ff <= 1e-6f);
+        }
+        float radius = L[5].w, k = 1.5f, end = 0.05f;
+        f3 ldir = spl(0.0f);
+        if (go) {
+            float dist = len3(to_light);
+            k = clmp(dist / radius, 1.5f, 96.0f);
+            ldir = to_light / dist;
+            end = fmx(dist - radius * 1.25f, 0.05f);
+        }
         uint32_t s = 0;
-        float vis = soft_shadow(S, p + n * 3e-3f, ldir, 0.015f, end, k, s, stk);
+        float vis = soft_shadow(S, p + n * 3e-3f, ldir, 0.015f, end, k, s, stk, go);
+        if (!go) continue;
         steps += s;
         if (vis <= 1e-4f) continue;
         f3 radiance = ld3(L[4]) * vis;
@@ -574,8 +592,9 @@ IFX_HD f3 direct(const SceneView& S, const LightsView& Lv, f3 p, f3 n, f3 v, con
     for (uint32_t li = 0; li < Lv.n_sph; ++li) {  // EXT
         const float4* L = Lv.sph + 2 * li;
         ExtGeo g;
-        if (!sphere_geo(L, p, n, g)) continue;
-        float vis = ext_shadow(S, p, n, g, steps, stk);
+        const bool go = on && sphere_geo(L, p, n, g);
+        float vis = ext_shadow(S, p, n, g, steps, stk, go);
+        if (!go) continue;
         if (vis <= 1e-4f) continue;
         f3 radiance = ld3(L[1]) * vis;
         out = out + (m.kd * radiance) * g.ff;
@@ -595,8 +614,9 @@ IFX_HD f3 direct(const SceneView& S, const LightsView& Lv, f3 p, f3 n, f3 v, con
     for (uint32_t li = 0; li < Lv.n_pnt; ++li) {  // EXT
         const float4* L = Lv.pnt + 2 * li;
         ExtGeo g;
-        if (!point_geo(L, p, n, g)) continue;
-        float vis = ext_shadow(S, p, n, g, steps, stk);
+        const bool go = on && point_geo(L, p, n, g);
+        float vis = ext_shadow(S, p, n, g, steps, stk, go);
+        if (!go) continue;
         if (vis <= 1e-4f) continue;
         f3 emit = ld3(L[1]);
         out = out + (m.kd * emit) * (g.ff * vis);

@@ -16,10 +16,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneView Sg, LightsView Lg, L
     uint32_t x = blockIdx.x * 16u + (wv & 1u) * 8u + (l & 7u);
     uint32_t y = blockIdx.y * 16u + (wv >> 1) * 8u + (l >> 3);
     uint32_t local = 0;
-    if (x < A.W && y < A.H) {
-        f3 c = shade_pixel(S, L, A, x, y, local, stk);
-        film[(size_t)y * A.W + x] = st4(c);
-    }
+    const bool on = x < A.W && y < A.H;  // every lane calls (pixels off the film follow along)
+    f3 c = shade_pixel(S, L, A, on ? x : 0u, on ? y : 0u, local, stk, on);
+    if (on) film[(size_t)y * A.W + x] = st4(c);
     count_steps(evals, local);
     stk_flush(stk, ops);
 }
@@ -37,10 +36,9 @@ __global__ __launch_bounds__(kBlock) void k_shade_views(SceneView Sg, LightsView
     uint32_t x = blockIdx.x * 16u + (wv & 1u) * 8u + (l & 7u);
     uint32_t y = blockIdx.y * 16u + (wv >> 1) * 8u + (l >> 3);
     uint32_t local = 0;
-    if (x < A.W && y < A.H) {
-        f3 c = shade_pixel(S, L, A, x, y, local, stk);
-        film[(size_t)blockIdx.z * A.W * A.H + (size_t)y * A.W + x] = st4(c);
-    }
+    const bool on = x < A.W && y < A.H;
+    f3 c = shade_pixel(S, L, A, on ? x : 0u, on ? y : 0u, local, stk, on);
+    if (on) film[(size_t)blockIdx.z * A.W * A.H + (size_t)y * A.W + x] = st4(c);
     count_steps(evals, local);
     stk_flush(stk, ops);
 }
