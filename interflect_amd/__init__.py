@@ -558,7 +558,7 @@ class Batch:
     def set_graph(self, on):
         self._check(lib().ifx_batch_set_graph(self._h, 1 if on else 0))
 
-    def set_tuning(self, blocks_per_cu=8, chunk_div=8, wave_rows=0):
+    def set_tuning(self, blocks_per_cu=7, chunk_div=8, wave_rows=0):
         self._check(lib().ifx_batch_set_tuning(self._h, int(blocks_per_cu), int(chunk_div), int(wave_rows)))
 
     def autotune(self):

@@ -28,7 +28,7 @@ struct BatchSceneStats {
 };
 
 struct BatchTuning {
-    uint32_t blocks_per_cu = 8;  // persistent blocks per CU of the closure / visibility kernels
+    uint32_t blocks_per_cu = kMarchWaves;  // persistent blocks per CU of the closure / visibility kernels
     uint32_t chunk_div = 8;      // queue chunk = even share per wavefront / chunk_div, 64..256 items
     uint32_t wave_rows = 0;      // Jacobi gather: 0 lane per row, 1 wavefront per row
 };

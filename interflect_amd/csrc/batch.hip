@@ -172,7 +172,7 @@ __device__ __forceinline__ bool scene_open(const uint32_t* queue, uint32_t total
 // lanes refill from that scene's queue (direction-major over the scene's
 // Morton-ordered rows) until it is drained; blocks that finish a scene move
 // on while others finish its last rays.
-__global__ __launch_bounds__(kBlock) void kb_closure(const BSlot* __restrict__ slots, const uint32_t* __restrict__ list,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMarchWaves))) void kb_closure(const BSlot* __restrict__ slots, const uint32_t* __restrict__ list,
                                                      uint32_t nlist, const float4* __restrict__ spos,
                                                      const float4* __restrict__ snrm,
                                                      const uint32_t* __restrict__ rorder,
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void kb_closure(const BSlot* __restrict__ s
 
 // Link visibility (k_visibility) over each scene's segment of the
 // cluster-major candidate order, persistent over the scene list as kb_closure.
-__global__ __launch_bounds__(kBlock) void kb_visibility(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMarchWaves))) void kb_visibility(
     const BSlot* __restrict__ slots, const uint32_t* __restrict__ list, uint32_t nlist,
     const float4* __restrict__ spos, const float4* __restrict__ snrm, const float4* __restrict__ cpos,
     const float4* __restrict__ cnrm, const uint32_t* __restrict__ cand_ci, const uint32_t* __restrict__ cand_row,
@@ -1267,7 +1267,7 @@ int Batch::autotune(double* best_ms, std::string& err) {
         if (ms < best_t) best_t = ms, best = t;
         return 0;
     };
-    for (uint32_t a : {4u, 8u, 16u})
+    for (uint32_t a : {4u, kMarchWaves, 16u})
         for (uint32_t d : {4u, 16u})
             if (int rc = trial(BatchTuning{a, d, 0u})) return rc;
     if (int rc = trial(BatchTuning{best.blocks_per_cu, best.chunk_div, 1u})) return rc;

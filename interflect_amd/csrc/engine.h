@@ -16,6 +16,9 @@ namespace ifx {
 // take new work only when this many lanes of their wavefront are idle: 64
 // refills the whole wavefront at once.
 constexpr uint32_t kMarchMinIdle = 64;
+// Wavefronts per SIMD the persistent march kernels are compiled for; with 4
+// wavefronts per block, also their resident blocks per CU.
+constexpr uint32_t kMarchWaves = 7;
 
 struct LdsPlan {
     uint32_t scene_f4, light_f4;  // float4s of scene / light records staged in LDS (0 = global)

@@ -257,11 +257,12 @@ __global__ void k_morton_copy(uint32_t N, const uint32_t* __restrict__ rorder, c
 // traversal (wave_dist; the scene stays in global memory and is read
 // through the scalar cache).  Results sit at fixed positions, so the work
 // order never changes one.
-// At least 7 wavefronts per SIMD (72 VGPRs; a few bytes spill to scratch
-// outside the sample loop): latency hiding is worth more than the registers
-// (stress scene: 96.5 ms at 5 waves, 88.7 at 6, 84.5 at 7, 85.5 at 8).
+// At least kMarchWaves wavefronts per SIMD (72 VGPRs; a few bytes spill to
+// scratch outside the sample loop): latency hiding is worth more than the
+// registers (stress scene: 96.5 ms at 5 waves, 88.7 at 6, 84.5 at 7, 85.5
+// at 8).  The persistent grid is one resident wave of blocks.
 template <int MODE, bool LDS>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_visibility(SceneView Sg, LightsView Lg, LdsPlan P,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMarchWaves))) void k_visibility(SceneView Sg, LightsView Lg, LdsPlan P,
                                                        const float4* __restrict__ spos,
                                                        const float4* __restrict__ snrm,
                                                        const float4* __restrict__ cpos,
@@ -333,7 +334,7 @@ __global__ void k_vis_scatter(uint32_t M, const uint32_t* __restrict__ order, co
 // its surfel's integer open count.  dirs[k] = (r cos phi, r sin phi,
 // sqrt(1 - u1)).
 template <int MODE, bool LDS>
-__global__ __launch_bounds__(kBlock) void k_closure(SceneView Sg, LightsView Lg, LdsPlan P,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMarchWaves))) void k_closure(SceneView Sg, LightsView Lg, LdsPlan P,
                                                     const float4* __restrict__ mpos,
                                                     const float4* __restrict__ mnrm, uint32_t N,
                                                     const uint32_t* __restrict__ row_order,
@@ -1095,7 +1096,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     CK(d_queue.ensure(16));
     CK(hipMemsetAsync(d_queue.p, 0, 16, stream));
     uint32_t* queue = d_queue.as<uint32_t>();
-    const uint32_t persist = (uint32_t)n_cu_ * 8u;
+    const uint32_t persist = (uint32_t)n_cu_ * kMarchWaves;  // resident blocks of the march kernels
 
     // ---- direct-light terms and closure rays: they need only the surfels,
     // the scene and the lights, so they run on the side stream while this
@@ -1631,7 +1632,7 @@ int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool 
                                             d_tstart.as<uint32_t>(), d_tci.as<uint32_t>(), d_trow.as<uint32_t>(),
                                             nullptr, 0u, nullptr, nullptr);
     if (M) {
-        const uint32_t persist = (uint32_t)n_cu_ * 8u;
+        const uint32_t persist = (uint32_t)n_cu_ * kMarchWaves;  // resident blocks of the march kernels
         const uint32_t visb = vis_blocks ? vis_blocks : std::max(1u, std::min<uint32_t>(persist, blocks_for(M, kBlock)));
         const Lds ld = plan_lds(false, false);
         const LightsView Lg{};
