@@ -527,6 +527,7 @@ __device__ __forceinline__ int trace_step(const SceneView& S, TraceMarch& m, boo
 // loads are issued together, then summed in link order (the order, and so
 // every bit, of the serial sum).
 constexpr uint32_t kRowBatch = 8;
+constexpr uint32_t kSliceBatch = 16;  // the same for the sliced rows
 
 // Kernel prologue: stage, then place this lane's stack column after the staged records.
 #define IFX_SETUP(LDS, LIGHTS, COUNT) IFX_SETUP_T(LDS, LIGHTS, typename StackOf<COUNT>::T)

@@ -416,19 +416,24 @@ def _frame_bytes(ctx):
     return [np.asarray(a).view(np.uint8).copy() for a in ctx.transfer_arrays() + ctx.radiosity()] + [r["rgb8"]]
 
 
-@pytest.mark.parametrize("name", ["product.rad", "stress.rad"])
+@pytest.mark.parametrize("name", ["product.rad", "stress.rad", "stress seeded"])
 def test_launch_config_invariance(name):
     """Thread-count invariance (ci.yml:81-93) restated for launch shapes: LDS
     staging on/off, visibility / cull / Jacobi grid sizes, one stream or
-    two, and the instrumented counting kernels all give the same bytes."""
+    two, the march kernels' refill policy and queue chunks, the flat object
+    list or the BVH walk in shared queries, and the instrumented counting
+    kernels all give the same bytes."""
     say("launch configs", name)
-    src = scene_text(name)
+    src = scenes.stress(surfels=4000) if name == "stress seeded" else scene_text(name)
     ctx = ia.Context(0)
     ctx.load_scene(src)
     ctx.generate_surfels(4000)
     base = _frame_bytes(ctx)
     for key, val in [("side_stream", 0), ("lds", 0), ("vis_blocks", 7), ("ff_blocks", 3), ("jacobi_blocks", 5),
-                     ("jacobi_rows", 1), ("jacobi_rows", 2), ("jacobi_rows", 3), ("side_stream", 1)]:
+                     ("jacobi_rows", 1), ("jacobi_rows", 2), ("jacobi_rows", 3), ("side_stream", 1),
+                     ("flat_query", 0), ("vis_min_idle", 1), ("closure_min_idle", 1), ("vis_chunk", 256),
+                     ("flat_query", 1), ("vis_min_idle", 24), ("closure_min_idle", 40), ("vis_chunk", 64),
+                     ("vis_min_idle", 64), ("closure_min_idle", 64), ("vis_chunk", 0), ("vis_blocks", 0)]:
         ctx.set_tuning(key, val)
         got = _frame_bytes(ctx)
         assert all(np.array_equal(a, b) for a, b in zip(base, got)), key
