@@ -910,6 +910,9 @@ void Engine::tally_ops(const char* name, const unsigned long long* cv, int slot)
     } else if (count_exec_) {
         kops_[k + ".exec_prims"] += cv[slot + 8];
         kops_[k + ".lane_slots"] += cv[slot + 12];
+        kops_[k + ".flat_queries"] += cv[slot + 16];
+        kops_[k + ".walk_queries"] += cv[slot + 20];
+        kops_[k + ".listed"] += cv[slot + 24];
     }
 }
 

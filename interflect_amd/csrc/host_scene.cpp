@@ -467,19 +467,23 @@ void flatten_scene(HostScene& s, const std::vector<BuildNode>& nodes, const f3* 
     // lin[i] = slot of object i (linear scan order); lin[n + k] = leaf node of slot k.
     const size_t n_obj = s.objects.size();
     s.lin.assign(2 * n_obj, 0);
-    s.obox_rec.assign(2 * s.objects.size(), make_float4(0, 0, 0, 0));
     for (size_t slot = 0; slot < s.order.size(); ++slot) {
-        const HObject& o = s.objects[s.order[slot]];
-        object_rec(o, &s.obj_rec[4 * slot]);
+        object_rec(s.objects[s.order[slot]], &s.obj_rec[4 * slot]);
         s.lin[s.order[slot]] = (uint32_t)slot;
-        f3 lo, hi;  // the object's box as the BVH build grows it (src/bvh.rs:62-72)
-        prim_bounds(o.prim, lo, hi);
-        s.obox_rec[2 * slot] = st4(lo - spl(o.blend));
-        s.obox_rec[2 * slot + 1] = st4(hi + spl(o.blend));
     }
     for (size_t ni = 0; ni < nodes.size(); ++ni)
         for (uint32_t k = nodes[ni].start; nodes[ni].count > 0 && k < nodes[ni].start + nodes[ni].count; ++k)
             s.lin[n_obj + k] = (uint32_t)ni;
+    // Each object's box as the BVH build grows it (src/bvh.rs:62-72), with
+    // its leaf node's index in lo.w.
+    s.obox_rec.assign(2 * s.objects.size(), make_float4(0, 0, 0, 0));
+    for (size_t slot = 0; slot < s.order.size(); ++slot) {
+        const HObject& o = s.objects[s.order[slot]];
+        f3 lo, hi;
+        prim_bounds(o.prim, lo, hi);
+        s.obox_rec[2 * slot] = st4(lo - spl(o.blend), bitsf(s.lin[n_obj + slot]));
+        s.obox_rec[2 * slot + 1] = st4(hi + spl(o.blend));
+    }
 
     s.plane_rec.assign(4 * s.planes.size(), make_float4(0, 0, 0, 0));
     for (size_t i = 0; i < s.planes.size(); ++i) object_rec(s.planes[i], &s.plane_rec[4 * i]);

@@ -162,9 +162,12 @@ constexpr uint64_t kWaveShadeMaxPixels = 1ull << 21;
 // Counter slots of a tracing kernel, from its `ops` pointer: ops[0] counted
 // FP32 operations and ops[4] primitive evaluations of the reference
 // traversal (CountStack), ops[8] primitive evaluations executed by the
-// timed traversal (ExecStack).  Every lane of the wavefront calls stk_flush.
+// timed traversal (ExecStack), ops[12] lane slots of the persistent
+// kernels' march steps, ops[16] / ops[20] wavefront queries answered by the
+// flat list / the BVH walk, ops[24] objects the flat lists held.  Every lane
+// of the wavefront calls stk_flush.
 __device__ __forceinline__ void stk_init(LdsStack&) {}
-__device__ __forceinline__ void stk_init(ExecStack& s) { s.xe = s.xw = 0u; }
+__device__ __forceinline__ void stk_init(ExecStack& s) { s.xe = s.xw = s.xf = s.xb = s.xl = 0u; }
 __device__ __forceinline__ void stk_init(CountStack& s) {
     s.ops = 0ull;
     s.prims = 0u;
@@ -172,8 +175,12 @@ __device__ __forceinline__ void stk_init(CountStack& s) {
 __device__ __forceinline__ void stk_flush(LdsStack&, unsigned long long*) {}
 __device__ __forceinline__ void stk_flush(ExecStack& s, unsigned long long* ops) {
     const uint32_t t = wave_sum(s.xe);
-    if (lane_id() == (uint32_t)__builtin_ctzll(__ballot(true)) && t + s.xw)
-        atomicAdd(ops + 8, (unsigned long long)t + s.xw);
+    if (lane_id() == (uint32_t)__builtin_ctzll(__ballot(true))) {
+        if (t + s.xw) atomicAdd(ops + 8, (unsigned long long)t + s.xw);
+        if (s.xf) atomicAdd(ops + 16, (unsigned long long)s.xf);
+        if (s.xb) atomicAdd(ops + 20, (unsigned long long)s.xb);
+        if (s.xl) atomicAdd(ops + 24, (unsigned long long)s.xl);
+    }
 }
 __device__ __forceinline__ void stk_flush(CountStack& s, unsigned long long* ops) {
     if (s.ops) atomicAdd(ops, s.ops);
@@ -262,6 +269,8 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
     if (!S.use_bvh || S.n_nodes == 0) return need ? dist_mat(S, p, stk) : DistHit{0.0f, 0u};
     const ConstF4 nodes = as_const(S.g_nodes), objs = as_const(S.g_obj), subs = as_const(S.g_subs);
     const ConstF4 planes = as_const(S.g_planes);
+    const __attribute__((address_space(4))) uint32_t* lin =
+        (const __attribute__((address_space(4))) uint32_t*)S.g_lin;  // uniform reads: scalar loads
     if (__ballot(need) == 0ull) return DistHit{0.0f, 0u};
     constexpr uint32_t kNone = 0xffffffffu;
     const bool blend = S.blend_k > 0.0f;
@@ -287,9 +296,9 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
     const float ub0 = fmx(ub, 0.0f);  // objects containing p (box distance 0) stay openable
     const float ucap = ub < kFltMax ? (blend ? (ub0 + 1.25f * K) + 2.0f * delta : ub0 + 2.0f * delta) : kFltMax;
     float blin = kFltMax, open2 = kFltMax;
+    float sm = blend ? smin_poly(b1, b2, K) : kFltMax;  // the blend pair's smooth minimum, kept current
     auto refresh = [&]() {
-        float b = blend ? (fmn(hard, smin_poly(b1, b2, K)) + 1.25f * K) + 2.0f * delta
-                        : fmx(hard, 0.0f) + 2.0f * delta;
+        float b = blend ? (fmn(hard, sm) + 1.25f * K) + 2.0f * delta : fmx(hard, 0.0f) + 2.0f * delta;
         blin = fmn(b, ucap);
         open2 = blin > 0.0f ? blin * blin : -1.0f;
     };
@@ -319,6 +328,7 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
             } else if (d < b2) {
                 b2 = d, s2 = k;
             }
+            sm = smin_poly(b1, b2, K);
         } else if (d < hard) {
             if (ni != la) h2 = hard;  // everything evaluated so far lies outside this leaf
             hard = d, a = k, la = ni, hmat = mat;
@@ -330,6 +340,7 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
     // and the reductions read every lane.
     const bool flat = S.g_obox != nullptr && S.n_obj > kWaveScanObjs && __ballot(true) == ~0ull &&
                       __ballot(need && !(ub < kFltMax)) == 0ull;
+    if (S.n_obj > kWaveScanObjs) stk.count_query(flat);
     if (flat) {
         const ConstF4 obox = as_const(S.g_obox);
         f3 wlo = need ? p : spl(kFltMax), whi = need ? p : spl(-kFltMax);
@@ -350,9 +361,11 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
             while (m) {
                 const uint32_t k = g + (uint32_t)__builtin_ctzll(m);
                 m &= m - 1ull;
-                const f3 e = max3(max3(ld3(obox[2 * k]) - p, p - ld3(obox[2 * k + 1])), spl(0.0f));
+                stk.count_listed();
+                const float4 lo = obox[2 * k];
+                const f3 e = max3(max3(ld3(lo) - p, p - ld3(obox[2 * k + 1])), spl(0.0f));
                 if (__ballot(need && dot3(e, e) < open2) == 0ull) continue;
-                take(k, __builtin_amdgcn_readfirstlane(S.lin[S.n_obj + k]));
+                take(k, fbits(lo.w));
                 refresh();
             }
         }
@@ -360,7 +373,7 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
         // few objects: the leaves in slot order (no interior nodes), each
         // opened under the same test
         for (uint32_t k = 0; k < S.n_obj;) {
-            const uint32_t ni = __builtin_amdgcn_readfirstlane(S.lin[S.n_obj + k]);
+            const uint32_t ni = lin[S.n_obj + k];
             const uint32_t cnt = fbits(nodes[2 * ni + 1].w);
             if (__ballot(need && box2(ni) < open2) != 0ull) {
                 for (uint32_t j = k; j < k + cnt; ++j) take(j, ni);
@@ -411,7 +424,6 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
             if (MAT && !(h2 > hard)) ok = false;
         }
     } else {
-        const float sm = smin_poly(b1, b2, K);
         res = sm < hard ? sm : hard;
         if (MAT) {
             if (sm < hard) {

@@ -60,15 +60,17 @@ struct SceneView {
     // The same records in global memory (the pointers above may be moved to
     // an LDS copy); the wavefront-shared traversal reads these.
     const float4 *g_obj, *g_planes, *g_subs, *g_nodes;
-    // Each object's box (lo, hi; 2 x float4 per slot) in global memory, for
-    // the flat candidate list of the wavefront-shared query; null: that
-    // query always walks the BVH.
+    const uint32_t* g_lin;
+    // Each object's box (lo with the leaf node index's bits in lo.w, hi;
+    // 2 x float4 per slot) in global memory, for the flat candidate list of
+    // the wavefront-shared query; null: that query always walks the BVH.
     const float4* g_obox;
     IFX_HD void keep_global() {
         g_obj = obj;
         g_planes = planes;
         g_subs = subs;
         g_nodes = nodes;
+        g_lin = lin;
     }
 };
 
@@ -252,6 +254,8 @@ struct LdsStack {
     __device__ __forceinline__ void count_aabb(uint32_t) {}
     __device__ __forceinline__ void count_eval() {}
     __device__ __forceinline__ void count_wave_prims(uint32_t) {}
+    __device__ __forceinline__ void count_query(bool) {}
+    __device__ __forceinline__ void count_listed() {}
 };
 
 // LdsStack of the instrumented instantiation that tallies the primitive
@@ -260,10 +264,14 @@ struct LdsStack {
 // `xw` per wavefront for the shared traversal (each object it opens is
 // evaluated by every active lane).  A separate instantiation: the counters
 // cost the timed kernels registers.
+// It also counts, per wavefront, the shared queries answered by the flat
+// list (xf) or by the BVH walk (xb) and the objects the flat lists held (xl).
 struct ExecStack : LdsStack {
-    uint32_t xe, xw;
+    uint32_t xe, xw, xf, xb, xl;
     __device__ __forceinline__ void count_prim(uint32_t, uint32_t) { ++xe; }
     __device__ __forceinline__ void count_wave_prims(uint32_t lanes) { xw += lanes; }
+    __device__ __forceinline__ void count_query(bool flat) { flat ? ++xf : ++xb; }
+    __device__ __forceinline__ void count_listed() { ++xl; }
 };
 
 struct CountStack {
