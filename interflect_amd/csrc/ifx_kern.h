@@ -263,9 +263,12 @@ __device__ __forceinline__ ConstF4 as_const(const float4* p) {
 //  1-Lipschitz), else kFltMax.  The opening bound starts from it, so nodes
 //  farther than ub (plus the same margins) are never opened: every object
 //  under them is then at least ub + delta >= the result + delta away.
+//  hint (optional, per lane): in, the slot of the object that decided the
+//  lane's previous sample (or kNone); out, the object deciding this one.
+//  See the flat list.
 template <bool MAT, class Stk>
 __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool need, Stk& stk,
-                                                 float ub = kFltMax) {
+                                                 float ub = kFltMax, uint32_t* hint = nullptr) {
     if (!S.use_bvh || S.n_nodes == 0) return need ? dist_mat(S, p, stk) : DistHit{0.0f, 0u};
     const ConstF4 nodes = as_const(S.g_nodes), objs = as_const(S.g_obj), subs = as_const(S.g_subs);
     const ConstF4 planes = as_const(S.g_planes);
@@ -334,6 +337,11 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
             hard = d, a = k, la = ni, hmat = mat;
         } else if (ni != la) {
             h2 = fmn(h2, d);
+        } else if (MAT && d == hard && k < a) {
+            // a tie within the leaf: the per-lane traversal takes the leaf's
+            // objects in slot order, so the lower slot's material wins (hints
+            // can be taken before the rest of their leaf)
+            a = k, hmat = mat;
         }
     };
     // The flat list needs the whole wavefront: lane l tests objects l, l + 64, ...
@@ -343,6 +351,28 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
     if (S.n_obj > kWaveScanObjs) stk.count_query(flat);
     if (flat) {
         const ConstF4 obox = as_const(S.g_obox);
+        // Hints first: any one object's value bounds the scene distance from
+        // above, so once the objects that decided the lanes' previous samples
+        // are taken, most lanes' opening bounds sit near their final values
+        // and the list below stays short.  At most two distinct objects, each
+        // the hint of the first lane not yet covered; the list skips them (a
+        // blend object must not be taken twice).
+        uint32_t t0 = kNone, t1 = kNone;
+        if (hint) {
+            const uint32_t hv = *hint;
+            uint64_t c = __ballot(need && hv < S.n_obj);
+            if (c) {
+                t0 = (uint32_t)__builtin_amdgcn_readlane((int)hv, (int)__builtin_ctzll(c));
+                take(t0, fbits(obox[2 * t0].w));
+                refresh();
+                c = __ballot(need && hv < S.n_obj && hv != t0);
+                if (c) {
+                    t1 = (uint32_t)__builtin_amdgcn_readlane((int)hv, (int)__builtin_ctzll(c));
+                    take(t1, fbits(obox[2 * t1].w));
+                    refresh();
+                }
+            }
+        }
         f3 wlo = need ? p : spl(kFltMax), whi = need ? p : spl(-kFltMax);
         float wb = need ? blin : 0.0f;
         wlo = mk(wave_min_full(wlo.x), wave_min_full(wlo.y), wave_min_full(wlo.z));
@@ -361,6 +391,7 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
             while (m) {
                 const uint32_t k = g + (uint32_t)__builtin_ctzll(m);
                 m &= m - 1ull;
+                if (k == t0 || k == t1) continue;
                 stk.count_listed();
                 const float4 lo = obox[2 * k];
                 const f3 e = max3(max3(ld3(lo) - p, p - ld3(obox[2 * k + 1])), spl(0.0f));
@@ -409,6 +440,7 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
         }
     }
     if (!need) return DistHit{0.0f, 0u};
+    if (hint) *hint = blend && sm < hard ? s1 : a;
     auto leaf_box = [&](uint32_t slot) {
         const uint32_t n = S.lin[S.n_obj + slot];
         return aabb_dist(p, S.nodes[2 * n], S.nodes[2 * n + 1]);
@@ -479,7 +511,7 @@ __device__ __forceinline__ int miss_step_shared(const SceneView& S, TraceMarch& 
         if (m.pre(p)) st = 2;
         else need = true;
     }
-    const float h = wave_dist(S, p, need, stk, m.dist_bound());
+    const float h = wave_dist_mat<false>(S, p, need, stk, m.dist_bound(), &m.hint).d;
     if (need) st = m.post(h);
     return st;
 }
@@ -493,7 +525,7 @@ __device__ __forceinline__ int trace_step_shared(const SceneView& S, TraceMarch&
         if (m.pre(p)) st = 2;
         else need = true;
     }
-    const DistHit h = wave_dist_mat<true>(S, p, need, stk, m.dist_bound());
+    const DistHit h = wave_dist_mat<true>(S, p, need, stk, m.dist_bound(), &m.hint);
     if (need) st = m.post(h.d);
     mat = h.mat;
     have_mat = true;

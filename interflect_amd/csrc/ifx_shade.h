@@ -42,6 +42,7 @@ struct TraceMarch {
     f3 ro, rd;
     float t, tmax, pd, ld;  // pd: previous step's distance; ld: the last sample's
     uint32_t steps;
+    uint32_t hint;  // wavefront-shared queries: the object deciding the last sample
     IFX_HD void begin(f3 ro_, f3 rd_, float tmin, float tmax_) {
         ro = ro_;
         rd = rd_;
@@ -50,6 +51,7 @@ struct TraceMarch {
         pd = kFltMax;
         ld = kFltMax;
         steps = 0;
+        hint = 0xffffffffu;
     }
     // An upper bound on the scene distance at the next sample: the previous
     // sample's distance plus the step taken from it (the step is that
