@@ -255,6 +255,77 @@ def test_cornell_unit_full_size_matches_oracle():
     assert abs(r["evals"] - ev) <= 1e-4 * ev
 
 
+def test_product_literal_full_size_matches_oracle():
+    """Configuration B at full size: the literal product scene (sphere, bowl
+    with 6 drilled cylinders, torus, capsule, floor and walls, 2 sphere
+    lights), 20000 surfels requested, 640 x 480.  Every transfer row exact,
+    iterations and evaluation counts equal, radiosity within 1e-4, image
+    within 1/255 on >= 99.9 % of pixels."""
+    say("product literal full")
+    text = scenes.product_literal()
+    ctx = ia.Context(0)
+    ctx.load_scene(text)
+    ctx.generate_surfels()
+    sc = po.Scene(text)
+    sc.set_surfels(*ctx.surfels())
+    tg, to = ctx.transfer(8), sc.transfer(8)
+    compare_transfer(ctx, sc, tg, to)
+    g, o = ctx.solve(), sc.solve()
+    assert g["iterations"] == o["iterations"] and g["evals"] == o["evals"]
+    for x, y in zip(ctx.radiosity(), sc.radiosity()):
+        assert np.linalg.norm(x - y) <= 1e-4 * np.linalg.norm(y)
+    r = ctx.shade("beauty")
+    film, ev = sc.render("beauty")
+    assert r["rgb8"].shape == (480, 640, 3)
+    frac, worst = image_close(r["rgb8"], po.encode(film, sc.exposure, True))
+    assert frac >= 0.999, (frac, worst)
+
+
+def test_stress_full_size_sampled_against_the_oracle_and_repeated():
+    """Configuration C at full size: the seeded stress scene (256 primitives,
+    25 % blended, 4 point lights), 200000 surfels, 1920 x 1080.  The oracle
+    checks a sample the host finishes in seconds: 1000 transfer rows exact
+    with their closure sky, the direct light of 10000 surfels within 1e-4,
+    and 6 film rows shaded from the device's radiosity within 1/255 on
+    >= 99.9 % of pixels; a second frame repeats the first bit for bit."""
+    say("stress full")
+    text = scenes.stress()
+    ctx = ia.Context(0)
+    ctx.load_scene(text)
+    N = ctx.generate_surfels()
+    assert N == 200000
+
+    def frame():
+        t = ctx.transfer(8)
+        s = ctx.solve()
+        r = ctx.shade("beauty")
+        arrays = [np.asarray(a).view(np.uint8).copy() for a in ctx.transfer_arrays() + ctx.radiosity()]
+        return t, s, r, arrays
+
+    t1, s1, r1, a1 = frame()
+    t2, s2, r2, a2 = frame()
+    assert (t1["links"], t1["evals"], s1["iterations"], s1["evals"], r1["evals"]) == \
+        (t2["links"], t2["evals"], s2["iterations"], s2["evals"], r2["evals"])
+    assert all(np.array_equal(x, y) for x, y in zip(a1, a2)) and np.array_equal(r1["rgb8"], r2["rgb8"])
+    say("stress full: oracle sample")
+    sc = po.Scene(text)
+    p, n, a, ar = ctx.surfels()
+    sc.set_surfels(p, n, a, ar)
+    r0 = N // 3
+    to = sc.transfer(8, rows=(r0, r0 + 1000))
+    compare_transfer(ctx, sc, t1, to, rows=(r0, r0 + 1000))
+    direct, irr, rad = ctx.radiosity()
+    k = 10000
+    dco, _ = sc.direct_diffuse(p[:k], n[:k], a[:k])
+    assert np.linalg.norm(direct[:k] - dco) <= 1e-4 * np.linalg.norm(dco)
+    sc.set_radiosity(direct, irr, rad)
+    for y0 in (300, 760):
+        film, _ = sc.render("beauty", rows=(y0, y0 + 3))
+        img = po.encode(film[y0:y0 + 3], sc.exposure, True)
+        frac, worst = image_close(r1["rgb8"][y0:y0 + 3], img)
+        assert frac >= 0.999, (y0, frac, worst)
+
+
 def _bytes_of(ctx, solve_first=True):
     return [np.asarray(x).view(np.uint8).copy() for x in ctx.transfer_arrays() + ctx.radiosity()]
 
