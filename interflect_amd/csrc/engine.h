@@ -309,7 +309,7 @@ class Engine {
     DevBuf d_smat, d_scl, d_sscale, d_lrow, d_perm, d_perm2, d_ckeyt, d_ckeyt2, d_crow, d_cw, d_colcnt, d_colstart,
         d_segcnt, d_segfirst, d_seg, d_segsum, d_ubar, d_uacc, d_umax, d_mu, d_v, d_dmax, d_pmat, d_pdr, d_pde, d_sdr,
         d_sde, d_partial;
-    DevBuf d_ufilm, d_stage, d_sbase, d_sidx, d_sw, d_sperm, d_skey, d_skey2;
+    DevBuf d_ufilm, d_stage, d_sbase, d_sidx, d_sw, d_sperm, d_skey, d_skey2, d_ffmask;
     // link tiles of the budgeted backward pass
     DevBuf d_cnt2, d_segn, d_segoff, d_tcount, d_tstart, d_tci, d_trow, d_tv, d_tkeep, d_tlstart, d_tidx, d_tw,
         d_tlrow, d_tperm, d_tperm2, d_tkeys, d_tcrow, d_tcw, d_tcolcnt, d_tcolstart, d_tqueue;

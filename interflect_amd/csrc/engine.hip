@@ -178,17 +178,20 @@ __global__ void k_cluster_compact(uint32_t nseg, const uint32_t* valid, const ui
 // One wavefront per surfel row (rows row0 .. row0 + n; count and start are
 // indexed from row0); lanes sweep the clusters 64 at a time and a ballot
 // keeps survivors in ascending cluster order.  EMIT=false counts, EMIT=true
-// writes candidates at the row's scanned offset.
+// writes candidates at the row's scanned offset.  With `masks`, the counting
+// pass stores each row's ballots (ceil(C/64) words per row) and the emitting
+// pass reads them instead of repeating the culls.
 template <bool EMIT>
 __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, const float4* __restrict__ snrm,
                                                uint32_t row0, uint32_t n, const float4* __restrict__ cpos,
                                                const float4* __restrict__ cnrm, uint32_t C, uint32_t* count,
                                                const uint32_t* __restrict__ start, uint32_t* cand_ci,
                                                uint32_t* cand_row, const uint32_t* __restrict__ morton,
-                                               uint32_t mbits, uint32_t* key, uint32_t* val) {
+                                               uint32_t mbits, uint32_t* key, uint32_t* val, uint64_t* masks) {
     uint32_t lane = lane_id();
     uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     uint32_t nwaves = (gridDim.x * kBlock) >> 6;
+    const uint32_t groups = (C + 63u) / 64u;
     for (uint32_t j = wave; j < n; j += nwaves) {
         const uint32_t i = row0 + j;
         f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
@@ -197,13 +200,20 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
         uint32_t run = 0;
         for (uint32_t c0 = 0; c0 < C; c0 += 64) {
             uint32_t c = c0 + lane;
-            bool pass = false;
-            if (c < C) {
-                float r, ff;
-                f3 dir;
-                pass = ff_geom(sp, sn, cpos[c], cnrm[c], r, dir, ff);
+            uint64_t m;
+            if (EMIT && masks) {
+                m = masks[(size_t)j * groups + (c0 >> 6)];
+            } else {
+                bool pass = false;
+                if (c < C) {
+                    float r, ff;
+                    f3 dir;
+                    pass = ff_geom(sp, sn, cpos[c], cnrm[c], r, dir, ff);
+                }
+                m = __ballot(pass);
+                if (!EMIT && masks && lane == 0) masks[(size_t)j * groups + (c0 >> 6)] = m;
             }
-            uint64_t m = __ballot(pass);
+            const bool pass = (m >> lane) & 1ull;
             if (EMIT && pass) {
                 uint32_t o = base + run + lanes_below(m);
                 cand_ci[o] = c;
@@ -1240,11 +1250,12 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     CK(d_ccount.ensure(4ull * (N + 1)));
     CK(d_cstart.ensure(4ull * (N + 1)));
     CK(hipMemsetAsync(d_ccount.as<uint32_t>() + N, 0, 4, stream));
+    CK(d_ffmask.ensure(8ull * N * ((C + 63u) / 64u)));
     const uint32_t ffb = ff_blocks ? ff_blocks : std::min<uint32_t>(blocks_for(N, 4), 8192);
     launch("ff_count", [&] {
         k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, 0u, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
                                                  d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, 0u,
-                                                 nullptr, nullptr);
+                                                 nullptr, nullptr, d_ffmask.as<uint64_t>());
     });
     if (int rc_s = scan_u32(d_ccount.as<uint32_t>(), d_cstart.as<uint32_t>(), (N + 1), err))
         return rc_s;
@@ -1277,7 +1288,8 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
                                                 d_cstart.as<uint32_t>(), d_cand_ci.as<uint32_t>(),
                                                 d_cand_row.as<uint32_t>(), d_morton.as<uint32_t>(), mbits,
                                                 sort_cand ? d_vkey.as<uint32_t>() : nullptr,
-                                                sort_cand ? d_vval.as<uint32_t>() : nullptr);
+                                                sort_cand ? d_vval.as<uint32_t>() : nullptr,
+                                                d_ffmask.as<uint64_t>());
     });
     if (sort_cand) {
         if (int rc_s = sort_u32(d_vkey.as<uint32_t>(), d_vval.as<uint32_t>(), d_vkey_s.as<uint32_t>(),
@@ -1649,7 +1661,7 @@ int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool 
     CK(hipMemsetAsync(d_tcount.as<uint32_t>() + nr, 0, 4, stream));
     k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, r0, nr, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
                                              d_tcount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, 0u, nullptr,
-                                             nullptr);
+                                             nullptr, nullptr);
     if (int rc = scan_u32(d_tcount.as<uint32_t>(), d_tstart.as<uint32_t>(), nr + 1, err)) return rc;
     if (!sized) {
         CK(hipMemcpyAsync(&M, d_tstart.as<uint32_t>() + nr, 4, hipMemcpyDeviceToHost, stream));
@@ -1660,7 +1672,7 @@ int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool 
     }
     k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, r0, nr, d_cpos.as<float4>(), d_cnrm.as<float4>(), C, nullptr,
                                             d_tstart.as<uint32_t>(), d_tci.as<uint32_t>(), d_trow.as<uint32_t>(),
-                                            nullptr, 0u, nullptr, nullptr);
+                                            nullptr, 0u, nullptr, nullptr, nullptr);
     if (M) {
         const uint32_t persist = (uint32_t)n_cu_ * kMarchWaves;  // resident blocks of the march kernels
         const uint32_t visb = vis_blocks ? vis_blocks : std::max(1u, std::min<uint32_t>(persist, blocks_for(M, kBlock)));
