@@ -228,7 +228,8 @@ __device__ __forceinline__ void place_stack(Stk& stk, float4* lds, uint32_t f4, 
 //    of the unevaluated ones); a lane whose check fails takes the per-lane
 //    traversal, so every lane returns dist() bit for bit.
 //  Scenes of at most kWaveScanObjs objects skip the interior nodes: their
-//  leaves are tested in slot order, each under the same opening test.
+//  objects (or, without object boxes, their leaves) are tested in slot
+//  order, each under the same opening test.
 //  Flat list (S.g_obox set, all 64 lanes active, every lane that needs a
 //  value has a bound ub): instead of walking the BVH, each lane tests the boxes of objects lane,
 //  lane + 64, ... against the box of the wavefront's points, grown by the
@@ -399,6 +400,17 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
                 take(k, fbits(lo.w));
                 refresh();
             }
+        }
+    } else if (S.n_obj <= kWaveScanObjs && S.g_obox) {
+        // few objects: every object in slot order, each taken when its own
+        // box passes some lane's opening test (tighter than its leaf's box)
+        const ConstF4 obox = as_const(S.g_obox);
+        for (uint32_t k = 0; k < S.n_obj; ++k) {
+            const float4 lo = obox[2 * k];
+            const f3 e = max3(max3(ld3(lo) - p, p - ld3(obox[2 * k + 1])), spl(0.0f));
+            if (__ballot(need && dot3(e, e) < open2) == 0ull) continue;
+            take(k, fbits(lo.w));
+            refresh();
         }
     } else if (S.n_obj <= kWaveScanObjs) {
         // few objects: the leaves in slot order (no interior nodes), each
