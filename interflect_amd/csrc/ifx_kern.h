@@ -433,8 +433,19 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
             if (__ballot(need && dn2 < open2) == 0ull) continue;
             const uint32_t w0 = fbits(nodes[2 * ni].w), cnt = fbits(nodes[2 * ni + 1].w);
             if (cnt > 0) {
-                for (uint32_t k = w0; k < w0 + cnt; ++k) take(k, ni);
-                refresh();
+                if (S.g_obox) {
+                    // each object of the leaf when its own box passes some lane's opening test
+                    const ConstF4 obox = as_const(S.g_obox);
+                    for (uint32_t k = w0; k < w0 + cnt; ++k) {
+                        const f3 e = max3(max3(ld3(obox[2 * k]) - p, p - ld3(obox[2 * k + 1])), spl(0.0f));
+                        if (__ballot(need && dot3(e, e) < open2) == 0ull) continue;
+                        take(k, ni);
+                        refresh();
+                    }
+                } else {
+                    for (uint32_t k = w0; k < w0 + cnt; ++k) take(k, ni);
+                    refresh();
+                }
             } else {
                 const uint32_t l = ni + 1, r = w0;
                 const float dl = box2(l), dr = box2(r);
