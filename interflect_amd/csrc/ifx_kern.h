@@ -464,18 +464,33 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
     }
     if (!need) return DistHit{0.0f, 0u};
     if (hint) *hint = blend && sm < hard ? s1 : a;
-    auto leaf_box = [&](uint32_t slot) {
-        const uint32_t n = S.lin[S.n_obj + slot];
-        return aabb_dist(p, S.nodes[2 * n], S.nodes[2 * n + 1]);
+    // The checks compare aabb_dist of a leaf's box at p (the correctly
+    // rounded length of the box offset) with thresholds.  The squared
+    // offset d2 decides unless it lies within a relative 1e-5 of the
+    // threshold's square, which bounds the rounding of d2, t * t and the
+    // factor; there the length itself is taken.  la is the leaf of a.
+    auto leaf_d2 = [&](uint32_t n) {
+        const f3 e = max3(max3(ld3(S.nodes[2 * n]) - p, p - ld3(S.nodes[2 * n + 1])), spl(0.0f));
+        return dot3(e, e);
     };
+    auto len_below = [](float d2, float t, bool strict) {
+        if (t > 1e-15f && t < 1e15f) {
+            const float t2 = t * t;
+            if (d2 < t2 * 0.99999f) return true;
+            if (d2 > t2 * 1.00001f) return false;
+        }
+        const float l = sqrtf(d2);
+        return strict ? l < t : l <= t;
+    };
+    auto slot_leaf = [&](uint32_t slot) { return S.lin[S.n_obj + slot]; };
     bool ok = true;
     float res;
     uint32_t rmat = hmat;
     if (!blend) {
         res = hard;
         if (a != kNone) {
-            const float lb = leaf_box(a);
-            ok = lb <= hard || (lb < h2 && lb < fmx(hard, 0.0f) + delta);
+            const float d2 = leaf_d2(la);
+            ok = len_below(d2, hard, false) || (len_below(d2, h2, true) && len_below(d2, fmx(hard, 0.0f) + delta, true));
             if (MAT && !(h2 > hard)) ok = false;
         }
     } else {
@@ -491,10 +506,10 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
         // the per-lane traversal's final prune threshold, lowered by a few ulps of its terms
         float thr = fmn(hard, sm) + K;
         thr = thr - 1e-6f * (fabsf(thr) + K + fabsf(res));
-        if (!(sm < hard) && a != kNone && !(leaf_box(a) < thr)) ok = false;
+        if (!(sm < hard) && a != kNone && !len_below(leaf_d2(la), thr, true)) ok = false;
         if (sm < hard + 1e-5f * (fabsf(hard) + K)) {  // the blend pair decides (or nearly)
-            if (s1 != kNone && !(leaf_box(s1) < thr)) ok = false;
-            if (b2 - b1 < K && s2 != kNone && !(leaf_box(s2) < thr)) ok = false;
+            if (s1 != kNone && !len_below(leaf_d2(slot_leaf(s1)), thr, true)) ok = false;
+            if (b2 - b1 < K && s2 != kNone && !len_below(leaf_d2(slot_leaf(s2)), thr, true)) ok = false;
         }
     }
     return ok ? DistHit{res, rmat} : dist_mat(S, p, stk);
