@@ -22,29 +22,42 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
-// Minimum / maximum of a finite float over all 64 lanes, which must all be
-// active: on order-preserving integer keys, DPP steps within each row of 16
-// lanes (xor 1, xor 2, half-row mirror, row mirror), then the four rows'
-// values through readlane.  The result is wave-uniform.
+// Order-preserving integer keys of finite floats: key order is float order.
 __device__ __forceinline__ int32_t f2key(float x) {
     const int32_t b = __float_as_int(x);
     return b ^ ((b >> 31) & 0x7fffffff);
 }
 __device__ __forceinline__ float key2f(int32_t k) { return __int_as_float(k ^ ((k >> 31) & 0x7fffffff)); }
 
-template <bool MAX>
-__device__ __forceinline__ int32_t wave_reduce_full(int32_t v) {
-    auto op = [](int32_t a, int32_t b) { return MAX ? (a > b ? a : b) : (a < b ? a : b); };
-    v = op(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));   // quad_perm [1,0,3,2]
-    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));   // quad_perm [2,3,0,1]
-    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));  // row_half_mirror
-    v = op(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));  // row_mirror
-    const int32_t a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
-    const int32_t c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
-    return op(op(a, b), op(c, d));
+// One DPP step of wave_min_keys: each key of the rows in RM takes the
+// minimum with the key CTRL moves to it; other rows see the largest key
+// (the identity of min) and keep theirs.
+template <int CTRL, int RM, int N>
+__device__ __forceinline__ void dpp_min_step(int32_t (&k)[N]) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const int32_t t = __builtin_amdgcn_update_dpp(0x7fffffff, k[i], CTRL, RM, 0xF, false);
+        k[i] = t < k[i] ? t : k[i];
+    }
 }
-__device__ __forceinline__ float wave_min_full(float v) { return key2f(wave_reduce_full<false>(f2key(v))); }
-__device__ __forceinline__ float wave_max_full(float v) { return key2f(wave_reduce_full<true>(f2key(v))); }
+
+// Minimum of each of N keys over all 64 lanes, which must all be active;
+// the results are wave-uniform.  DPP within each row of 16 lanes (xor 1,
+// xor 2, half-row mirror, row mirror), then row 0 into row 1 and row 2
+// into row 3 (row_bcast:15), then row 1 into rows 2 and 3 (row_bcast:31):
+// lane 63 holds the minimum.  The N chains are interleaved, so no step
+// waits on the one before it.
+template <int N>
+__device__ __forceinline__ void wave_min_keys(int32_t (&k)[N]) {
+    dpp_min_step<0xB1, 0xF>(k);   // quad_perm [1,0,3,2]
+    dpp_min_step<0x4E, 0xF>(k);   // quad_perm [2,3,0,1]
+    dpp_min_step<0x141, 0xF>(k);  // row_half_mirror
+    dpp_min_step<0x140, 0xF>(k);  // row_mirror
+    dpp_min_step<0x142, 0xA>(k);  // row_bcast:15
+    dpp_min_step<0x143, 0xC>(k);  // row_bcast:31
+#pragma unroll
+    for (int i = 0; i < N; ++i) k[i] = __builtin_amdgcn_readlane(k[i], 63);
+}
 
 // Adds a per-lane step count to a 64-bit counter, one atomic per wave.
 __device__ __forceinline__ void count_steps(unsigned long long* ctr, uint32_t steps) {
@@ -374,11 +387,15 @@ __device__ __forceinline__ DistHit wave_dist_mat(const SceneView& S, f3 p, bool 
                 }
             }
         }
-        f3 wlo = need ? p : spl(kFltMax), whi = need ? p : spl(-kFltMax);
-        float wb = need ? blin : 0.0f;
-        wlo = mk(wave_min_full(wlo.x), wave_min_full(wlo.y), wave_min_full(wlo.z));
-        whi = mk(wave_max_full(whi.x), wave_max_full(whi.y), wave_max_full(whi.z));
-        wb = wave_max_full(wb);
+        // the needing lanes' point box and largest opening bound (maxima as
+        // minima of complemented keys)
+        int32_t kv[7] = {f2key(need ? p.x : kFltMax),   f2key(need ? p.y : kFltMax),   f2key(need ? p.z : kFltMax),
+                         ~f2key(need ? p.x : -kFltMax), ~f2key(need ? p.y : -kFltMax), ~f2key(need ? p.z : -kFltMax),
+                         ~f2key(need ? blin : 0.0f)};
+        wave_min_keys(kv);
+        const f3 wlo = mk(key2f(kv[0]), key2f(kv[1]), key2f(kv[2]));
+        const f3 whi = mk(key2f(~kv[3]), key2f(~kv[4]), key2f(~kv[5]));
+        const float wb = key2f(~kv[6]);
         const float wb2 = wb * wb;
         const uint32_t lane = lane_id();
         for (uint32_t g = 0; g < S.n_obj; g += 64u) {
