@@ -551,7 +551,7 @@ __device__ __forceinline__ bool shadow_step_shared(const SceneView& S, ShadowMar
         ended = m.pre(vis, p);
         need = !ended;
     }
-    const float h = wave_dist(S, p, need, stk, m.dist_bound());
+    const float h = wave_dist_mat<false>(S, p, need, stk, m.dist_bound(), &m.hint).d;
     if (need) ended = m.post(h, vis);
     return live && ended;
 }

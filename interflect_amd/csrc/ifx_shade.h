@@ -162,6 +162,7 @@ struct ShadowMarch {
     f3 ro, rd;
     float t, tmax, k, ph, res;
     uint32_t steps;
+    uint32_t hint;  // wavefront-shared queries: the object deciding the last sample
     IFX_HD void begin(f3 ro_, f3 rd_, float tmin, float tmax_, float k_) {
         ro = ro_;
         rd = rd_;
@@ -171,6 +172,7 @@ struct ShadowMarch {
         ph = 1e20f;
         res = 1.0f;
         steps = 0;
+        hint = 0xffffffffu;
     }
     // An upper bound on the scene distance at the next sample: the previous
     // sample's distance plus the step taken from it (1-Lipschitz SDF);
