@@ -274,6 +274,10 @@ ifx_status ifx_set_counting(ifx_ctx* ctx, int32_t on);
  *   "vis_min_idle"  1..64: idle link-visibility lanes take new candidates once this many
  *                   lanes of their wavefront are idle (default 64)
  *   "closure_min_idle" 1..64: the same for the closure rays (default 64)
+ *   "vis_tail"      0..32: a whole-wavefront link-visibility batch parks its rays once this
+ *                   many or fewer still march, and a second launch marches the parked rays
+ *                   (default 8; 0 = off; needs vis_min_idle 64)
+ *   "vis_tail_items" 0..2^30: parking applies from this many candidates up (default 2^23)
  *   "flat_query"    1 = a wavefront's scene queries along marches test every object's box
  *                   against the wavefront's points (default), 0 = they walk the BVH
  *   "ff_blocks"     grid of the candidate-cull kernels (0 = automatic)

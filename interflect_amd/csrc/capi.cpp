@@ -632,6 +632,14 @@ ifx_status ifx_set_tuning(ifx_ctx* ctx, const char* key, int64_t value) {
                 return IFX_E_INVALID;
             }
             ctx->eng.vis_min_idle = (uint32_t)value;
+        } else if (k == "vis_tail") {
+            if (value < 0 || value > 32) {
+                ctx->err = "vis_tail must be in 0..32";
+                return IFX_E_INVALID;
+            }
+            ctx->eng.vis_tail = (uint32_t)value;
+        } else if (k == "vis_tail_items") {
+            ctx->eng.vis_tail_items = (uint32_t)value;
         } else if (k == "closure_min_idle") {
             if (value < 1 || value > 64) {
                 ctx->err = "closure_min_idle must be in 1..64";

@@ -16,6 +16,13 @@ namespace ifx {
 // take new work only when this many lanes of their wavefront are idle: 64
 // refills the whole wavefront at once.
 constexpr uint32_t kMarchMinIdle = 64;
+// Link-visibility batches park their rays once this many or fewer still
+// march (tuning key vis_tail), in transfers of at least kVisTailItems
+// candidates (vis_tail_items): below that the second launch and its index
+// cost more than the idle lanes (product scene, 1.4M candidates: 1.32 ->
+// 1.61 ms; stress scene, 68M: 76.2 -> 74.6 ms).
+constexpr uint32_t kVisTail = 8;
+constexpr uint32_t kVisTailItems = 1u << 23;
 // Wavefronts per SIMD the persistent march kernels are compiled for; with 4
 // wavefronts per block, also their resident blocks per CU.
 constexpr uint32_t kMarchWaves = 7;
@@ -182,6 +189,11 @@ class Engine {
     uint32_t vis_chunk = 0;     // link-visibility queue chunk per wavefront (0 = automatic)
     uint32_t vis_min_idle = kMarchMinIdle;  // link visibility: idle lanes refill once this many are idle
     uint32_t closure_min_idle = kMarchMinIdle;  // closure rays: the same for k_closure
+    // Link visibility: a lockstep batch parks its rays once at most this
+    // many still march, and a second launch marches the parked rays (0: off;
+    // needs vis_min_idle 64 and at least vis_tail_items candidates).
+    uint32_t vis_tail = kVisTail;
+    uint32_t vis_tail_items = kVisTailItems;
     // Wavefront-shared queries of marches with a distance bound test each
     // object's box against the wavefront's points instead of walking the BVH.
     bool flat_query = true;
@@ -310,6 +322,7 @@ class Engine {
         d_segcnt, d_segfirst, d_seg, d_segsum, d_ubar, d_uacc, d_umax, d_mu, d_v, d_dmax, d_pmat, d_pdr, d_pde, d_sdr,
         d_sde, d_partial;
     DevBuf d_ufilm, d_stage, d_sbase, d_sidx, d_sw, d_sperm, d_skey, d_skey2, d_ffmask;
+    DevBuf d_tail_st, d_tail_cnt, d_tail_start, d_tail_idx;  // parked link-visibility rays
     // link tiles of the budgeted backward pass
     DevBuf d_cnt2, d_segn, d_segoff, d_tcount, d_tstart, d_tci, d_trow, d_tv, d_tkeep, d_tlstart, d_tidx, d_tw,
         d_tlrow, d_tperm, d_tperm2, d_tkeys, d_tcrow, d_tcw, d_tcolcnt, d_tcolstart, d_tqueue;

@@ -271,6 +271,26 @@ __global__ void k_morton_copy(uint32_t N, const uint32_t* __restrict__ rorder, c
 // scratch outside the sample loop): latency hiding is worth more than the
 // registers (stress scene: 96.5 ms at 5 waves, 88.7 at 6, 84.5 at 7, 85.5
 // at 8).  The persistent grid is one resident wave of blocks.
+//
+// Parked tails (VisTail).  A lockstep batch (64 consecutive items, taken
+// when the whole wavefront is idle; batch b starts at item 64 b) whose
+// marching rays drop to `at` or fewer parks them: each stores its march
+// state in st (four float4s: ro | t, rd | tmax, k ph res ff, steps hint
+// slot) at b * at + its rank, lane 0 stores their number in cnt[b], and the
+// wavefront takes the next batch instead of stepping with most lanes idle.
+// A second launch (idx set) marches the parked rays, taken in batch order,
+// to their ends: consecutive batches look at one cluster from neighbouring
+// surfels, so its wavefronts stay coherent.  A ray's march is the same
+// sample sequence wherever it runs, so results and step counts do not
+// change.
+struct VisTail {
+    float4* st = nullptr;
+    uint32_t* cnt = nullptr;          // first launch: rays parked per batch (0 at the batch's start)
+    const uint32_t* idx = nullptr;    // second launch: the record of each item (null: first launch)
+    const uint32_t* total = nullptr;  // second launch: the number of items
+    uint32_t at = 0;                  // first launch: park at this many marching rays or fewer (0: never)
+};
+
 template <int MODE, bool LDS>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMarchWaves))) void k_visibility(SceneView Sg, LightsView Lg, LdsPlan P,
                                                        const float4* __restrict__ spos,
@@ -280,19 +300,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMarchWa
                                                        const uint32_t* __restrict__ ckey, uint32_t shift,
                                                        const uint32_t* __restrict__ crow, uint32_t M, float off,
                                                        float tmin, float* vis_out, uint32_t* queue, uint32_t chunk,
-                                                       uint32_t min_idle, unsigned long long* evals,
+                                                       uint32_t min_idle, VisTail tl, unsigned long long* evals,
                                                        unsigned long long* ops) {
     IFX_SETUP(LDS, false, MODE);
     uint32_t acc = 0;
     bool live = false;
+    const bool resume = tl.idx != nullptr;
+    const uint32_t total = resume ? *tl.total : M;
     WaveQueue wq{0u, 0u, chunk, false, min_idle};
-    uint32_t slot = 0, wsteps = 0;
+    uint32_t slot = 0, wsteps = 0, batch = 0;
     float ff = 0.0f;
     ShadowMarch m;
     m.begin(spl(0.0f), spl(0.0f), 0.0f, 0.0f, 0.0f);
     for (;;) {
-        uint32_t q;
-        if (refill(live, wq, queue, M, q)) {
+        uint32_t q = 0;
+        const bool got = refill(live, wq, queue, total, q);
+        if (got && resume) {
+            const float4* rec = tl.st + 4ull * tl.idx[q];
+            const float4 a = rec[0], b = rec[1], c = rec[2], d = rec[3];
+            m.ro = ld3(a), m.t = a.w;
+            m.rd = ld3(b), m.tmax = b.w;
+            m.k = c.x, m.ph = c.y, m.res = c.z, ff = c.w;
+            m.steps = fbits(d.x), m.hint = fbits(d.y), slot = fbits(d.z);
+            live = true;
+        } else if (got) {
             slot = q;
             const uint32_t i = crow[q], c = ckey[q] >> shift;
             f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
@@ -304,6 +335,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMarchWa
             m.begin(sp + sn * off, dir, tmin, r - CN.w * 0.75f, kk);
             live = true;
         }
+        if (tl.at && __ballot(got) != 0ull) {  // a new batch: lane 0 holds its first item
+            batch = (uint32_t)__builtin_amdgcn_readlane((int)q, 0) >> 6;
+            if (lane_id() == 0) tl.cnt[batch] = 0u;
+        }
         if (__ballot(live) == 0) break;
         if constexpr (MODE == 2) ++wsteps;
         float vis = 0.0f;
@@ -311,6 +346,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMarchWa
             vis_out[slot] = vis <= 1e-3f ? 0.0f : ff * vis;
             acc += m.steps;
             live = false;
+        }
+        if (tl.at) {
+            const uint64_t lm = __ballot(live);
+            const uint32_t na = (uint32_t)__popcll(lm);
+            if (na != 0u && na <= tl.at) {
+                if (live) {
+                    float4* rec = tl.st + 4ull * (batch * tl.at + lanes_below(lm));
+                    rec[0] = make_float4(m.ro.x, m.ro.y, m.ro.z, m.t);
+                    rec[1] = make_float4(m.rd.x, m.rd.y, m.rd.z, m.tmax);
+                    rec[2] = make_float4(m.k, m.ph, m.res, ff);
+                    rec[3] = make_float4(bitsf(m.steps), bitsf(m.hint), bitsf(slot), 0.0f);
+                    live = false;
+                }
+                if (lane_id() == 0) tl.cnt[batch] = na;
+            }
         }
     }
     count_steps(evals, acc);
@@ -335,6 +385,16 @@ __global__ void k_vis_scatter(uint32_t M, const uint32_t* __restrict__ order, co
                               float* cand_v) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < M) cand_v[order[q]] = vis[q];
+}
+
+// The parked rays' records in batch order: batch b's cnt[b] records from
+// position start[b] (the exclusive sum of the counts).
+__global__ void k_tail_index(uint32_t nb, uint32_t at, const uint32_t* __restrict__ cnt,
+                             const uint32_t* __restrict__ start, uint32_t* idx) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)nb * at) return;
+    const uint32_t b = (uint32_t)(t / at), r = (uint32_t)(t % at);
+    if (r < cnt[b]) idx[start[b] + r] = (uint32_t)t;
 }
 
 // Hemisphere closure (src/trace.rs:165-186), persistent like k_visibility.
@@ -1314,15 +1374,49 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
                                                                     d_mrank.as<uint32_t>(), vrow);
             });
         }
+        // Parked tails need whole-wavefront batches starting at multiples of 64.
+        const uint32_t min_idle = std::min(64u, std::max(1u, vis_min_idle));
+        const uint32_t at = min_idle == 64u && M >= vis_tail_items ? std::min(vis_tail, 32u) : 0u;
+        uint32_t chunk = vis_chunk ? vis_chunk : wave_chunk(M, visb);
+        const uint32_t nb = (M + 63u) / 64u;
+        VisTail tl{};
+        if (at) {
+            chunk = std::max(64u, chunk & ~63u);
+            CK(d_tail_st.ensure(64ull * nb * at));
+            CK(d_tail_cnt.ensure(4ull * (nb + 1)));
+            CK(d_tail_start.ensure(4ull * (nb + 1)));
+            CK(d_tail_idx.ensure(4ull * nb * at));
+            CK(hipMemsetAsync(d_tail_cnt.as<uint32_t>() + nb, 0, 4, stream));
+            tl.st = d_tail_st.as<float4>();
+            tl.cnt = d_tail_cnt.as<uint32_t>();
+            tl.at = at;
+        }
+        int rc_t = 0;
+        // both launches (and the parked rays' index between them) time as one
         launch("visibility", [&] {
             LAUNCH_CM(k_visibility, tally_mode(), ld.staged, visb, ld.bytes, S, Lg, ld.plan,
                       sort_cand ? d_mpos.as<float4>() : spos, sort_cand ? d_mnrm.as<float4>() : snrm,
                       d_cpos.as<float4>(), d_cnrm.as<float4>(),
                       sort_cand ? d_vkey_s.as<uint32_t>() : d_cand_ci.as<uint32_t>(), sort_cand ? mbits : 0u,
                       sort_cand ? vrow : d_cand_row.as<uint32_t>(), M, diag * 5e-4f, diag * 1e-3f, vis, queue,
-                      vis_chunk ? vis_chunk : wave_chunk(M, visb), std::min(64u, std::max(1u, vis_min_idle)), ctr,
-                      tally() ? ctr + 1 : nullptr);
+                      chunk, min_idle, tl, ctr, tally() ? ctr + 1 : nullptr);
+            if (!at) return;
+            rc_t = scan_u32(d_tail_cnt.as<uint32_t>(), d_tail_start.as<uint32_t>(), nb + 1, err);
+            if (rc_t) return;
+            k_tail_index<<<blocks_for((uint64_t)nb * at, 256), 256, 0, stream>>>(
+                nb, at, d_tail_cnt.as<uint32_t>(), d_tail_start.as<uint32_t>(), d_tail_idx.as<uint32_t>());
+            VisTail rt{};
+            rt.st = d_tail_st.as<float4>();
+            rt.idx = d_tail_idx.as<uint32_t>();
+            rt.total = d_tail_start.as<uint32_t>() + nb;
+            LAUNCH_CM(k_visibility, tally_mode(), ld.staged, visb, ld.bytes, S, Lg, ld.plan,
+                      sort_cand ? d_mpos.as<float4>() : spos, sort_cand ? d_mnrm.as<float4>() : snrm,
+                      d_cpos.as<float4>(), d_cnrm.as<float4>(),
+                      sort_cand ? d_vkey_s.as<uint32_t>() : d_cand_ci.as<uint32_t>(), sort_cand ? mbits : 0u,
+                      sort_cand ? vrow : d_cand_row.as<uint32_t>(), M, diag * 5e-4f, diag * 1e-3f, vis, queue + 2,
+                      64u, 64u, rt, ctr, tally() ? ctr + 1 : nullptr);
         });
+        if (rc_t) return rc_t;
         if (sort_cand) {
             launch("vis_scatter", [&] {
                 k_vis_scatter<<<blocks_for(M, 256), 256, 0, stream>>>(M, d_vorder.as<uint32_t>(), vis,
@@ -1684,7 +1778,7 @@ int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool 
         LAUNCH_CM(k_visibility, 0, ld.staged, visb, ld.bytes, S, Lg, ld.plan, spos, snrm, d_cpos.as<float4>(),
                   d_cnrm.as<float4>(), d_tci.as<uint32_t>(), 0u, d_trow.as<uint32_t>(), M, diag * 5e-4f,
                   diag * 1e-3f, d_tv.as<float>(), d_tqueue.as<uint32_t>(), wave_chunk(M, visb),
-                  std::min(64u, std::max(1u, vis_min_idle)), ctr + 20, nullptr);
+                  std::min(64u, std::max(1u, vis_min_idle)), VisTail{}, ctr + 20, nullptr);
     }
     CK(hipMemsetAsync(d_tkeep.as<uint32_t>() + nr, 0, 4, stream));
     k_keep_count<<<blocks_for(nr, 256), 256, 0, stream>>>(nr, d_tstart.as<uint32_t>(), d_tv.as<float>(),

@@ -492,7 +492,8 @@ def test_launch_config_invariance(name):
     """Thread-count invariance (ci.yml:81-93) restated for launch shapes: LDS
     staging on/off, visibility / cull / Jacobi grid sizes, one stream or
     two, the march kernels' refill policy and queue chunks, the flat object
-    list or the BVH walk in shared queries, and the instrumented counting
+    list or the BVH walk in shared queries, parked link-visibility tails
+    (at several thresholds and queue chunks), and the instrumented counting
     kernels all give the same bytes."""
     say("launch configs", name)
     src = scenes.stress(surfels=4000) if name == "stress seeded" else scene_text(name)
@@ -504,15 +505,19 @@ def test_launch_config_invariance(name):
                      ("jacobi_rows", 1), ("jacobi_rows", 2), ("jacobi_rows", 3), ("side_stream", 1),
                      ("flat_query", 0), ("vis_min_idle", 1), ("closure_min_idle", 1), ("vis_chunk", 256),
                      ("flat_query", 1), ("vis_min_idle", 24), ("closure_min_idle", 40), ("vis_chunk", 64),
-                     ("vis_min_idle", 64), ("closure_min_idle", 64), ("vis_chunk", 0), ("vis_blocks", 0)]:
+                     ("vis_min_idle", 64), ("closure_min_idle", 64), ("vis_chunk", 0), ("vis_blocks", 0),
+                     ("vis_tail_items", 0), ("vis_tail", 1), ("vis_chunk", 100), ("vis_tail", 32), ("vis_chunk", 0),
+                     ("vis_tail", 8)]:
         ctx.set_tuning(key, val)
         got = _frame_bytes(ctx)
-        assert all(np.array_equal(a, b) for a, b in zip(base, got)), key
-    ctx.set_counting(True)
-    got = _frame_bytes(ctx)
-    ctx.set_counting(False)
-    assert all(np.array_equal(a, b) for a, b in zip(base, got)), "counting"
+        assert all(np.array_equal(a, b) for a, b in zip(base, got)), (key, val)
+    for mode in (1, 2):  # counting instantiations, with parked tails
+        ctx.set_counting(mode)
+        got = _frame_bytes(ctx)
+        ctx.set_counting(0)
+        assert all(np.array_equal(a, b) for a, b in zip(base, got)), ("counting", mode)
     assert ctx.kernel_ops("shade") > 0 and ctx.kernel_ops("visibility") > 0
+    ctx.set_tuning("vis_tail_items", 1 << 23)
 
 
 def test_repeat_runs_are_bit_identical():

@@ -112,11 +112,13 @@ def test_tuning_keys_and_ranges():
     ctx = ia.Context(-1)
     for key, v in (("link_budget", 1 << 40), ("link_budget", 0), ("adj_tile_rows", 512), ("adj_seg", 256),
                    ("side_stream", 0), ("jacobi_rows", 3), ("vis_min_idle", 1), ("vis_min_idle", 64),
-                   ("closure_min_idle", 32), ("vis_chunk", 128), ("flat_query", 0), ("flat_query", 1)):
+                   ("closure_min_idle", 32), ("vis_chunk", 128), ("flat_query", 0), ("flat_query", 1),
+                   ("vis_tail", 0), ("vis_tail", 32), ("vis_tail_items", 0), ("vis_tail_items", 1 << 30)):
         ctx.set_tuning(key, v)
     for key, v in (("link_budget", -1), ("link_budget", (1 << 50) + 1), ("adj_tile_rows", (1 << 30) + 1),
                    ("spp", 0), ("no_such_key", 1), ("vis_min_idle", 0), ("vis_min_idle", 65),
-                   ("closure_min_idle", 0), ("closure_min_idle", 65)):
+                   ("closure_min_idle", 0), ("closure_min_idle", 65), ("vis_tail", -1), ("vis_tail", 33),
+                   ("vis_tail_items", -1), ("vis_tail_items", (1 << 30) + 1)):
         with pytest.raises(ia.IfxError) as e:
             ctx.set_tuning(key, v)
         assert e.value.status == -1
