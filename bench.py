@@ -68,9 +68,11 @@ def source_hash():
 
 
 def profiled_traffic(workload):
-    """HBM bytes per dispatch from profiles/<workload>_hbm_traffic_pmc.md
-    (lines `traffic <kernel> <bytes>` under `source <hash>`), or {} when the
-    profile is missing or was taken on other sources."""
+    """HBM bytes per timed launch group from
+    profiles/<workload>_hbm_traffic_pmc.md (lines `traffic <kernel> <bytes>`
+    under `source <hash>`: the link visibility per transfer, both of its
+    launches; the Jacobi pair per bounce), or {} when the profile is missing
+    or was taken on other sources."""
     path = os.path.join(ROOT, "profiles", "%s_hbm_traffic_pmc.md" % workload)
     if not os.path.exists(path):
         return {}
@@ -410,7 +412,8 @@ def measure(name, steps, warmup, with_cpu, prof_frames=2):
         measured = traffic.get(dominant)
         roof = {"kernel": dominant, "bound": "valu", "achieved": tfl, "peak": PEAK_VALU_TFLOPS, "unit": "TFLOP/s",
                 "frac": tfl / PEAK_VALU_TFLOPS, "traffic": measured["bytes"] if measured else None,
-                "traffic_unit": "bytes per dispatch (rocprofv3 PMC)",
+                "traffic_unit": "bytes per timed launch group (rocprofv3 PMC; the link visibility's group is "
+                                "its first launch plus the parked tails' launch)",
                 "traffic_profile": measured["profile"] if measured else None, "ops_per_launch": ops[dominant]}
     elif dominant == "jacobi":
         roof = jac_roof
