@@ -274,8 +274,10 @@ class Engine {
     void join_side();  // the main stream waits for side_'s work
     hipStream_t side_ = nullptr;
     hipEvent_t fork_ = nullptr, join_ = nullptr, closed_ = nullptr;  // closed_: the closure kernel is done
+    hipEvent_t vis_done_ = nullptr;  // the link visibility launch is done (deferred direct-light terms wait)
     bool direct_pre_ = false;    // terms from the last transfer are pending on side_ and still valid
     bool side_pending_ = false;  // side_ holds work the main stream has not waited for
+    uint64_t last_candidates_ = 0;  // candidates of the last transfer (schedules the side stream)
     DevBuf d_dctr;
 
     unsigned char* scratch_ = nullptr;  // device address of the static scratch array on device_
@@ -309,6 +311,7 @@ class Engine {
     // work order (Morton rows, cluster-major candidates) and queues
     DevBuf d_mpos, d_mnrm, d_mrank;  // surfel positions / normals in Morton order, each surfel's rank
     DevBuf d_morton, d_mval, d_mkey_s, d_rorder, d_vkey, d_vval, d_vkey_s, d_vorder, d_queue, d_open;
+    DevBuf d_mcount, d_mstart;  // candidate counts / offsets of the rows in Morton order
     // candidates and links
     DevBuf d_ccount, d_cstart, d_cand_ci, d_cand_row, d_cand_v, d_sky, d_lcount, d_lstart, d_scale, d_rowsum, d_idx, d_w;
     // solve

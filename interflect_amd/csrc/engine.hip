@@ -180,14 +180,19 @@ __global__ void k_cluster_compact(uint32_t nseg, const uint32_t* valid, const ui
 // keeps survivors in ascending cluster order.  EMIT=false counts, EMIT=true
 // writes candidates at the row's scanned offset.  With `masks`, the counting
 // pass stores each row's ballots (ceil(C/64) words per row) and the emitting
-// pass reads them instead of repeating the culls.
+// pass reads them instead of repeating the culls.  With `key`, the emitting
+// pass also writes the visiting-order sort's input with the rows in Morton
+// order (row i's candidates from mstart[mrank[i]]): key = cluster, value =
+// the candidate's row-major slot, so a stable sort by cluster alone leaves
+// each cluster's candidates in Morton order of their rows.
 template <bool EMIT>
 __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, const float4* __restrict__ snrm,
                                                uint32_t row0, uint32_t n, const float4* __restrict__ cpos,
                                                const float4* __restrict__ cnrm, uint32_t C, uint32_t* count,
                                                const uint32_t* __restrict__ start, uint32_t* cand_ci,
-                                               uint32_t* cand_row, const uint32_t* __restrict__ morton,
-                                               uint32_t mbits, uint32_t* key, uint32_t* val, uint64_t* masks) {
+                                               uint32_t* cand_row, const uint32_t* __restrict__ mrank,
+                                               const uint32_t* __restrict__ mstart, uint32_t* key, uint32_t* val,
+                                               uint64_t* masks) {
     uint32_t lane = lane_id();
     uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     uint32_t nwaves = (gridDim.x * kBlock) >> 6;
@@ -196,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
         const uint32_t i = row0 + j;
         f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
         uint32_t base = EMIT ? start[j] : 0u;
-        uint32_t mc = (EMIT && key) ? morton[i] : 0u;
+        const uint32_t mbase = (EMIT && key) ? mstart[mrank[i]] : 0u;
         uint32_t run = 0;
         for (uint32_t c0 = 0; c0 < C; c0 += 64) {
             uint32_t c = c0 + lane;
@@ -215,18 +220,28 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
             }
             const bool pass = (m >> lane) & 1ull;
             if (EMIT && pass) {
-                uint32_t o = base + run + lanes_below(m);
+                const uint32_t r = run + lanes_below(m);
+                const uint32_t o = base + r;
                 cand_ci[o] = c;
                 cand_row[o] = i;
                 if (key) {
-                    key[o] = (c << mbits) | (mc >> (kMortonBits - mbits));
-                    val[o] = o;
+                    key[mbase + r] = c;
+                    val[mbase + r] = o;
                 }
             }
             run += (uint32_t)__popcll(m);
         }
         if (!EMIT && lane == 0) count[j] = run;
     }
+}
+
+// Candidate counts of the rows in Morton order (rorder[r]: the r-th row in
+// Morton order); their exclusive sum places each row's candidates in the
+// visiting-order sort's input.
+__global__ void k_morton_counts(uint32_t N, const uint32_t* __restrict__ rorder, const uint32_t* __restrict__ count,
+                                uint32_t* mcount) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < N) mcount[r] = count[rorder[r]];
 }
 
 __global__ void k_morton(const float4* __restrict__ spos, uint32_t N, f3 lo, f3 scale, uint32_t* code,
@@ -881,6 +896,7 @@ Engine::~Engine() {
         if (fork_) (void)hipEventDestroy(fork_);
         if (join_) (void)hipEventDestroy(join_);
         if (closed_) (void)hipEventDestroy(closed_);
+        if (vis_done_) (void)hipEventDestroy(vis_done_);
         if (side_) (void)hipStreamDestroy(side_);
         (void)hipStreamDestroy(stream);
     }
@@ -904,6 +920,7 @@ int Engine::init(int device, std::string& err) {
     CK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
     CK(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
     CK(hipEventCreateWithFlags(&closed_, hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&vis_done_, hipEventDisableTiming));
     {
         int cus = 0;
         CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -1203,25 +1220,29 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     // candidates: 343 against 313 ms with the terms beside it, 345 with the
     // closure).  A short one (up to kVisOverlapMax candidates) runs beside
     // the side work, which then fills its tail (product scene, 1.4M
-    // candidates: frame 4.0 against 4.7 ms).
+    // candidates: frame 4.0 against 4.7 ms).  When the last transfer had a
+    // long one, the direct-light terms wait for this transfer's link
+    // visibility and run beside the link scan and compaction instead, so the
+    // link visibility waits for the closure alone (stress scene: it starts
+    // once the candidate sort is done).
     CK(d_sky.ensure(4ull * N));
     CK(d_open.ensure(4ull * N));
     CK(hipMemsetAsync(d_open.p, 0, 4ull * N, stream));
     const bool side = overlap_direct && !tally();
     const bool pre = side && n_lights(dev_lights()) > 0;
+    const bool defer = pre && last_candidates_ > kVisOverlapMax;
     if (pre) {
         CK(d_dctr.ensure(64));
         CK(hipMemsetAsync(d_dctr.p, 0, 8, stream));
     }
-    if (side) {
-        CK(hipEventRecord(fork_, stream));
-        CK(hipStreamWaitEvent(side_, fork_, 0));
-        const int rc = pre ? launch_direct(side_, d_dctr.as<unsigned long long>(), nullptr, err) : 0;
-        side_pending_ = true;
-        direct_pre_ = pre && rc == 0;
-        if (rc) return rc;
-    }
-    {
+    // The closure rays on the side stream (or this one), after the work
+    // this stream has queued so far.
+    auto closure_rays = [&]() -> int {
+        if (side) {
+            CK(hipEventRecord(fork_, stream));
+            CK(hipStreamWaitEvent(side_, fork_, 0));
+            side_pending_ = true;
+        }
         hipStream_t cs = side ? side_ : stream;
         const Lds ld = plan_lds(false, false);
         const LightsView Lg{};
@@ -1234,11 +1255,25 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
                       wave_chunk(16ull * N, grid), std::min(64u, std::max(1u, closure_min_idle)), ctr,
                       tally() ? ctr + 2 : nullptr);
         });
+        if (side) {
+            CK(hipEventRecord(closed_, side_));
+            CK(hipEventRecord(join_, side_));
+        }
+        return 0;
+    };
+    if (side && !defer) {
+        CK(hipEventRecord(fork_, stream));
+        CK(hipStreamWaitEvent(side_, fork_, 0));
+        const int rc = pre ? launch_direct(side_, d_dctr.as<unsigned long long>(), nullptr, err) : 0;
+        side_pending_ = true;
+        direct_pre_ = pre && rc == 0;
+        if (rc) return rc;
     }
-    if (side) {
-        CK(hipEventRecord(closed_, side_));
-        CK(hipEventRecord(join_, side_));
-    }
+    // Deferred terms: the closure waits for the candidate count below, so
+    // this stream's cluster build and culls do not queue behind it (the
+    // persistent closure holds every CU until it ends).
+    if (!defer)
+        if (int rc = closure_rays()) return rc;
 
     // ---- clusters: key, stable sort, segments, serial sums, compaction
     const f3 lo = scene.lo, ext = scene.hi - scene.lo;
@@ -1314,9 +1349,11 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     const uint32_t ffb = ff_blocks ? ff_blocks : std::min<uint32_t>(blocks_for(N, 4), 8192);
     launch("ff_count", [&] {
         k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, 0u, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
-                                                 d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, 0u,
-                                                 nullptr, nullptr, d_ffmask.as<uint64_t>());
+                                                 d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr,
+                                                 nullptr, nullptr, nullptr, d_ffmask.as<uint64_t>());
     });
+    if (defer)
+        if (int rc = closure_rays()) return rc;
     if (int rc_s = scan_u32(d_ccount.as<uint32_t>(), d_cstart.as<uint32_t>(), (N + 1), err))
         return rc_s;
     uint32_t M = 0;
@@ -1328,10 +1365,12 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         return -6;
     }
     // Cluster-major, Morton-minor visiting order of the candidates: the
-    // cluster above as many Morton bits of the surfel's cell as fit in 32.
+    // emission lays the sort's input out with the rows in Morton order, so
+    // a stable sort by cluster alone (2 passes of 8 bits up to 65536
+    // clusters) gives it.
     const uint32_t cbits = bits_for(C ? C - 1 : 0);
-    const uint32_t mbits = cbits < 32u ? std::min(kMortonBits, 32u - cbits) : 0u;
-    const uint32_t vkey_bits = cbits + mbits;
+    const uint32_t mbits = 0u;  // the sorted keys hold the cluster alone
+    const uint32_t vkey_bits = cbits;
     const bool sort_cand = cbits <= 17u && M > 0;
     CK(d_cand_ci.ensure(4ull * M));
     CK(d_cand_row.ensure(4ull * M));
@@ -1341,13 +1380,19 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         CK(d_vval.ensure(4ull * M));
         CK(d_vkey_s.ensure(4ull * M));
         CK(d_vorder.ensure(4ull * M));
+        CK(d_mcount.ensure(4ull * (N + 1)));
+        CK(d_mstart.ensure(4ull * (N + 1)));
+        CK(hipMemsetAsync(d_mcount.as<uint32_t>() + N, 0, 4, stream));
+        k_morton_counts<<<blocks_for(N, 256), 256, 0, stream>>>(N, d_rorder.as<uint32_t>(), d_ccount.as<uint32_t>(),
+                                                                d_mcount.as<uint32_t>());
+        if (int rc_s = scan_u32(d_mcount.as<uint32_t>(), d_mstart.as<uint32_t>(), N + 1, err)) return rc_s;
     }
     launch("ff_emit", [&] {
         k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, 0u, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
                                                 nullptr,
                                                 d_cstart.as<uint32_t>(), d_cand_ci.as<uint32_t>(),
-                                                d_cand_row.as<uint32_t>(), d_morton.as<uint32_t>(), mbits,
-                                                sort_cand ? d_vkey.as<uint32_t>() : nullptr,
+                                                d_cand_row.as<uint32_t>(), d_mrank.as<uint32_t>(),
+                                                d_mstart.as<uint32_t>(), sort_cand ? d_vkey.as<uint32_t>() : nullptr,
                                                 sort_cand ? d_vval.as<uint32_t>() : nullptr,
                                                 d_ffmask.as<uint64_t>());
     });
@@ -1358,7 +1403,9 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     }
 
     // ---- link visibility, in visiting order (the unsorted fallback visits row-major)
-    if (side && M > kVisOverlapMax) CK(hipStreamWaitEvent(stream, join_, 0));  // the side stream is done
+    last_candidates_ = M;
+    // the side stream is done (the closure alone when the terms are deferred)
+    if (side && M > kVisOverlapMax) CK(hipStreamWaitEvent(stream, defer ? closed_ : join_, 0));
     if (M) {
         const uint32_t visb =
             vis_blocks ? vis_blocks : std::max(1u, std::min<uint32_t>(persist, blocks_for(M, kBlock)));
@@ -1423,6 +1470,14 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
                                                                       d_cand_v.as<float>());
             });
         }
+    }
+    if (defer) {  // the deferred direct-light terms, beside the link scan and compaction
+        CK(hipEventRecord(vis_done_, stream));
+        CK(hipStreamWaitEvent(side_, vis_done_, 0));
+        const int rc = launch_direct(side_, d_dctr.as<unsigned long long>(), nullptr, err);
+        if (rc) return rc;
+        direct_pre_ = true;
+        CK(hipEventRecord(join_, side_));
     }
 
     // ---- calibration, link scan, compaction
@@ -1754,8 +1809,8 @@ int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool 
     const uint32_t ffb = ff_blocks ? ff_blocks : std::min<uint32_t>(blocks_for(nr, 4), 8192);
     CK(hipMemsetAsync(d_tcount.as<uint32_t>() + nr, 0, 4, stream));
     k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, r0, nr, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
-                                             d_tcount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, 0u, nullptr,
-                                             nullptr, nullptr);
+                                             d_tcount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                                             nullptr, nullptr, nullptr);
     if (int rc = scan_u32(d_tcount.as<uint32_t>(), d_tstart.as<uint32_t>(), nr + 1, err)) return rc;
     if (!sized) {
         CK(hipMemcpyAsync(&M, d_tstart.as<uint32_t>() + nr, 4, hipMemcpyDeviceToHost, stream));
@@ -1766,7 +1821,7 @@ int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool 
     }
     k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, r0, nr, d_cpos.as<float4>(), d_cnrm.as<float4>(), C, nullptr,
                                             d_tstart.as<uint32_t>(), d_tci.as<uint32_t>(), d_trow.as<uint32_t>(),
-                                            nullptr, 0u, nullptr, nullptr, nullptr);
+                                            nullptr, nullptr, nullptr, nullptr, nullptr);
     if (M) {
         const uint32_t persist = (uint32_t)n_cu_ * kMarchWaves;  // resident blocks of the march kernels
         const uint32_t visb = vis_blocks ? vis_blocks : std::max(1u, std::min<uint32_t>(persist, blocks_for(M, kBlock)));
