@@ -1269,9 +1269,9 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         direct_pre_ = pre && rc == 0;
         if (rc) return rc;
     }
-    // Deferred terms: the closure waits for the candidate count below, so
-    // this stream's cluster build and culls do not queue behind it (the
-    // persistent closure holds every CU until it ends).
+    // Deferred terms: the closure starts once the candidate count is read
+    // below, so this stream's cluster build, culls and that read do not
+    // queue behind it (the persistent closure holds every CU until it ends).
     if (!defer)
         if (int rc = closure_rays()) return rc;
 
@@ -1352,13 +1352,13 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
                                                  d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr,
                                                  nullptr, nullptr, nullptr, d_ffmask.as<uint64_t>());
     });
-    if (defer)
-        if (int rc = closure_rays()) return rc;
     if (int rc_s = scan_u32(d_ccount.as<uint32_t>(), d_cstart.as<uint32_t>(), (N + 1), err))
         return rc_s;
     uint32_t M = 0;
     CK(hipMemcpyAsync(&M, d_cstart.as<uint32_t>() + N, 4, hipMemcpyDeviceToHost, stream));
     CK(hipStreamSynchronize(stream));
+    if (defer)
+        if (int rc = closure_rays()) return rc;
     out.candidates = M;
     if (M >= (1u << 31)) {
         err = "candidate count exceeds 2^31";
