@@ -761,17 +761,55 @@ __global__ void k_sell_widths(uint32_t N, const uint32_t* __restrict__ lstart, c
     if ((q & 63u) == 0 && q < N) width[q >> 6] = len;
 }
 
-__global__ void k_sell_fill(uint32_t N, const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ perm,
-                            const uint32_t* __restrict__ idx, const float* __restrict__ w,
-                            const uint32_t* __restrict__ sbase, uint32_t* sidx, float* sw) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= N) return;
-    const uint32_t i = perm[q];
-    const uint32_t s0 = lstart[i], len = lstart[i + 1] - s0;
-    const uint64_t b = 64ull * sbase[q >> 6] + (q & 63u);
-    for (uint32_t k = 0; k < len; ++k) {
-        sidx[b + 64ull * k] = idx[s0 + k];
-        sw[b + 64ull * k] = w[s0 + k];
+// One 256-thread block per 64-row slice: the slice's links pass through an
+// LDS tile of 64 rows x 64 links, read along each row (wavefront w reads
+// rows 16w..16w+15, a row's 64 links per load) and written along each
+// column (wavefront w writes columns 16w..16w+15, one row per lane), so both
+// sides are coalesced.  Row r of slice s (sorted position q = 64 s + r)
+// keeps link k at 64 sbase[s] + 64 k + r; padding cells are not written.
+constexpr uint32_t kSellBlock = 256;
+__global__ __launch_bounds__(kSellBlock) void k_sell_fill(uint32_t N, const uint32_t* __restrict__ lstart,
+                                                          const uint32_t* __restrict__ perm,
+                                                          const uint32_t* __restrict__ idx,
+                                                          const float* __restrict__ w,
+                                                          const uint32_t* __restrict__ sbase, uint32_t* sidx,
+                                                          float* sw) {
+    __shared__ uint32_t ti[64][65];
+    __shared__ float tw[64][65];
+    __shared__ uint32_t rs0[64], rlen[64];
+    const uint32_t s = blockIdx.x, t = threadIdx.x, lane = t & 63u, r0 = (t >> 6) * 16u;
+    if (t < 64u) {
+        const uint32_t q = 64u * s + t;
+        uint32_t s0 = 0, len = 0;
+        if (q < N) {
+            const uint32_t i = perm[q];
+            s0 = lstart[i];
+            len = lstart[i + 1] - s0;
+        }
+        rs0[t] = s0;
+        rlen[t] = len;
+    }
+    __syncthreads();
+    const uint32_t width = sbase[s + 1] - sbase[s];  // the slice's longest row
+    const uint64_t base = 64ull * sbase[s];
+    const uint32_t my_len = rlen[lane];
+    for (uint32_t k0 = 0; k0 < width; k0 += 64u) {
+        for (uint32_t r = r0; r < r0 + 16u; ++r) {
+            const uint32_t k = k0 + lane;
+            if (k < rlen[r]) {
+                ti[r][lane] = idx[rs0[r] + k];
+                tw[r][lane] = w[rs0[r] + k];
+            }
+        }
+        __syncthreads();
+        for (uint32_t c = r0; c < r0 + 16u; ++c) {
+            const uint32_t k = k0 + c;
+            if (k < my_len) {
+                sidx[base + 64ull * k + lane] = ti[lane][c];
+                sw[base + 64ull * k + lane] = tw[lane][c];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1884,7 +1922,7 @@ int Engine::build_sliced(std::string& err) {
     CK(d_sidx.ensure(256ull * cols + 4));
     CK(d_sw.ensure(256ull * cols + 4));
     if (N)
-        k_sell_fill<<<blocks_for(N, 256), 256, 0, stream>>>(N, d_lstart.as<uint32_t>(), d_sperm.as<uint32_t>(),
+        k_sell_fill<<<ns, kSellBlock, 0, stream>>>(N, d_lstart.as<uint32_t>(), d_sperm.as<uint32_t>(),
                                                             d_idx.as<uint32_t>(), d_w.as<float>(),
                                                             d_sbase.as<uint32_t>(), d_sidx.as<uint32_t>(),
                                                             d_sw.as<float>());
