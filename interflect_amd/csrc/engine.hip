@@ -467,21 +467,32 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMarchWa
 __global__ void k_row_sum(uint32_t N, const uint32_t* __restrict__ cstart, const float* __restrict__ cand_v,
                           const uint32_t* __restrict__ open, float* sky_out, uint32_t* lcount, float* scale,
                           float* row_sum) {
-    // one wavefront per row: its candidates are read coalesced, and the kept
-    // values are added in candidate order (readlane), the serial sum
+    // one wavefront per row: its candidates are read coalesced, four
+    // 64-candidate chunks in flight at a time, and the kept values are added
+    // in candidate order (readlane), the serial sum.  Every lane's value is
+    // added, a dropped candidate's as +0.0f: the sum starts at +0.0f and
+    // only positive values join it, so adding +0.0f leaves its bits as they
+    // are, and the fixed lane sequence unrolls without per-value branches.
     const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = lane_id();
     if (i >= N) return;
     float sum = 0.0f;
     uint32_t cnt = 0;
-    for (uint32_t b = cstart[i], e = cstart[i + 1]; b < e; b += 64u) {
-        const uint32_t k = b + lane;
-        const float v = k < e ? cand_v[k] : 0.0f;
-        uint64_t m = __ballot(k < e && v > 0.0f);
-        cnt += (uint32_t)__popcll(m);
-        while (m) {
-            const uint32_t j = (uint32_t)__builtin_ctzll(m);
-            sum += bitsf(__builtin_amdgcn_readlane(fbits(v), j));
-            m &= m - 1ull;
+    for (uint32_t b = cstart[i], e = cstart[i + 1]; b < e; b += 256u) {
+        float v[4];
+#pragma unroll
+        for (uint32_t c = 0; c < 4u; ++c) {
+            const uint32_t k = b + 64u * c + lane;
+            v[c] = k < e ? cand_v[k] : 0.0f;
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < 4u; ++c) {
+            const uint32_t k = b + 64u * c + lane;
+            if (b + 64u * c >= e) break;  // uniform: no candidates left in this row
+            const bool keep = k < e && v[c] > 0.0f;
+            cnt += (uint32_t)__popcll(__ballot(keep));
+            const int32_t x = keep ? (int32_t)fbits(v[c]) : 0;
+#pragma unroll
+            for (int32_t j = 0; j < 64; ++j) sum += bitsf((uint32_t)__builtin_amdgcn_readlane(x, j));
         }
     }
     if (lane) return;
@@ -498,7 +509,8 @@ __global__ void k_row_sum(uint32_t N, const uint32_t* __restrict__ cstart, const
     row_sum[i] = cnt ? fmn(sum, 1.0f) : -1.0f;
 }
 
-// One wavefront per row: kept candidates in candidate order (ballot ranks).
+// One wavefront per row: kept candidates in candidate order (ballot ranks),
+// four 64-candidate chunks in flight at a time.
 __global__ void k_compact(uint32_t N, const uint32_t* __restrict__ cstart, const float* __restrict__ cand_v,
                           const uint32_t* __restrict__ cand_ci, const uint32_t* __restrict__ lstart,
                           const float* __restrict__ scale, uint32_t* idx, float* w) {
@@ -506,17 +518,25 @@ __global__ void k_compact(uint32_t N, const uint32_t* __restrict__ cstart, const
     if (i >= N) return;
     uint32_t pos = lstart[i];
     const float sc = scale[i];
-    for (uint32_t b = cstart[i], e = cstart[i + 1]; b < e; b += 64u) {
-        const uint32_t k = b + lane;
-        const float v = k < e ? cand_v[k] : 0.0f;
-        const bool keep = k < e && v > 0.0f;
-        const uint64_t m = __ballot(keep);
-        if (keep) {
-            const uint32_t at = pos + lanes_below(m);
-            idx[at] = cand_ci[k];
-            w[at] = v * sc;
+    for (uint32_t b = cstart[i], e = cstart[i + 1]; b < e; b += 256u) {
+        float v[4];
+#pragma unroll
+        for (uint32_t c = 0; c < 4u; ++c) {
+            const uint32_t k = b + 64u * c + lane;
+            v[c] = k < e ? cand_v[k] : 0.0f;
         }
-        pos += (uint32_t)__popcll(m);
+#pragma unroll
+        for (uint32_t c = 0; c < 4u; ++c) {
+            const uint32_t k = b + 64u * c + lane;
+            const bool keep = k < e && v[c] > 0.0f;
+            const uint64_t m = __ballot(keep);
+            if (keep) {
+                const uint32_t at = pos + lanes_below(m);
+                idx[at] = cand_ci[k];
+                w[at] = v[c] * sc;
+            }
+            pos += (uint32_t)__popcll(m);
+        }
     }
 }
 
