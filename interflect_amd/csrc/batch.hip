@@ -389,8 +389,10 @@ __global__ __launch_bounds__(kBlock) void kb_jacobi_agg(BJacobi A, uint32_t it) 
                 uint32_t m = A.members[k];
                 v = ld3(A.rad[m]) * A.spos[m].w;
             }
-            uint32_t cnt = r.y - b < 64u ? r.y - b : 64u;
-            for (uint32_t j = 0; j < cnt; ++j) {
+            // every lane in order; lanes past the cluster's end add +0.0f,
+            // which leaves acc's bits as they are (acc is never -0.0f)
+#pragma unroll
+            for (int32_t j = 0; j < 64; ++j) {
                 f3 vj = mk(bitsf(__builtin_amdgcn_readlane(fbits(v.x), j)),
                            bitsf(__builtin_amdgcn_readlane(fbits(v.y), j)),
                            bitsf(__builtin_amdgcn_readlane(fbits(v.z), j)));
