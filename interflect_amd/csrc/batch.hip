@@ -128,11 +128,7 @@ __global__ __launch_bounds__(kBlock) void kb_ff(const float4* __restrict__ spos,
         for (uint32_t cb = c0; cb < c1; cb += 64) {
             uint32_t c = cb + lane;
             bool pass = false;
-            if (c < c1) {
-                float r, ff;
-                f3 dir;
-                pass = ff_geom(sp, sn, cpos[c], cnrm[c], r, dir, ff);
-            }
+            if (c < c1) pass = ff_pass(sp, sn, cpos[c], cnrm[c]);
             uint64_t m = __ballot(pass);
             if (EMIT && pass) {
                 uint32_t o = base + run + lanes_below(m);

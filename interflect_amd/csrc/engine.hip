@@ -210,11 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
                 m = masks[(size_t)j * groups + (c0 >> 6)];
             } else {
                 bool pass = false;
-                if (c < C) {
-                    float r, ff;
-                    f3 dir;
-                    pass = ff_geom(sp, sn, cpos[c], cnrm[c], r, dir, ff);
-                }
+                if (c < C) pass = ff_pass(sp, sn, cpos[c], cnrm[c]);
                 m = __ballot(pass);
                 if (!EMIT && masks && lane == 0) masks[(size_t)j * groups + (c0 >> 6)] = m;
             }

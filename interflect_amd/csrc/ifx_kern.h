@@ -666,6 +666,38 @@ __device__ __forceinline__ bool ff_geom(f3 sp, f3 sn, float4 cp, float4 cn, floa
     return !(ff < 2e-5f);
 }
 
+// ff_geom's decision alone, for the candidate culls.  The cosines and the
+// form factor are first estimated with the hardware square root and
+// reciprocal (a few ulps) instead of the correctly rounded operations:
+// with unit normals (cluster normals are normalised; a surfel normal whose
+// squared length is off 1 by more than 1e-4 takes ff_geom directly) each
+// estimated cosine lies within 1e-6 of ff_geom's
+// (rounding of the direction, the dot product and 1/r, each a few 1e-8
+// of |d| / r), so a cosine more than 1e-5 from the 1e-3 threshold, and a
+// form factor further from 2e-5 than its relative error bound (4e-6 over
+// each cosine plus 1e-5), decide as ff_geom does.  Near a threshold the
+// decision is ff_geom's own.
+__device__ __forceinline__ bool ff_pass(f3 sp, f3 sn, float4 cp, float4 cn) {
+    const f3 d = ld3(cp) - sp;
+    const float r2 = dot3(d, d);
+    if (r2 < 1e-9f) return false;
+    const float sn2 = dot3(sn, sn);
+    const bool unit = sn2 >= 0.9999f && sn2 <= 1.0001f;
+    const float ir = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(r2));
+    const float ci = dot3(sn, d) * ir, cc = -dot3(ld3(cn), d) * ir;
+    constexpr float kCosMargin = 1e-5f, kCosErr = 4e-6f;
+    if (unit && (ci <= 1e-3f - kCosMargin || cc <= 1e-3f - kCosMargin)) return false;
+    if (unit && ci > 1e-3f + kCosMargin && cc > 1e-3f + kCosMargin) {
+        const float ff = ((ci * cc) * cp.w) * __builtin_amdgcn_rcpf(kPi * r2 + cp.w);
+        const float rel = kCosErr / ci + kCosErr / cc + 1e-5f;
+        if (ff > 2e-5f * (1.0f + rel)) return true;
+        if (ff < 2e-5f * (1.0f - rel)) return false;
+    }
+    float r, ff;
+    f3 dir;
+    return ff_geom(sp, sn, cp, cn, r, dir, ff);
+}
+
 // 15-bit Morton code (5 bits per axis) of each surfel's cell in the scene
 // box; it only orders work (coherent wavefronts), never enters a result.
 __device__ __forceinline__ uint32_t spread5(uint32_t v) {
