@@ -188,6 +188,7 @@ def test_parameter_setters_match_scene_text():
         a.set_light_emit(l, [float(x) for x in v.split(",")])
     out = []
     for ctx in (a, b):
+        ctx.set_tuning("surfels_gpu", 0)  # gradient rooms are placed on the host (see the fixture)
         ctx.generate_surfels()
         ctx.transfer(8)
         ctx.solve()
@@ -198,6 +199,7 @@ def test_parameter_setters_match_scene_text():
 def test_backward_state_and_material_checks():
     ctx = ia.Context(0)
     ctx.load_scene(scenes.diff_cornell(8, surfels=1000, width=32, height=32))
+    ctx.set_tuning("surfels_gpu", 0)
     ctx.generate_surfels()
     ctx.transfer(8)
     u = np.zeros((32, 32, 3), np.float32)
