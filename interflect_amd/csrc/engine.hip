@@ -183,8 +183,10 @@ __global__ void k_cluster_compact(uint32_t nseg, const uint32_t* valid, const ui
 // pass reads them instead of repeating the culls.  With `key`, the emitting
 // pass also writes the visiting-order sort's input with the rows in Morton
 // order (row i's candidates from mstart[mrank[i]]): key = cluster, value =
-// the candidate's row-major slot, so a stable sort by cluster alone leaves
-// each cluster's candidates in Morton order of their rows.
+// the candidate's row-major slot, or with jbits set the row's Morton rank
+// and the candidate's index in its row packed as (rank << jbits) | index, so
+// a stable sort by cluster alone leaves each cluster's candidates in Morton
+// order of their rows.  cand_row is written when given.
 template <bool EMIT>
 __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, const float4* __restrict__ snrm,
                                                uint32_t row0, uint32_t n, const float4* __restrict__ cpos,
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
                                                const uint32_t* __restrict__ start, uint32_t* cand_ci,
                                                uint32_t* cand_row, const uint32_t* __restrict__ mrank,
                                                const uint32_t* __restrict__ mstart, uint32_t* key, uint32_t* val,
-                                               uint64_t* masks) {
+                                               uint64_t* masks, uint32_t jbits) {
     uint32_t lane = lane_id();
     uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6;
     uint32_t nwaves = (gridDim.x * kBlock) >> 6;
@@ -201,7 +203,8 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
         const uint32_t i = row0 + j;
         f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
         uint32_t base = EMIT ? start[j] : 0u;
-        const uint32_t mbase = (EMIT && key) ? mstart[mrank[i]] : 0u;
+        const uint32_t mr = (EMIT && key) ? mrank[i] : 0u;
+        const uint32_t mbase = (EMIT && key) ? mstart[mr] : 0u;
         uint32_t run = 0;
         for (uint32_t c0 = 0; c0 < C; c0 += 64) {
             uint32_t c = c0 + lane;
@@ -219,10 +222,10 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
                 const uint32_t r = run + lanes_below(m);
                 const uint32_t o = base + r;
                 cand_ci[o] = c;
-                cand_row[o] = i;
+                if (cand_row) cand_row[o] = i;
                 if (key) {
                     key[mbase + r] = c;
-                    val[mbase + r] = o;
+                    val[mbase + r] = jbits ? (mr << jbits) | r : o;
                 }
             }
             run += (uint32_t)__popcll(m);
@@ -309,7 +312,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMarchWa
                                                        const float4* __restrict__ cpos,
                                                        const float4* __restrict__ cnrm,
                                                        const uint32_t* __restrict__ ckey, uint32_t shift,
-                                                       const uint32_t* __restrict__ crow, uint32_t M, float off,
+                                                       const uint32_t* __restrict__ crow, uint32_t rshift, uint32_t M, float off,
                                                        float tmin, float* vis_out, uint32_t* queue, uint32_t chunk,
                                                        uint32_t min_idle, VisTail tl, unsigned long long* evals,
                                                        unsigned long long* ops) {
@@ -336,7 +339,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMarchWa
             live = true;
         } else if (got) {
             slot = q;
-            const uint32_t i = crow[q], c = ckey[q] >> shift;
+            const uint32_t i = crow[q] >> rshift, c = ckey[q] >> shift;
             f3 sp = ld3(spos[i]), sn = ld3(snrm[i]);
             float4 CN = cnrm[c];
             float r = 0.0f;
@@ -396,6 +399,25 @@ __global__ void k_vis_scatter(uint32_t M, const uint32_t* __restrict__ order, co
                               float* cand_v) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < M) cand_v[order[q]] = vis[q];
+}
+
+// The same from packed sort values (Morton rank << jbits | index in the
+// row): the slot is the row's candidate start (mbase, by Morton rank) plus
+// the index.
+__global__ void k_vis_scatter_packed(uint32_t M, const uint32_t* __restrict__ packed, uint32_t jbits,
+                                     const uint32_t* __restrict__ mbase, const float* __restrict__ vis,
+                                     float* cand_v) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= M) return;
+    const uint32_t v = packed[q];
+    cand_v[mbase[v >> jbits] + (v & ((1u << jbits) - 1u))] = vis[q];
+}
+
+// Each Morton rank's row-major candidate start.
+__global__ void k_morton_base(uint32_t N, const uint32_t* __restrict__ rorder, const uint32_t* __restrict__ cstart,
+                              uint32_t* mbase) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < N) mbase[r] = cstart[rorder[r]];
 }
 
 // The parked rays' records in batch order: batch b's cnt[b] records from
@@ -1406,7 +1428,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     launch("ff_count", [&] {
         k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, 0u, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
                                                  d_ccount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr,
-                                                 nullptr, nullptr, nullptr, d_ffmask.as<uint64_t>());
+                                                 nullptr, nullptr, nullptr, d_ffmask.as<uint64_t>(), 0u);
     });
     if (int rc_s = scan_u32(d_ccount.as<uint32_t>(), d_cstart.as<uint32_t>(), (N + 1), err))
         return rc_s;
@@ -1428,6 +1450,12 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
     const uint32_t mbits = 0u;  // the sorted keys hold the cluster alone
     const uint32_t vkey_bits = cbits;
     const bool sort_cand = cbits <= 17u && M > 0;
+    // Sort values packed as (row's Morton rank << jbits) | index in the row
+    // when both fit in 32 bits: the link visibility then reads its row's rank
+    // from the sorted values (no gather pass) and the result scatter finds the
+    // slot from the row's candidate start.  Otherwise the values are slots.
+    const uint32_t jbits0 = std::max(1u, bits_for(C ? C - 1 : 0));
+    const uint32_t jbits = sort_cand && bits_for(N ? N - 1 : 0) + jbits0 <= 32u ? jbits0 : 0u;
     CK(d_cand_ci.ensure(4ull * M));
     CK(d_cand_row.ensure(4ull * M));
     CK(d_cand_v.ensure(4ull * M));
@@ -1447,10 +1475,10 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, 0u, N, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
                                                 nullptr,
                                                 d_cstart.as<uint32_t>(), d_cand_ci.as<uint32_t>(),
-                                                d_cand_row.as<uint32_t>(), d_mrank.as<uint32_t>(),
+                                                jbits ? nullptr : d_cand_row.as<uint32_t>(), d_mrank.as<uint32_t>(),
                                                 d_mstart.as<uint32_t>(), sort_cand ? d_vkey.as<uint32_t>() : nullptr,
                                                 sort_cand ? d_vval.as<uint32_t>() : nullptr,
-                                                d_ffmask.as<uint64_t>());
+                                                d_ffmask.as<uint64_t>(), jbits);
     });
     if (sort_cand) {
         if (int rc_s = sort_u32(d_vkey.as<uint32_t>(), d_vval.as<uint32_t>(), d_vkey_s.as<uint32_t>(),
@@ -1470,7 +1498,8 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
         // the sort's input keys and values are dead: they hold the visiting-order rows and results
         uint32_t* vrow = d_vkey.as<uint32_t>();
         float* vis = sort_cand ? d_vval.as<float>() : d_cand_v.as<float>();
-        if (sort_cand) {
+        const uint32_t* crow = !sort_cand ? d_cand_row.as<uint32_t>() : jbits ? d_vorder.as<uint32_t>() : vrow;
+        if (sort_cand && !jbits) {
             launch("vis_order", [&] {
                 k_vis_order<<<blocks_for(M, 256), 256, 0, stream>>>(M, d_vorder.as<uint32_t>(),
                                                                     d_cand_row.as<uint32_t>(),
@@ -1501,7 +1530,7 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
                       sort_cand ? d_mpos.as<float4>() : spos, sort_cand ? d_mnrm.as<float4>() : snrm,
                       d_cpos.as<float4>(), d_cnrm.as<float4>(),
                       sort_cand ? d_vkey_s.as<uint32_t>() : d_cand_ci.as<uint32_t>(), sort_cand ? mbits : 0u,
-                      sort_cand ? vrow : d_cand_row.as<uint32_t>(), M, diag * 5e-4f, diag * 1e-3f, vis, queue,
+                      crow, jbits, M, diag * 5e-4f, diag * 1e-3f, vis, queue,
                       chunk, min_idle, tl, ctr, tally() ? ctr + 1 : nullptr);
             if (!at) return;
             rc_t = scan_u32(d_tail_cnt.as<uint32_t>(), d_tail_start.as<uint32_t>(), nb + 1, err);
@@ -1516,14 +1545,22 @@ int Engine::transfer(int32_t res, TransferOut& out, std::string& err) {
                       sort_cand ? d_mpos.as<float4>() : spos, sort_cand ? d_mnrm.as<float4>() : snrm,
                       d_cpos.as<float4>(), d_cnrm.as<float4>(),
                       sort_cand ? d_vkey_s.as<uint32_t>() : d_cand_ci.as<uint32_t>(), sort_cand ? mbits : 0u,
-                      sort_cand ? vrow : d_cand_row.as<uint32_t>(), M, diag * 5e-4f, diag * 1e-3f, vis, queue + 2,
+                      crow, jbits, M, diag * 5e-4f, diag * 1e-3f, vis, queue + 2,
                       64u, 64u, rt, ctr, tally() ? ctr + 1 : nullptr);
         });
         if (rc_t) return rc_t;
         if (sort_cand) {
             launch("vis_scatter", [&] {
-                k_vis_scatter<<<blocks_for(M, 256), 256, 0, stream>>>(M, d_vorder.as<uint32_t>(), vis,
-                                                                      d_cand_v.as<float>());
+                if (jbits) {
+                    // the Morton counts are dead after the emission: they hold the starts by rank
+                    k_morton_base<<<blocks_for(N, 256), 256, 0, stream>>>(N, d_rorder.as<uint32_t>(),
+                                                                          d_cstart.as<uint32_t>(),
+                                                                          d_mcount.as<uint32_t>());
+                    k_vis_scatter_packed<<<blocks_for(M, 256), 256, 0, stream>>>(
+                        M, d_vorder.as<uint32_t>(), jbits, d_mcount.as<uint32_t>(), vis, d_cand_v.as<float>());
+                } else
+                    k_vis_scatter<<<blocks_for(M, 256), 256, 0, stream>>>(M, d_vorder.as<uint32_t>(), vis,
+                                                                          d_cand_v.as<float>());
             });
         }
     }
@@ -1866,7 +1903,7 @@ int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool 
     CK(hipMemsetAsync(d_tcount.as<uint32_t>() + nr, 0, 4, stream));
     k_ff<false><<<ffb, kBlock, 0, stream>>>(spos, snrm, r0, nr, d_cpos.as<float4>(), d_cnrm.as<float4>(), C,
                                              d_tcount.as<uint32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr,
-                                             nullptr, nullptr, nullptr);
+                                             nullptr, nullptr, nullptr, 0u);
     if (int rc = scan_u32(d_tcount.as<uint32_t>(), d_tstart.as<uint32_t>(), nr + 1, err)) return rc;
     if (!sized) {
         CK(hipMemcpyAsync(&M, d_tstart.as<uint32_t>() + nr, 4, hipMemcpyDeviceToHost, stream));
@@ -1877,7 +1914,7 @@ int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool 
     }
     k_ff<true><<<ffb, kBlock, 0, stream>>>(spos, snrm, r0, nr, d_cpos.as<float4>(), d_cnrm.as<float4>(), C, nullptr,
                                             d_tstart.as<uint32_t>(), d_tci.as<uint32_t>(), d_trow.as<uint32_t>(),
-                                            nullptr, nullptr, nullptr, nullptr, nullptr);
+                                            nullptr, nullptr, nullptr, nullptr, nullptr, 0u);
     if (M) {
         const uint32_t persist = (uint32_t)n_cu_ * kMarchWaves;  // resident blocks of the march kernels
         const uint32_t visb = vis_blocks ? vis_blocks : std::max(1u, std::min<uint32_t>(persist, blocks_for(M, kBlock)));
@@ -1887,7 +1924,7 @@ int Engine::tile_links(uint32_t r0, uint32_t nr, uint32_t& M, uint32_t& L, bool 
         unsigned long long* ctr = d_counter.as<unsigned long long>();
         CK(hipMemsetAsync(d_tqueue.p, 0, 4, stream));
         LAUNCH_CM(k_visibility, 0, ld.staged, visb, ld.bytes, S, Lg, ld.plan, spos, snrm, d_cpos.as<float4>(),
-                  d_cnrm.as<float4>(), d_tci.as<uint32_t>(), 0u, d_trow.as<uint32_t>(), M, diag * 5e-4f,
+                  d_cnrm.as<float4>(), d_tci.as<uint32_t>(), 0u, d_trow.as<uint32_t>(), 0u, M, diag * 5e-4f,
                   diag * 1e-3f, d_tv.as<float>(), d_tqueue.as<uint32_t>(), wave_chunk(M, visb),
                   std::min(64u, std::max(1u, vis_min_idle)), VisTail{}, ctr + 20, nullptr);
     }
