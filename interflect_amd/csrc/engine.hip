@@ -206,11 +206,22 @@ __global__ __launch_bounds__(kBlock) void k_ff(const float4* __restrict__ spos, 
         const uint32_t mr = (EMIT && key) ? mrank[i] : 0u;
         const uint32_t mbase = (EMIT && key) ? mstart[mr] : 0u;
         uint32_t run = 0;
+        uint32_t mlo = 0, mhi = 0;  // emission: 64 groups' stored ballots, one per lane
         for (uint32_t c0 = 0; c0 < C; c0 += 64) {
             uint32_t c = c0 + lane;
             uint64_t m;
             if (EMIT && masks) {
-                m = masks[(size_t)j * groups + (c0 >> 6)];
+                // the row's ballots are loaded 64 at a time, so the groups do
+                // not each wait on a load of their own
+                const uint32_t g = c0 >> 6;
+                if ((g & 63u) == 0u) {
+                    const uint32_t gl = g + lane;
+                    const uint64_t w = gl < groups ? masks[(size_t)j * groups + gl] : 0ull;
+                    mlo = (uint32_t)w;
+                    mhi = (uint32_t)(w >> 32);
+                }
+                m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)mhi, (int)(g & 63u)) << 32) |
+                    (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)mlo, (int)(g & 63u));
             } else {
                 bool pass = false;
                 if (c < C) pass = ff_pass(sp, sn, cpos[c], cnrm[c]);
